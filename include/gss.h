@@ -1,0 +1,124 @@
+/*
+ * gss.h -- C ABI of the MI355X (gfx950) MSD radix string sort.
+ *
+ * Drop-in boundary for the reference's routine plugin ABI: every entry point
+ * takes plain pointers and sizes, has C linkage and needs no torch types.
+ * Reference paths are relative to the rantala/string-sorting source tree.
+ *
+ * Ordering is the reference's: unsigned bytewise lexicographic order with NUL
+ * as the terminator (strcmp).  The sort is stable -- equal strings keep their
+ * input order -- so the output permutation equals msd_CE2's pointer for
+ * pointer (src/msd_ce.cpp:143-178 is stable) and its bytes equal every other
+ * reference routine's.
+ */
+#ifndef GSS_H
+#define GSS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * Drop-in routines, signature of `struct routine::f` (src/routine.h:32-37).
+ *
+ * gss_msd replaces the msd_CE / msd_ci / msd_A entry points
+ * (src/msd_ce.cpp:176-178, src/msd_ci.cpp:156-177, src/msd_a.cpp:111-113):
+ * it permutes strings[0..n) in place into sorted order.  The strings are
+ * NUL-terminated; they may point into one text buffer (sortstring,
+ * src/sortstring.c:310-335), into separate allocations (unit-test/main.cpp:223)
+ * or overlap (suffix mode, src/sortstring.c:349-358).  String bytes are never
+ * written.  n <= 1 is a no-op.  The ABI has no error channel, so any device
+ * failure prints a message on stderr and calls abort(), as the reference's
+ * routines do on fatal errors (src/msd_ci.cpp:158-164).
+ * The device is initialised lazily on the first call, never at load time.
+ */
+void gss_msd(unsigned char **strings, size_t n);
+
+/* As gss_msd, and also writes the LCP array: lcp[i] = number of equal bytes
+ * of strings[i] and strings[i+1] before the first mismatch or NUL, for
+ * i in [0, n-2] (n-1 entries, none written when n <= 1).  Follows lcp() and
+ * the array convention of src/mergesort_lcp.cpp:75,88-100,222-226; the
+ * reference computes this array inside mergesort_lcp but never returns it. */
+void gss_msd_lcp(unsigned char **strings, size_t n, size_t *lcp);
+
+/* ------------------------------------------------------------------------
+ * Device-resident entry (no reference counterpart: the reference has no
+ * device).  The text and the 32-bit string offsets already sit in device
+ * memory; this is the path the throughput figures are quoted on.
+ *
+ *   d_text     device pointer, 8-byte aligned, text_bytes readable bytes;
+ *              every string starts at an offset < text_bytes and its NUL
+ *              lies inside the text (a string running off the end is cut
+ *              there, as if a NUL followed).
+ *   d_off_in   n string offsets into d_text (any order, may repeat).
+ *   d_off_out  n offsets, receives the sorted order (may equal d_off_in).
+ *   d_lcp      NULL, or n-1 uint64 LCP values as for gss_msd_lcp.
+ *   stream     a hipStream_t (NULL = the default stream).
+ *
+ * Returns 0 on success, a negative GSS_E* code otherwise (gss_last_error()
+ * explains).  The call returns once the result is complete on the stream.
+ */
+int gss_sort_device32(const unsigned char *d_text, uint64_t text_bytes,
+		      const uint32_t *d_off_in, uint32_t *d_off_out,
+		      uint64_t *d_lcp, uint64_t n, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Tuning knobs (no reference counterpart: the reference fixes its
+ * thresholds at compile time, e.g. src/msd_ce.cpp:79 n < 32 and :216
+ * n >= 0x10000).  Values outside the supported range are rejected with
+ * GSS_E_INVALID. */
+typedef struct gss_tuning {
+	uint32_t local_threshold; /* segments of at most this many strings finish
+				     inside one workgroup (64..8192)          */
+	uint32_t radix_bits;      /* digit width of a global MSD pass (8..11)  */
+	uint32_t tile;            /* strings per workgroup in a global pass
+				     (1024..16384, multiple of 1024)           */
+} gss_tuning;
+
+int gss_get_tuning(gss_tuning *out);
+int gss_set_tuning(const gss_tuning *in);
+
+/* ------------------------------------------------------------------------
+ * Per-call statistics of the last sort on this thread's library instance
+ * (no reference counterpart).  active[r] is the number of strings still
+ * unresolved when global MSD pass r began, so the algorithmic bytes of a
+ * pass can be recomputed from it. */
+#define GSS_MAX_PASSES 64
+typedef struct gss_stats {
+	uint64_t n;
+	uint64_t text_bytes;
+	uint32_t global_passes;
+	uint32_t local_segments;
+	uint64_t active[GSS_MAX_PASSES];
+	uint64_t local_strings;  /* strings finished inside one workgroup */
+	uint32_t local_rounds_max;
+} gss_stats;
+
+int gss_get_stats(gss_stats *out);
+
+/* Kernel timing: when enabled, every kernel launch is bracketed by HIP events
+ * recorded on the stream the kernel runs on; totals per kernel accumulate
+ * until reset.  `name` selects the kernel ("gather", "histogram", "scatter",
+ * "local", ...). */
+int gss_profile_enable(int on);
+int gss_profile_reset(void);
+int gss_profile_get(const char *name, uint64_t *launches, double *total_ms,
+		    uint64_t *items);
+
+const char *gss_last_error(void);
+
+#define GSS_OK 0
+#define GSS_E_INVALID (-1)
+#define GSS_E_DEVICE (-2)
+#define GSS_E_NOMEM (-3)
+#define GSS_E_TOO_LARGE (-4)
+#define GSS_E_INTERNAL (-5)
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSS_H */

@@ -1,0 +1,286 @@
+// Internal declarations of the MI355X string sort: device records, device
+// primitives and the launch wrappers shared by the host orchestration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace gss {
+
+constexpr int kWave = 64;
+
+// Largest segment a single workgroup finishes (the local kernel's capacity).
+constexpr uint32_t kLocalCapMax = 8192;
+
+// Global MSD pass geometry: strings per tile = kTileThreads * kTileItems.
+constexpr int kTileThreads = 512;
+constexpr int kTileItems = 8;
+constexpr uint32_t kTile = kTileThreads * kTileItems;
+
+// Retire ranges are split into chunks of at most this many strings so that a
+// huge run of identical strings spreads over many workgroups.
+constexpr uint32_t kRetireChunk = 16384;
+
+// A segment still too large for one workgroup: strings [begin, begin+size)
+// of index buffer `buf`, all equal on their first `depth` bytes.
+struct BigSeg {
+  uint32_t begin;
+  uint32_t size;
+  uint32_t depth;
+  uint32_t buf;
+};
+
+// One workgroup's share of a global pass over a BigSeg.
+struct Tile {
+  uint32_t seg;    // index into the pass's BigSeg list
+  uint32_t begin;  // absolute position of the first string
+  uint32_t len;    // strings in this tile (<= kTile)
+  uint32_t first;  // 1 for the first tile of its segment
+};
+
+// A segment finished inside one workgroup.
+struct LocalSeg {
+  uint32_t begin;
+  uint32_t size;
+  uint32_t depth;
+  uint32_t buf;
+};
+
+// Strings whose order is final but that still need their LCPs written (and,
+// when copy != 0, their offsets copied from index buffer `buf` to the output).
+struct Retire {
+  uint32_t begin;  // first position of this chunk
+  uint32_t len;    // positions in this chunk
+  uint32_t last;   // last position of the whole run (no pair starts there)
+  uint32_t buf;
+  uint32_t lcp;    // LCP of every adjacent pair inside the run
+  uint32_t copy;
+};
+
+// An adjacent pair (pos, pos+1) split by a global pass at `depth`.
+struct Boundary {
+  uint32_t pos;
+  uint32_t depth;
+  uint32_t buf;
+  uint32_t pad;
+};
+
+// Device-side counters, zeroed at the start of every sort.
+struct Counters {
+  uint32_t big_next;     // BigSegs emitted for the next pass
+  uint32_t local[3];     // LocalSegs per size class
+  uint32_t retire;       // Retire chunks
+  uint32_t boundary;     // Boundaries of the current pass
+  uint32_t tile_ticket;  // dynamic tile ids, monotonic over the whole sort
+  uint32_t error;        // nonzero: a bounded wait gave up or a list overflowed
+  uint32_t local_strings[3];  // strings handed to each local size class
+};
+
+// Size classes of the local kernel: (threads, capacity).
+constexpr uint32_t kLocalClassCap[3] = {64, 1024, 8192};
+constexpr uint32_t kLocalClassThreads[3] = {64, 256, 1024};
+
+// Pointers into the workspace for one sort.
+struct Plan {
+  const uint64_t* text64;
+  uint64_t nwords;       // readable 8-byte words of the text
+  uint32_t* idx[3];      // index buffers: [0] = input, [1], [2] ping-pong
+  uint32_t* out;         // sorted offsets
+  uint64_t* lcp;         // NULL or n-1 LCPs
+  uint64_t* keys;        // n cached keys (global passes)
+  BigSeg* big[2];        // BigSeg lists (current, next)
+  uint64_t* segmm;       // per BigSeg: min, max key
+  uint32_t* seghist;     // per BigSeg: 2^radix_bits digit counts
+  uint32_t* bstart;      // per BigSeg: digit bucket starts | final flag
+  Tile* tiles;
+  uint64_t* status;      // look-back words, tiles * 2^radix_bits
+  LocalSeg* local[3];
+  Retire* retire;
+  Boundary* bounds;
+  Counters* ctr;
+  uint32_t local_threshold;
+  uint32_t cap_big, cap_retire, cap_bounds, cap_tiles;
+  uint32_t cap_local[3];
+};
+
+// ---------------------------------------------------------------------------
+// Device primitives
+
+// 8 bytes of the text starting at byte `pos`, big-endian (first byte most
+// significant), zero after the first NUL; words beyond the text read as 0.
+// Same packing as fill_cache in src/multikey_cache.cpp:107-125.
+__device__ __forceinline__ uint64_t load_key(const uint64_t* __restrict__ text64,
+                                             uint64_t nwords, uint64_t pos) {
+  const uint64_t w = pos >> 3;
+  const uint32_t sh = (uint32_t)(pos & 7) * 8;
+  const uint64_t lo = w < nwords ? text64[w] : 0;
+  uint64_t v = lo;
+  if (sh) {
+    const uint64_t hi = (w + 1 < nwords) ? text64[w + 1] : 0;
+    v = (lo >> sh) | (hi << (64 - sh));
+  }
+  // Little-endian v: byte i is text[pos+i].  The lowest flagged byte of the
+  // zero-byte test is the first NUL (false positives only sit above it).
+  const uint64_t z = (v - 0x0101010101010101ull) & ~v & 0x8080808080808080ull;
+  if (z) {
+    const uint32_t zb = (uint32_t)__builtin_ctzll(z) >> 3;
+    v = zb ? (v & (~0ull >> (64 - 8 * zb))) : 0;
+  }
+  return __builtin_bswap64(v);
+}
+
+// Number of string bytes before the NUL in a key whose low byte is zero.
+__device__ __forceinline__ uint32_t key_len(uint64_t k) {
+  return k ? 8u - ((uint32_t)__builtin_ctzll(k) >> 3) : 0u;
+}
+
+// Leading equal bytes of two different keys.
+__device__ __forceinline__ uint32_t key_lcp(uint64_t a, uint64_t b) {
+  return (uint32_t)__builtin_clzll(a ^ b) >> 3;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  return (1ull << lane_id()) - 1ull;
+}
+
+// Lanes of the wave whose `bits`-bit digit equals this lane's, among `live`.
+template <int BITS>
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t live) {
+  uint64_t m = live;
+#pragma unroll
+  for (int b = 0; b < BITS; ++b) {
+    const uint64_t bal = __ballot((d >> b) & 1u);
+    m &= ((d >> b) & 1u) ? bal : ~bal;
+  }
+  return m;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o, kWave);
+    if ((int)lane_id() >= o) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint64_t wave_or(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, kWave);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o, kWave));
+  return v;
+}
+
+__device__ __forceinline__ uint64_t wave_min64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t t = __shfl_xor(v, o, kWave);
+    v = t < v ? t : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint64_t wave_max64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t t = __shfl_xor(v, o, kWave);
+    v = t > v ? t : v;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan; `scratch` holds BLOCK/64 + 1 words.  Every
+// thread of the block must call it.  Returns the exclusive prefix and the
+// block total in *total.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* scratch,
+                                                    uint32_t* total) {
+  constexpr int W = BLOCK / kWave;
+  const uint32_t inc = wave_incl_scan(v);
+  const int wid = threadIdx.x / kWave;
+  if (W == 1) {
+    *total = __shfl(inc, kWave - 1, kWave);
+    return inc - v;
+  }
+  if ((int)lane_id() == kWave - 1) scratch[wid] = inc;
+  __syncthreads();
+  if (wid == 0) {
+    uint32_t t = (int)lane_id() < W ? scratch[lane_id()] : 0u;
+    const uint32_t ti = wave_incl_scan(t);
+    if ((int)lane_id() < W) scratch[lane_id()] = ti - t;
+    if ((int)lane_id() == W - 1) scratch[W] = ti;
+  }
+  __syncthreads();
+  const uint32_t r = scratch[wid] + inc - v;
+  *total = scratch[W];
+  __syncthreads();
+  return r;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ uint64_t block_or(uint64_t v, uint64_t* scratch) {
+  constexpr int W = BLOCK / kWave;
+  v = wave_or(v);
+  if (W == 1) return v;
+  const int wid = threadIdx.x / kWave;
+  if (lane_id() == 0) scratch[wid] = v;
+  __syncthreads();
+  uint64_t r = 0;
+#pragma unroll
+  for (int i = 0; i < W; ++i) r |= scratch[i];
+  __syncthreads();
+  return r;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_max(uint32_t v, uint32_t* scratch) {
+  constexpr int W = BLOCK / kWave;
+  v = wave_max(v);
+  if (W == 1) return v;
+  const int wid = threadIdx.x / kWave;
+  if (lane_id() == 0) scratch[wid] = v;
+  __syncthreads();
+  uint32_t r = 0;
+#pragma unroll
+  for (int i = 0; i < W; ++i) r = max(r, scratch[i]);
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers (kernels.hip).  All return hipError_t of the launch.
+
+hipError_t launch_init_segs(const Plan& p, uint32_t nsegs, uint32_t nbins,
+                            hipStream_t s);
+hipError_t launch_gather_minmax(const Plan& p, const BigSeg* segs, uint32_t ntiles,
+                                hipStream_t s);
+hipError_t launch_histogram(const Plan& p, const BigSeg* segs, uint32_t ntiles,
+                            uint32_t radix_bits, hipStream_t s);
+hipError_t launch_segscan(const Plan& p, const BigSeg* segs, BigSeg* next,
+                          uint32_t nsegs, uint32_t pass, uint32_t radix_bits,
+                          hipStream_t s);
+hipError_t launch_scatter(const Plan& p, const BigSeg* segs, uint32_t ntiles,
+                          uint32_t pass, uint32_t epoch, uint32_t radix_bits,
+                          hipStream_t s);
+hipError_t launch_boundary_lcp(const Plan& p, uint32_t first, hipStream_t s);
+hipError_t launch_retire(const Plan& p, uint32_t nretire, hipStream_t s);
+hipError_t launch_local(const Plan& p, int cls, uint32_t nsegs, hipStream_t s);
+hipError_t launch_zero_counters(const Plan& p, hipStream_t s);
+hipError_t launch_fill_u64(uint64_t* dst, uint64_t n, uint64_t v, hipStream_t s);
+hipError_t launch_ptr_to_off(const uint64_t* ptrs, uint64_t base, uint32_t* off,
+                             uint64_t n, hipStream_t s);
+hipError_t launch_off_to_ptr(const uint32_t* off, uint64_t base, uint64_t* ptrs,
+                             uint64_t n, hipStream_t s);
+hipError_t launch_off_to_orig_ptr(const uint32_t* off, const uint32_t* starts,
+                                  const uint64_t* orig, uint64_t* ptrs, uint64_t n,
+                                  hipStream_t s);
+
+}  // namespace gss

@@ -1,0 +1,439 @@
+// Device-resident MSD radix string sort: workspace, pass loop, statistics,
+// profiling and the C entry points of gss.h that do not touch host strings.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gss.h"
+#include "gss_internal.h"
+#include "host.h"
+
+namespace gss {
+
+namespace {
+
+constexpr int kMaxDevices = 16;
+
+struct Workspace {
+  char* base = nullptr;
+  size_t bytes = 0;
+  uint64_t n_cap = 0;
+  uint32_t rb = 0;
+  uint32_t threshold = 0;
+  Plan plan{};
+  uint32_t* idx_copy = nullptr;
+  uint32_t cap_local[3] = {0, 0, 0};
+  // Pinned host mirrors.
+  Counters* h_ctr = nullptr;
+  BigSeg* h_big = nullptr;
+  Tile* h_tiles = nullptr;
+  LocalSeg* h_local = nullptr;
+};
+
+struct DeviceCtx {
+  bool ready = false;
+  hipStream_t stream = nullptr;
+  Workspace ws;
+  uint32_t epoch = 0;
+};
+
+struct ProfTotal {
+  uint64_t launches = 0;
+  double ms = 0;
+  uint64_t items = 0;
+};
+
+struct Pending {
+  hipEvent_t a, b;
+  std::string name;
+  uint64_t items;
+};
+
+std::recursive_mutex g_mu;
+DeviceCtx g_dev[kMaxDevices];
+gss_tuning g_tuning = {8192u, 8u, kTile};
+gss_stats g_stats;
+bool g_prof = false;
+std::map<std::string, ProfTotal> g_prof_totals;
+std::vector<Pending> g_pending;
+std::vector<hipEvent_t> g_event_pool;
+thread_local std::string g_last_error;
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+#define GSS_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(GSS_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int current_ctx(DeviceCtx** out) {
+  int dev = 0;
+  GSS_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(GSS_E_DEVICE, "device id out of range");
+  DeviceCtx& c = g_dev[dev];
+  if (!c.ready) {
+    GSS_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    GSS_HIP(hipHostMalloc((void**)&c.ws.h_ctr, sizeof(Counters), hipHostMallocDefault));
+    c.ready = true;
+  }
+  *out = &c;
+  return GSS_OK;
+}
+
+hipEvent_t take_event() {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void free_workspace(Workspace& w) {
+  if (w.base) (void)hipFree(w.base);
+  if (w.h_big) (void)hipHostFree(w.h_big);
+  if (w.h_tiles) (void)hipHostFree(w.h_tiles);
+  if (w.h_local) (void)hipHostFree(w.h_local);
+  w.base = nullptr;
+  w.h_big = nullptr;
+  w.h_tiles = nullptr;
+  w.h_local = nullptr;
+  w.bytes = 0;
+  w.n_cap = 0;
+}
+
+// Sizes every device array for up to n strings, 2^rb digits and local
+// threshold t, and carves them out of one allocation.
+int ensure_workspace(DeviceCtx& c, uint64_t n, uint32_t rb, uint32_t t) {
+  Workspace& w = c.ws;
+  if (w.base && n <= w.n_cap && rb == w.rb && t == w.threshold) return GSS_OK;
+  GSS_HIP(hipStreamSynchronize(c.stream));
+  uint64_t ncap = std::max<uint64_t>(n, w.n_cap);
+  if (ncap < 4096) ncap = 4096;
+  free_workspace(w);
+  const uint64_t nb = 1ull << rb;
+  const uint64_t cap_big = ncap / (t + 1) + 2;
+  const uint64_t cap_tiles = ncap / kTile + cap_big + 2;
+  const uint64_t cap_local[3] = {ncap / 2 + 2, ncap / (kLocalClassCap[0] + 1) + 2,
+                                 ncap / (kLocalClassCap[1] + 1) + 2};
+  const uint64_t cap_retire = ncap / 2 + ncap / kRetireChunk + 8;
+  const uint64_t cap_bounds = ncap + 2;
+
+  struct Item {
+    void** dst;
+    size_t bytes;
+  };
+  Plan& p = w.plan;
+  p = Plan{};
+  std::vector<Item> items = {
+      {(void**)&p.idx[1], 4 * ncap},
+      {(void**)&p.idx[2], 4 * ncap},
+      {(void**)&w.idx_copy, 4 * ncap},
+      {(void**)&p.keys, 8 * ncap},
+      {(void**)&p.big[0], sizeof(BigSeg) * cap_big},
+      {(void**)&p.big[1], sizeof(BigSeg) * cap_big},
+      {(void**)&p.segmm, 16 * cap_big},
+      {(void**)&p.seghist, 4 * cap_big * nb},
+      {(void**)&p.bstart, 4 * cap_big * nb},
+      {(void**)&p.tiles, sizeof(Tile) * cap_tiles},
+      {(void**)&p.status, 8 * cap_tiles * nb},
+      {(void**)&p.local[0], sizeof(LocalSeg) * cap_local[0]},
+      {(void**)&p.local[1], sizeof(LocalSeg) * cap_local[1]},
+      {(void**)&p.local[2], sizeof(LocalSeg) * cap_local[2]},
+      {(void**)&p.retire, sizeof(Retire) * cap_retire},
+      {(void**)&p.bounds, sizeof(Boundary) * cap_bounds},
+      {(void**)&p.ctr, sizeof(Counters)},
+  };
+  size_t total = 0;
+  for (auto& it : items) total += align_up(it.bytes, 256);
+  char* base = nullptr;
+  if (hipMalloc((void**)&base, total) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(GSS_E_NOMEM, "device workspace allocation of " + std::to_string(total) +
+                                 " bytes failed");
+  }
+  size_t off = 0;
+  for (auto& it : items) {
+    *it.dst = base + off;
+    off += align_up(it.bytes, 256);
+  }
+  w.base = base;
+  w.bytes = total;
+  w.n_cap = ncap;
+  w.rb = rb;
+  w.threshold = t;
+  p.cap_big = (uint32_t)cap_big;
+  p.cap_tiles = (uint32_t)cap_tiles;
+  p.cap_retire = (uint32_t)cap_retire;
+  p.cap_bounds = (uint32_t)cap_bounds;
+  for (int i = 0; i < 3; ++i) w.cap_local[i] = p.cap_local[i] = (uint32_t)cap_local[i];
+  GSS_HIP(hipHostMalloc((void**)&w.h_big, sizeof(BigSeg) * cap_big, hipHostMallocDefault));
+  GSS_HIP(hipHostMalloc((void**)&w.h_tiles, sizeof(Tile) * cap_tiles, hipHostMallocDefault));
+  GSS_HIP(hipHostMalloc((void**)&w.h_local, sizeof(LocalSeg) * 4, hipHostMallocDefault));
+  // Look-back words carry a pass epoch; start from a clean array.
+  GSS_HIP(launch_fill_u64(p.status, cap_tiles * nb, 0, c.stream));
+  GSS_HIP(hipStreamSynchronize(c.stream));
+  c.epoch = 0;
+  return GSS_OK;
+}
+
+int local_class(uint64_t size) {
+  return size <= kLocalClassCap[0] ? 0 : (size <= kLocalClassCap[1] ? 1 : 2);
+}
+
+}  // namespace
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int library_stream(hipStream_t* out) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  DeviceCtx* c = nullptr;
+  int rc = current_ctx(&c);
+  if (rc) return rc;
+  *out = c->stream;
+  return GSS_OK;
+}
+
+ProfToken prof_begin(const char* name, uint64_t items, hipStream_t stream) {
+  ProfToken t;
+  if (!g_prof) return t;
+  t.a = take_event();
+  t.name = name;
+  t.items = items;
+  if (t.a) (void)hipEventRecord(t.a, stream);
+  return t;
+}
+
+void prof_end(ProfToken& t, hipStream_t stream) {
+  if (!g_prof || !t.a) return;
+  hipEvent_t b = take_event();
+  if (!b) return;
+  (void)hipEventRecord(b, stream);
+  g_pending.push_back(Pending{t.a, b, t.name, t.items});
+}
+
+void prof_collect() {
+  for (auto& p : g_pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      ProfTotal& tot = g_prof_totals[p.name];
+      tot.launches += 1;
+      tot.ms += ms;
+      tot.items += p.items;
+    }
+    g_event_pool.push_back(p.a);
+    g_event_pool.push_back(p.b);
+  }
+  g_pending.clear();
+}
+
+#define GSS_LAUNCH(name, items, call)            \
+  do {                                           \
+    ProfToken tok_ = prof_begin(name, items, st); \
+    GSS_HIP(call);                               \
+    prof_end(tok_, st);                          \
+  } while (0)
+
+int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
+                  uint32_t* out, uint64_t* lcp, uint64_t n, hipStream_t stream) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  memset(&g_stats, 0, sizeof(g_stats));
+  g_stats.n = n;
+  g_stats.text_bytes = text_bytes;
+  if (n == 0) return GSS_OK;
+  if (!text || !in || !out) return fail(GSS_E_INVALID, "NULL text, input or output pointer");
+  if (reinterpret_cast<uintptr_t>(text) & 7)
+    return fail(GSS_E_INVALID, "the device text must be 8-byte aligned");
+  if (n >= (1ull << 31))
+    return fail(GSS_E_TOO_LARGE, "at most 2^31-1 strings per sort on the 32-bit path");
+  DeviceCtx* c = nullptr;
+  int rc = current_ctx(&c);
+  if (rc) return rc;
+  hipStream_t st = stream ? stream : c->stream;
+  if (n == 1) {
+    GSS_HIP(hipMemcpyAsync(out, in, 4, hipMemcpyDeviceToDevice, st));
+    GSS_HIP(hipStreamSynchronize(st));
+    return GSS_OK;
+  }
+  const uint32_t rb = g_tuning.radix_bits;
+  const uint32_t T = g_tuning.local_threshold;
+  rc = ensure_workspace(*c, n, rb, T);
+  if (rc) return rc;
+  Workspace& w = c->ws;
+  Plan p = w.plan;
+  p.text64 = reinterpret_cast<const uint64_t*>(text);
+  p.nwords = (text_bytes + 7) / 8;
+  p.out = out;
+  p.lcp = lcp;
+  p.local_threshold = T;
+  p.idx[0] = const_cast<uint32_t*>(in);
+  if (out == in) {
+    GSS_HIP(hipMemcpyAsync(w.idx_copy, in, 4 * n, hipMemcpyDeviceToDevice, st));
+    p.idx[0] = w.idx_copy;
+  }
+  GSS_LAUNCH("init", 0, launch_zero_counters(p, st));
+  const uint32_t nb = 1u << rb;
+
+  uint32_t nlocal[3] = {0, 0, 0};
+  uint32_t local_strings[3] = {0, 0, 0};
+  uint32_t nretire = 0;
+  if (n <= T) {
+    const int cls = local_class(n);
+    w.h_local[0] = LocalSeg{0, (uint32_t)n, 0, 0};
+    GSS_HIP(hipMemcpyAsync(p.local[cls], w.h_local, sizeof(LocalSeg), hipMemcpyHostToDevice, st));
+    nlocal[cls] = 1;
+    local_strings[cls] = (uint32_t)n;
+  } else {
+    w.h_big[0] = BigSeg{0, (uint32_t)n, 0, 0};
+    GSS_HIP(hipMemcpyAsync(p.big[0], w.h_big, sizeof(BigSeg), hipMemcpyHostToDevice, st));
+    uint32_t nbig = 1, cur = 0, ticket = 0, bound_first = 0, pass = 0;
+    // Each pass either splits a segment or moves it >= 1 byte deeper, and no
+    // string reaches past the text, so the pass count is bounded.
+    const uint64_t max_passes = p.nwords * 8 + 64;
+    while (nbig) {
+      if (pass > max_passes) return fail(GSS_E_INTERNAL, "global passes do not terminate");
+      uint32_t ntiles = 0;
+      uint64_t active = 0;
+      for (uint32_t i = 0; i < nbig; ++i) {
+        const BigSeg& sg = w.h_big[i];
+        active += sg.size;
+        for (uint32_t o = 0; o < sg.size; o += kTile) {
+          if (ntiles >= p.cap_tiles) return fail(GSS_E_INTERNAL, "tile table overflow");
+          w.h_tiles[ntiles++] = Tile{i, sg.begin + o, std::min<uint32_t>(kTile, sg.size - o),
+                                     o == 0 ? 1u : 0u};
+        }
+      }
+      if (pass < GSS_MAX_PASSES) g_stats.active[pass] = active;
+      GSS_HIP(hipMemcpyAsync(p.tiles, w.h_tiles, sizeof(Tile) * ntiles, hipMemcpyHostToDevice,
+                             st));
+      if (c->epoch == 65535u) {  // epochs wrap: clear stale look-back words
+        GSS_HIP(launch_fill_u64(p.status, (uint64_t)p.cap_tiles * nb, 0, st));
+        c->epoch = 0;
+      }
+      c->epoch += 1;
+      const BigSeg* segs = p.big[cur];
+      BigSeg* next = p.big[cur ^ 1];
+      GSS_LAUNCH("init_segs", nbig, launch_init_segs(p, nbig, nb, st));
+      GSS_LAUNCH("gather", active, launch_gather_minmax(p, segs, ntiles, st));
+      GSS_LAUNCH("histogram", active, launch_histogram(p, segs, ntiles, rb, st));
+      GSS_HIP(hipMemsetAsync(&p.ctr->big_next, 0, sizeof(uint32_t), st));
+      GSS_LAUNCH("segscan", nbig, launch_segscan(p, segs, next, nbig, pass, rb, st));
+      GSS_LAUNCH("scatter", active, launch_scatter(p, segs, ntiles, ticket, c->epoch, rb, st));
+      ticket += ntiles;
+      if (lcp) GSS_LAUNCH("boundary", 0, launch_boundary_lcp(p, bound_first, st));
+      GSS_HIP(hipMemcpyAsync(w.h_ctr, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
+      GSS_HIP(hipMemcpyAsync(w.h_big, next, sizeof(BigSeg) * p.cap_big, hipMemcpyDeviceToHost,
+                             st));
+      GSS_HIP(hipStreamSynchronize(st));
+      if (w.h_ctr->error)
+        return fail(GSS_E_INTERNAL, "device pass failed (code " +
+                                        std::to_string(w.h_ctr->error) + ")");
+      nbig = w.h_ctr->big_next;
+      bound_first = w.h_ctr->boundary;
+      cur ^= 1;
+      ++pass;
+    }
+    g_stats.global_passes = pass;
+    for (int i = 0; i < 3; ++i) {
+      nlocal[i] = w.h_ctr->local[i];
+      local_strings[i] = w.h_ctr->local_strings[i];
+    }
+    nretire = w.h_ctr->retire;
+  }
+  if (nretire) GSS_LAUNCH("retire", nretire, launch_retire(p, nretire, st));
+  static const char* kLocalNames[3] = {"local64", "local1024", "local8192"};
+  for (int cls = 2; cls >= 0; --cls) {
+    if (nlocal[cls] > w.cap_local[cls])
+      return fail(GSS_E_INTERNAL, "local segment list overflow");
+    if (nlocal[cls])
+      GSS_LAUNCH(kLocalNames[cls], local_strings[cls], launch_local(p, cls, nlocal[cls], st));
+    g_stats.local_segments += nlocal[cls];
+    g_stats.local_strings += local_strings[cls];
+  }
+  GSS_HIP(hipMemcpyAsync(w.h_ctr, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
+  GSS_HIP(hipStreamSynchronize(st));
+  prof_collect();
+  if (w.h_ctr->error)
+    return fail(GSS_E_INTERNAL,
+                "device sort failed (code " + std::to_string(w.h_ctr->error) + ")");
+  return GSS_OK;
+}
+
+}  // namespace gss
+
+using namespace gss;
+
+extern "C" {
+
+int gss_sort_device32(const unsigned char* d_text, uint64_t text_bytes, const uint32_t* d_off_in,
+                      uint32_t* d_off_out, uint64_t* d_lcp, uint64_t n, void* stream) {
+  return sort_device32(d_text, text_bytes, d_off_in, d_off_out, d_lcp, n,
+                       reinterpret_cast<hipStream_t>(stream));
+}
+
+int gss_get_tuning(gss_tuning* out) {
+  if (!out) return fail(GSS_E_INVALID, "NULL tuning pointer");
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  *out = g_tuning;
+  return GSS_OK;
+}
+
+int gss_set_tuning(const gss_tuning* in) {
+  if (!in) return fail(GSS_E_INVALID, "NULL tuning pointer");
+  if (in->local_threshold < 64 || in->local_threshold > kLocalCapMax)
+    return fail(GSS_E_INVALID, "local_threshold must lie in [64, 8192]");
+  if (in->radix_bits != 8 && in->radix_bits != 11)
+    return fail(GSS_E_INVALID, "radix_bits must be 8 or 11");
+  if (in->tile != kTile) return fail(GSS_E_INVALID, "tile must be 4096");
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  g_tuning = *in;
+  return GSS_OK;
+}
+
+int gss_get_stats(gss_stats* out) {
+  if (!out) return fail(GSS_E_INVALID, "NULL stats pointer");
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  *out = g_stats;
+  return GSS_OK;
+}
+
+int gss_profile_enable(int on) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  g_prof = on != 0;
+  return GSS_OK;
+}
+
+int gss_profile_reset(void) {
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  g_prof_totals.clear();
+  return GSS_OK;
+}
+
+int gss_profile_get(const char* name, uint64_t* launches, double* total_ms, uint64_t* items) {
+  if (!name) return fail(GSS_E_INVALID, "NULL kernel name");
+  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  auto it = g_prof_totals.find(name);
+  ProfTotal t;
+  if (it != g_prof_totals.end()) t = it->second;
+  if (launches) *launches = t.launches;
+  if (total_ms) *total_ms = t.ms;
+  if (items) *items = t.items;
+  return GSS_OK;
+}
+
+const char* gss_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

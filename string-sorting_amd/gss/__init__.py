@@ -1,0 +1,246 @@
+"""Python binding of the MI355X string sort (libgss.so, C ABI in include/gss.h).
+
+Mirrors the reference's routine registry (src/routine.h:32-59,
+src/routines.c:31-75): each registered routine has a name, a description, a
+multicore flag and ``f(strings)``, which sorts a list of byte strings through
+the routine's ``void f(unsigned char**, size_t)`` entry point.  Device-resident
+sorting of torch tensors goes through :func:`sort_device`.
+
+The HIP library is required: importing works without it, but every call that
+needs it raises :class:`GssError` naming the missing file.  There is no CPU
+fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libgss.so"
+
+GSS_OK = 0
+GSS_MAX_PASSES = 64
+
+
+class GssError(RuntimeError):
+    pass
+
+
+class Tuning(C.Structure):
+    _fields_ = [("local_threshold", C.c_uint32), ("radix_bits", C.c_uint32), ("tile", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint64),
+        ("text_bytes", C.c_uint64),
+        ("global_passes", C.c_uint32),
+        ("local_segments", C.c_uint32),
+        ("active", C.c_uint64 * GSS_MAX_PASSES),
+        ("local_strings", C.c_uint64),
+        ("local_rounds_max", C.c_uint32),
+    ]
+
+
+# Every symbol include/gss.h declares, with its ctypes signature.
+_SIGNATURES = {
+    "gss_msd": (None, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "gss_msd_lcp": (None, [C.POINTER(C.c_void_p), C.c_size_t, C.POINTER(C.c_size_t)]),
+    "gss_sort_device32": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_uint64, C.c_void_p]),
+    "gss_get_tuning": (C.c_int, [C.POINTER(Tuning)]),
+    "gss_set_tuning": (C.c_int, [C.POINTER(Tuning)]),
+    "gss_get_stats": (C.c_int, [C.POINTER(Stats)]),
+    "gss_profile_enable": (C.c_int, [C.c_int]),
+    "gss_profile_reset": (C.c_int, []),
+    "gss_profile_get": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_uint64)]),
+    "gss_last_error": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads libgss.so (built by ``make -C string-sorting_amd``)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise GssError(f"HIP library not built: {LIB_PATH} is missing "
+                           "(run __graft_entry__.build() or make -C string-sorting_amd)")
+        l = C.CDLL(str(LIB_PATH), mode=C.RTLD_GLOBAL)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def exported_symbols() -> list:
+    return sorted(_SIGNATURES)
+
+
+def last_error() -> str:
+    return (lib().gss_last_error() or b"").decode()
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != GSS_OK:
+        raise GssError(f"{what} failed ({rc}): {last_error()}")
+
+
+# ---------------------------------------------------------------------------
+# Host pointer path (the routine.h drop-in)
+
+def _pointer_array(bufs):
+    arr = (C.c_void_p * max(1, len(bufs)))()
+    for i, b in enumerate(bufs):
+        arr[i] = C.addressof(b)
+    return arr
+
+
+def sort_pointers(ptrs, n: int, lcp: bool = False):
+    """Calls gss_msd / gss_msd_lcp on a ctypes array of n char pointers."""
+    if lcp:
+        out = (C.c_size_t * max(1, n - 1))()
+        lib().gss_msd_lcp(C.cast(ptrs, C.POINTER(C.c_void_p)), n, out)
+        return list(out[: max(0, n - 1)])
+    lib().gss_msd(C.cast(ptrs, C.POINTER(C.c_void_p)), n)
+    return None
+
+
+def sort_strings(strs, lcp: bool = False):
+    """Sorts byte strings through the pointer entry point; each string is a
+    separate allocation, as in the reference's unit test (strdup).
+    Returns the sorted strings (and the LCP list when lcp=True)."""
+    bufs = [C.create_string_buffer(bytes(s)) for s in strs]
+    ptrs = _pointer_array(bufs)
+    n = len(bufs)
+    l = sort_pointers(ptrs, n, lcp)
+    addr = {C.addressof(b): i for i, b in enumerate(bufs)}
+    order = [addr[ptrs[i]] for i in range(n)]
+    out = [bytes(strs[i]) for i in order]
+    return (out, l) if lcp else out
+
+
+# ---------------------------------------------------------------------------
+# Device-resident path
+
+def sort_device(text, offsets, out=None, lcp=None, text_bytes=None, stream=None):
+    """Sorts the strings at uint32 `offsets` into uint8 `text`, both torch
+    tensors on the GPU.  Writes the sorted offsets to `out` (allocated when
+    None) and, when `lcp` is an int64/uint64 tensor of n-1 entries, the LCPs.
+    Returns `out`."""
+    import torch
+
+    if not (text.is_cuda and offsets.is_cuda):
+        raise GssError("text and offsets must be device tensors")
+    if offsets.dtype not in (torch.int32, torch.uint32):
+        raise GssError("offsets must be 32-bit")
+    n = offsets.numel()
+    if out is None:
+        out = torch.empty_like(offsets)
+    if text_bytes is None:
+        text_bytes = text.numel()
+    if stream is None:
+        stream = torch.cuda.current_stream(text.device)
+    lp = 0
+    if lcp is not None:
+        if lcp.numel() < max(0, n - 1) or lcp.element_size() != 8:
+            raise GssError("lcp must hold n-1 64-bit entries")
+        lp = lcp.data_ptr()
+    rc = lib().gss_sort_device32(text.data_ptr(), int(text_bytes), offsets.data_ptr(),
+                                 out.data_ptr(), lp if lp else None, n, stream.cuda_stream)
+    _check(rc, "gss_sort_device32")
+    return out
+
+
+def get_tuning() -> Tuning:
+    t = Tuning()
+    _check(lib().gss_get_tuning(C.byref(t)), "gss_get_tuning")
+    return t
+
+
+def set_tuning(local_threshold=None, radix_bits=None, tile=None) -> Tuning:
+    t = get_tuning()
+    if local_threshold is not None:
+        t.local_threshold = local_threshold
+    if radix_bits is not None:
+        t.radix_bits = radix_bits
+    if tile is not None:
+        t.tile = tile
+    _check(lib().gss_set_tuning(C.byref(t)), "gss_set_tuning")
+    return t
+
+
+def stats() -> Stats:
+    s = Stats()
+    _check(lib().gss_get_stats(C.byref(s)), "gss_get_stats")
+    return s
+
+
+def profile_enable(on: bool = True) -> None:
+    _check(lib().gss_profile_enable(1 if on else 0), "gss_profile_enable")
+
+
+def profile_reset() -> None:
+    _check(lib().gss_profile_reset(), "gss_profile_reset")
+
+
+def profile_get(name: str):
+    """(launches, total_ms, items) accumulated for kernel `name`."""
+    launches, ms, items = C.c_uint64(), C.c_double(), C.c_uint64()
+    _check(lib().gss_profile_get(name.encode(), C.byref(launches), C.byref(ms), C.byref(items)),
+           "gss_profile_get")
+    return launches.value, ms.value, items.value
+
+
+# ---------------------------------------------------------------------------
+# Routine registry, mirroring src/routine.h:32-59 and src/routines.c:31-75.
+
+@dataclass(frozen=True)
+class Routine:
+    name: str
+    desc: str
+    multicore: bool
+    lcp: bool
+
+    def f(self, strs):
+        """Sorts `strs` (bytes) through this routine's C entry point."""
+        out = sort_strings(strs, lcp=self.lcp)
+        return out[0] if self.lcp else out
+
+
+ROUTINES_MAX = 256
+_routines = []
+
+
+def routine_register(r: Routine) -> None:
+    """routines.c:31-43: rejects missing fields and registry overflow."""
+    if r is None or not r.name or not r.desc:
+        raise GssError("routine_register: NULL routine, name or description")
+    if len(_routines) >= ROUTINES_MAX:
+        raise GssError("routine_register: registry full")
+    _routines.append(r)
+
+
+def routine_from_name(name: str):
+    """routines.c:45-53: first routine with this name, or None."""
+    for r in _routines:
+        if r.name == name:
+            return r
+    return None
+
+
+def routine_get_all():
+    """routines.c:55-75: sorted by (multicore, name)."""
+    return sorted(_routines, key=lambda r: (r.multicore, r.name))
+
+
+routine_register(Routine("gss_msd", "MI355X MSD radix sort, 8-byte keys (HIP, gfx950)", True, False))
+routine_register(Routine("gss_msd_with_lcp", "MI355X MSD radix sort + LCP array (HIP, gfx950)",
+                         True, True))

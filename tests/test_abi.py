@@ -1,0 +1,92 @@
+"""The C ABI library without a GPU: it loads, exports every symbol
+include/gss.h declares, validates arguments, and does not depend on the
+oracle.  CPU only: no call here reaches a device."""
+
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import gss
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "gss.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gss_\w+)\s*\(", text)))
+
+
+def test_library_loads():
+    assert gss.LIB_PATH.exists(), "build() must produce libgss.so"
+    gss.lib()
+
+
+def test_every_declared_symbol_is_exported():
+    decl = declared_functions()
+    assert decl, "no declarations parsed"
+    lib = gss.lib()
+    for name in decl:
+        assert hasattr(lib, name), name
+    assert sorted(gss.exported_symbols()) == decl
+    out = subprocess.run(["nm", "-D", "--defined-only", str(gss.LIB_PATH)], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (gss_\w+)", out))
+    assert set(decl) <= exported
+
+
+def test_library_does_not_link_the_oracle():
+    out = subprocess.run(["readelf", "-d", str(gss.LIB_PATH)], capture_output=True, text=True,
+                         check=True).stdout
+    assert "oracle" not in out and "libref" not in out
+    for src in (ROOT / "string-sorting_amd").rglob("*"):
+        if src.suffix in (".hip", ".cpp", ".h", ".c", ".py") and "oracle" in src.read_text():
+            body = src.read_text()
+            assert "import oracle" not in body and "oracle.h" not in body, src
+
+
+def test_tuning_validation():
+    t = gss.get_tuning()
+    try:
+        with pytest.raises(gss.GssError):
+            gss.set_tuning(local_threshold=10)
+        with pytest.raises(gss.GssError):
+            gss.set_tuning(local_threshold=20000)
+        with pytest.raises(gss.GssError):
+            gss.set_tuning(radix_bits=9)
+        gss.set_tuning(local_threshold=1024, radix_bits=11)
+        assert gss.get_tuning().radix_bits == 11
+    finally:
+        gss.set_tuning(t.local_threshold, t.radix_bits, t.tile)
+
+
+def test_device_entry_rejects_host_tensors():
+    torch = pytest.importorskip("torch")
+    text = torch.zeros(16, dtype=torch.uint8)
+    off = torch.zeros(1, dtype=torch.int32)
+    with pytest.raises(gss.GssError):
+        gss.sort_device(text, off)
+
+
+def test_registry_mirrors_reference():
+    """routines.c:31-75: lookup by name, listing sorted by (multicore, name),
+    registration rejects incomplete routines."""
+    assert gss.routine_from_name("gss_msd").multicore
+    assert gss.routine_from_name("no_such_routine") is None
+    names = [r.name for r in gss.routine_get_all()]
+    assert names == sorted(names)
+    with pytest.raises(gss.GssError):
+        gss.routine_register(gss.Routine("", "x", False, False))
+    with pytest.raises(gss.GssError):
+        gss.routine_register(gss.Routine("x", "", False, False))
+
+
+def test_stats_and_profile_without_device():
+    s = gss.stats()
+    assert s.global_passes >= 0
+    gss.profile_reset()
+    assert gss.profile_get("scatter") == (0, 0.0, 0)

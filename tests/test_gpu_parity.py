@@ -1,0 +1,129 @@
+"""GPU parity: the device sort (gss_sort_device32, through the C ABI) against
+the oracle's msd_CE2 restatement, offset for offset and LCP for LCP."""
+
+import numpy as np
+import pytest
+
+from gpu_util import assert_matches_oracle, gpu_sort, log
+
+pytestmark = pytest.mark.gpu
+
+import gss  # noqa: E402
+import gss.gen as gen  # noqa: E402
+
+
+@pytest.fixture(autouse=True)
+def default_tuning():
+    gss.lib()
+    t = gss.get_tuning()
+    yield
+    gss.set_tuning(t.local_threshold, t.radix_bits, t.tile)
+
+
+@pytest.mark.parametrize("config", [0, 1, 2, 3])
+@pytest.mark.parametrize("n", [3000, 60000])
+def test_workload_parity(config, n, oracle_mod):
+    ss = gen.make(config, n=n)
+    log(f"workload {gen.NAMES[config]} n={n}")
+    off, lcp = gpu_sort(ss)
+    assert_matches_oracle(ss, off, lcp, oracle_mod)
+
+
+def _set(strs, name):
+    return gen.from_strings(strs, name)
+
+
+EDGE_CASES = {
+    "two": [b"b", b"a"],
+    "three_equal": [b"aaa"] * 3,
+    "all_equal_201": [b"aaa"] * 201,
+    "all_equal_9000": [b"aaa"] * 9000,
+    "a_bb_bbb": [b"bb", b"a", b"bbb"] * 801,
+    "empty_strings": [b"", b"a", b"", b"b", b""] * 50,
+    "insertion_known": [b"c", b"", b"bbaaa", b"aaaaa", b"bbbbb"],
+    "high_bytes": [bytes([0x41, 0x61, 0x7F, 0x80, 0xC3, 0xFF][i % 6] for i in range(j % 13))
+                   + bytes([0xFF - (j % 7)]) for j in range(5000)],
+    "key_boundaries": [b"x" * L for L in range(0, 40)] + [b"x" * L + b"y" for L in range(0, 40)],
+    "modular_9": [bytes([ord("a") + (i % 30)]) * 9 for i in range(8001)],
+}
+
+
+@pytest.mark.parametrize("case", sorted(EDGE_CASES))
+def test_edge_cases(case, oracle_mod):
+    ss = _set(EDGE_CASES[case], case)
+    log(f"edge case {case} n={ss.n}")
+    off, lcp = gpu_sort(ss)
+    assert_matches_oracle(ss, off, lcp, oracle_mod)
+
+
+def test_empty_and_single():
+    for strs in ([], [b"only"]):
+        ss = _set(strs, "tiny")
+        off, lcp = gpu_sort(ss)
+        assert list(off) == list(ss.offsets)
+
+
+def _deep_prefix(n, plen, seed):
+    rng = np.random.default_rng(seed)
+    pre = b"p" * plen
+    return [pre + bytes(rng.integers(97, 100, size=int(rng.integers(0, 6))).astype(np.uint8))
+            for _ in range(n)]
+
+
+@pytest.mark.parametrize("plen", [7, 8, 64, 301])
+def test_deep_shared_prefix(plen, oracle_mod):
+    ss = _set(_deep_prefix(20000, plen, plen), f"deep{plen}")
+    log(f"deep prefix plen={plen}")
+    off, lcp = gpu_sort(ss)
+    assert_matches_oracle(ss, off, lcp, oracle_mod)
+
+
+@pytest.mark.parametrize("threshold", [64, 1024, 8192])
+@pytest.mark.parametrize("radix_bits", [8, 11])
+def test_tuning_knobs(threshold, radix_bits, oracle_mod):
+    gss.set_tuning(local_threshold=threshold, radix_bits=radix_bits)
+    for config in (0, 1, 3):
+        ss = gen.make(config, n=40000, seed=7 + config)
+        log(f"tuning threshold={threshold} radix_bits={radix_bits} workload {ss.name}")
+        off, lcp = gpu_sort(ss)
+        assert_matches_oracle(ss, off, lcp, oracle_mod)
+    ss = _set(_deep_prefix(30000, 40, 3), "deep40")
+    off, lcp = gpu_sort(ss)
+    assert_matches_oracle(ss, off, lcp, oracle_mod)
+
+
+def test_suffixes(oracle_mod):
+    """Overlapping strings: every suffix of one text (sortstring's
+    --suffix-sorting builds these, src/sortstring.c:349-358)."""
+    rng = np.random.default_rng(11)
+    body = bytes(rng.choice(np.frombuffer(b"ab", np.uint8), size=3000))
+    text = np.zeros(len(body) + 1 + gen.PAD, dtype=np.uint8)
+    text[: len(body)] = np.frombuffer(body, np.uint8)
+    ss = gen.StringSet(text, np.arange(len(body) + 1, dtype=np.uint32), "suffixes")
+    off, lcp = gpu_sort(ss)
+    assert_matches_oracle(ss, off, lcp, oracle_mod)
+
+
+def test_repeat_calls_reuse_workspace(oracle_mod):
+    for n in (50000, 1000, 70000, 10):
+        ss = gen.make(1, n=n, seed=n)
+        off, lcp = gpu_sort(ss)
+        assert_matches_oracle(ss, off, lcp, oracle_mod)
+
+
+def test_without_lcp(oracle_mod):
+    ss = gen.make(1, n=30000, seed=5)
+    off, _ = gpu_sort(ss, lcp=False)
+    assert np.array_equal(off, oracle_mod.sort_offsets(ss.text, ss.offsets))
+
+
+def test_in_place_output(oracle_mod):
+    import torch
+
+    ss = gen.make(0, n=20000, seed=3)
+    dev = torch.device("cuda:0")
+    text = torch.from_numpy(ss.text).to(dev)
+    off = torch.from_numpy(ss.offsets.view(np.int32)).to(dev)
+    gss.sort_device(text, off, off, None, text_bytes=ss.text_bytes)
+    got = off.cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, oracle_mod.sort_offsets(ss.text, ss.offsets))

@@ -1,0 +1,136 @@
+"""GPU: the routine.h drop-in (gss_msd / gss_msd_lcp) driven as the reference
+drives its routines -- by the reference's own unit test and CLI compiled from
+its sources with the GPU routine registered (oracle/_ref), and through the
+pointer ABI from Python with the unit test's string families."""
+
+import ctypes as C
+import hashlib
+import mmap
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from gpu_util import log
+
+pytestmark = pytest.mark.gpu
+
+import gss  # noqa: E402
+import gss.gen as gen  # noqa: E402
+
+ROOT = Path(__file__).resolve().parents[1]
+REF = ROOT / "oracle" / "_ref"
+
+
+def test_unit_test_families():
+    """unit-test/main.cpp:220-270, through the C pointer ABI (each string a
+    separate allocation, as strdup gives)."""
+    r = gss.routine_from_name("gss_msd")
+    for k in range(1, 2000, 200):
+        assert r.f([b"aaa"] * k) == [b"aaa"] * k
+    for k in range(1, 1000, 200):
+        out = r.f([b"bb", b"a", b"bbb"] * k)
+        assert out == [b"a"] * k + [b"bb"] * k + [b"bbb"] * k
+    for k in range(1, 10000, 2000):
+        strs = [bytes([ord("a") + (i % 30)]) * 9 for i in range(k)]
+        assert r.f(strs) == sorted(strs)
+
+
+def test_pointer_lcp(oracle_mod):
+    ss = gen.make(1, n=20000, seed=9)
+    strs = ss.strings()
+    out, lcp = gss.sort_strings(strs, lcp=True)
+    assert out == sorted(strs)
+    want = [0] * (len(out) - 1)
+    for i in range(len(out) - 1):
+        a, b = out[i], out[i + 1]
+        j = 0
+        while j < len(a) and j < len(b) and a[j] == b[j]:
+            j += 1
+        want[i] = j
+    assert lcp == want
+
+
+def test_pointer_span_text(oracle_mod):
+    """Pointers into one text buffer (sortstring's layout): the output is the
+    input pointer set permuted into msd_CE2's order."""
+    ss = gen.make(0, n=100000, seed=4)
+    base = ss.text.ctypes.data
+    ptrs = (C.c_void_p * ss.n)(*[base + int(o) for o in ss.offsets])
+    gss.sort_pointers(ptrs, ss.n)
+    got = np.array([p - base for p in ptrs], dtype=np.uint32)
+    assert np.array_equal(got, oracle_mod.sort_offsets(ss.text, ss.offsets))
+
+
+def test_pointer_packed_path(oracle_mod):
+    """Strings in two mappings separated by an unmapped hole force the packed
+    upload; the result must still be the caller's own pointers."""
+    libc = C.CDLL(None, use_errno=True)
+    libc.mmap.restype = C.c_void_p
+    libc.mmap.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_long]
+    libc.munmap.argtypes = [C.c_void_p, C.c_size_t]
+    page = mmap.PAGESIZE
+    size = 64 * page
+    addr = libc.mmap(None, 3 * size, mmap.PROT_READ | mmap.PROT_WRITE,
+                     mmap.MAP_PRIVATE | mmap.MAP_ANONYMOUS, -1, 0)
+    assert addr not in (None, C.c_void_p(-1).value)
+    assert libc.munmap(addr + size, size) == 0  # the hole
+    try:
+        rng = np.random.default_rng(1)
+        strs, ptrs = [], []
+        for region in (addr, addr + 2 * size):
+            pos = region
+            while pos + 40 < region + size:
+                s = bytes(rng.integers(97, 101, size=int(rng.integers(0, 12))).astype(np.uint8))
+                C.memmove(pos, s + b"\0", len(s) + 1)
+                strs.append(s)
+                ptrs.append(pos)
+                pos += len(s) + 1
+        order = rng.permutation(len(ptrs))
+        arr = (C.c_void_p * len(ptrs))(*[ptrs[i] for i in order])
+        lcp = gss.sort_pointers(arr, len(ptrs), lcp=True)
+        got = [C.string_at(p) for p in arr]
+        assert got == sorted(strs)
+        assert sorted(arr) == sorted(ptrs)
+        # stable: equal strings keep their input order
+        rank = {ptrs[i]: r for r, i in enumerate(order)}
+        for a, b in zip(arr[:-1], arr[1:]):
+            if C.string_at(a) == C.string_at(b):
+                assert rank[a] < rank[b]
+        assert lcp[0] == len(os.path.commonprefix([got[0], got[1]]))
+    finally:
+        libc.munmap(addr, size)
+        libc.munmap(addr + 2 * size, size)
+
+
+@pytest.mark.skipif(not (REF / "unit-test").exists(), reason="oracle/_ref not built")
+def test_reference_unit_test():
+    """The reference's own unit test (unit-test/main.cpp), unchanged, with the
+    GPU routines registered through routine.h."""
+    p = subprocess.run([str(REF / "unit-test")], capture_output=True, text=True, timeout=600)
+    log(p.stderr[-2000:])
+    assert p.returncode == 0
+    assert "[gss_msd]" in p.stderr and "[gss_msd_with_lcp]" in p.stderr
+    assert "*** All OK ***" in p.stderr
+
+
+@pytest.mark.skipif(not (REF / "sortstring").exists(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("config", [0, 1])
+def test_reference_sortstring(config, tmp_path):
+    """sortstring --check --write (src/sortstring.c:424-451): the GPU routine
+    reports 'Check: GOOD' and writes the same bytes as msd_CE2."""
+    ss = gen.make(config, n=200000, seed=21 + config)
+    inp = tmp_path / "input.txt"
+    inp.write_bytes(ss.newline_bytes())
+    digests = {}
+    for alg in ("msd_CE2", "gss_msd", "gss_msd_with_lcp"):
+        out = tmp_path / f"{alg}.out"
+        p = subprocess.run([str(REF / "sortstring"), "--check", f"--write={out}", alg, str(inp)],
+                           capture_output=True, text=True, timeout=600, cwd=tmp_path)
+        log(p.stdout[-500:] + p.stderr[-500:])
+        assert p.returncode == 0
+        assert "Check: GOOD" in p.stderr
+        digests[alg] = hashlib.md5(out.read_bytes()).hexdigest()
+    assert digests["gss_msd"] == digests["msd_CE2"] == digests["gss_msd_with_lcp"]
