@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X MSD radix string sort.
+
+One step = one device-resident sort (gss_sort_device32 through the C ABI) of
+10,000,000 URL-like strings (2,000 shared prefixes of 20-60 bytes plus 5-40
+byte suffixes over [a-z0-9./:-], 30% exact duplicates), with the text and
+offsets already in HBM.  `value` is Mstrings/s over the timed steps.
+
+Besides the contract fields the JSON line carries:
+  roofline      the dominant kernel's algorithmic bytes per launch / its
+                average launch time (HIP events on the launch stream),
+                against the 8 TB/s HBM peak;
+  cpu_baseline  the reference's msd_CE2 (compiled from its sources into
+                oracle/_ref) timed on this host's cores on the same workload;
+  hbm_gbps      algorithmic bytes of the whole sort / sort time;
+  e2e           the routine.h pointer path (gss_msd), PCIe transfers included.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path[:0] = [str(ROOT / "string-sorting_amd"), str(ROOT / "oracle")]
+
+METRIC = "Mstrings/s + HBM GB/s (10M URL-like, 1 GPU); batched p50/p99 vs plain"
+HBM_PEAK_GBPS = 8000.0
+# Algorithmic bytes per active string per MSD round: read its 32-bit offset
+# and 8 text bytes, write the partitioned (8-byte key, offset) pair.
+BYTES_PER_ROUND_STRING = 2 * (4 + 8)
+KERNELS = ["gather", "layout", "histogram", "flat_scan", "segscan", "scatter", "boundary",
+           "retire", "local64", "local1024", "local8192", "init", "init_segs"]
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=10_000_000)
+    ap.add_argument("--lcp", action="store_true", help="also emit the LCP array in the timed steps")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the pointer-path timing")
+    ap.add_argument("--cpu-runs", type=int, default=2)
+    return ap.parse_args()
+
+
+def cpu_baseline(ss, runs):
+    """msd_CE2 of the reference (oracle/_ref/libref.so) on the same strings,
+    one core.  Falls back to the oracle's C restatement when the reference
+    build is absent."""
+    import oracle
+
+    base = np.uint64(ss.text.ctypes.data)
+    ptrs0 = (base + ss.offsets.astype(np.uint64)).astype(np.uint64)
+    if oracle.ref_available():
+        kind, fn = "reference", lambda p: oracle.ref_sort_pointers("msd_CE2", p)
+        name = "msd_CE2 (reference sources, gcc -O3)"
+    else:
+        lib = oracle.lib()
+        kind, fn = "port", lambda p: lib.or_msd_ce2(p.ctypes.data, p.shape[0])
+        name = "msd_CE2 restatement (oracle.c)"
+    times = []
+    for _ in range(runs):
+        p = ptrs0.copy()
+        t0 = time.perf_counter()
+        fn(p)
+        times.append(time.perf_counter() - t0)
+    best = min(times)
+    out = {"value": ss.n / best / 1e6, "unit": "Mstrings/s", "cores": 1, "kind": kind,
+           "sample": f"{name} on the full {ss.n}-string URL-like set, best of {runs} runs "
+                     f"({', '.join(f'{t:.2f}' for t in times)} s)"}
+    if oracle.ref_available():
+        try:
+            threads = min(16, os.cpu_count() or 1)
+            os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+            p = ptrs0.copy()
+            t0 = time.perf_counter()
+            oracle.ref_sort_pointers("parallel_msd_radix_sort", p)
+            tp = time.perf_counter() - t0
+            out["multicore"] = {"routine": "parallel_msd_radix_sort", "threads": threads,
+                                "value": ss.n / tp / 1e6, "seconds": tp}
+        except Exception as e:  # the baseline is a report, not the target
+            out["multicore"] = {"error": str(e)}
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is a "
+              "design in DESIGN.md, not built.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import gss
+    import gss.gen as gen
+
+    t0 = time.time()
+    ss = gen.url_like(args.n)
+    log(f"generated {ss.n} strings, {ss.text_bytes} text bytes in {time.time() - t0:.1f}s")
+    dev = torch.device("cuda:0")
+    text = torch.from_numpy(ss.text).to(dev)
+    off = torch.from_numpy(ss.offsets.view(np.int32)).to(dev)
+    out = torch.empty_like(off)
+    lcp = torch.empty(max(1, ss.n - 1), dtype=torch.int64, device=dev) if args.lcp else None
+    stream = torch.cuda.current_stream(dev)
+    torch.cuda.synchronize()
+
+    def step():
+        gss.sort_device(text, off, out, lcp, text_bytes=ss.text_bytes, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    gss.profile_reset()
+    gss.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    gss.profile_enable(False)
+    st = gss.stats()
+
+    ms_per_step = dt / args.steps * 1e3
+    value = ss.n * args.steps / dt / 1e6
+
+    prof = {}
+    for k in KERNELS:
+        launches, ms, items = gss.profile_get(k)
+        if launches:
+            prof[k] = {"launches": launches, "ms_total": ms, "ms_avg": ms / launches,
+                       "items": items}
+    # Work units of one sort: active strings summed over all rounds.
+    global_rounds = [int(st.active[i]) for i in range(min(st.global_passes, 64))]
+    round_strings = sum(global_rounds) + int(st.local_round_strings)
+    alg_bytes = BYTES_PER_ROUND_STRING * round_strings + 4 * ss.n + (8 * (ss.n - 1) if args.lcp else 0)
+    hbm_gbps = alg_bytes / (ms_per_step * 1e-3) / 1e9
+
+    dominant = max(prof, key=lambda k: prof[k]["ms_total"])
+    d = prof[dominant]
+    per_launch_bytes = BYTES_PER_ROUND_STRING * d["items"] / d["launches"]
+    achieved = per_launch_bytes / (d["ms_avg"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                "alg_bytes_per_launch": per_launch_bytes, "avg_launch_ms": d["ms_avg"]}
+
+    res = {
+        "metric": METRIC, "value": value, "unit": "Mstrings/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+        "data": "synthetic (seeded URL-like generator, string-sorting_amd/gss/gen.py)",
+        "config": {"workload": "url_like_10M", "strings": ss.n, "text_bytes": ss.text_bytes,
+                   "prefixes": 2000, "prefix_len": [20, 60], "suffix_len": [5, 40],
+                   "duplicates": 0.3, "lcp": bool(args.lcp), "parallelism": "single GPU"},
+        "hbm_gbps": hbm_gbps,
+        "alg_bytes_per_sort": alg_bytes,
+        "rounds": {"global_passes": int(st.global_passes), "global_active": global_rounds,
+                   "local_segments": int(st.local_segments),
+                   "local_strings": int(st.local_strings),
+                   "local_round_strings": int(st.local_round_strings),
+                   "local_rounds_max": int(st.local_rounds_max)},
+        "kernels": prof,
+        "roofline": roofline,
+    }
+
+    # Parity of the benchmarked output (full size, size-independent checks).
+    import oracle
+
+    got = out.cpu().numpy().view(np.uint32)
+    ident, inval, wrong = oracle.check_result(ss.text, got)
+    perm_ok = bool(np.array_equal(np.sort(got), np.sort(ss.offsets)))
+    res["check"] = {"inversions": int(wrong), "identical_ptrs": int(ident), "permutation": perm_ok}
+
+    if not args.no_e2e:
+        base = ss.text.ctypes.data
+        ptrs = (np.uint64(base) + ss.offsets.astype(np.uint64)).astype(np.uint64)
+        arr = ptrs.ctypes.data_as(C.POINTER(C.c_void_p))
+        gss.lib().gss_msd(arr, ss.n)  # warm: device buffers for the pointer path
+        ptrs[:] = (np.uint64(base) + ss.offsets.astype(np.uint64))
+        t0 = time.perf_counter()
+        gss.lib().gss_msd(arr, ss.n)
+        te = time.perf_counter() - t0
+        e2e_ok = bool(np.array_equal((ptrs - np.uint64(base)).astype(np.uint32), got))
+        res["e2e"] = {"entry": "gss_msd (host pointers, H2D text+pointers, D2H pointers)",
+                      "seconds": te, "mstrings_per_s": ss.n / te / 1e6, "matches_device": e2e_ok}
+
+    if not args.no_cpu:
+        log("timing the CPU baseline ...")
+        res["cpu_baseline"] = cpu_baseline(ss, args.cpu_runs)
+        res["vs_cpu"] = value / res["cpu_baseline"]["value"]
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

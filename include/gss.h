@@ -94,7 +94,8 @@ typedef struct gss_stats {
 	uint32_t local_segments;
 	uint64_t active[GSS_MAX_PASSES];
 	uint64_t local_strings;  /* strings finished inside one workgroup */
-	uint32_t local_rounds_max;
+	uint32_t local_rounds_max;  /* most 8-byte key rounds one such segment took */
+	uint64_t local_round_strings; /* sum over those rounds of active strings */
 } gss_stats;
 
 int gss_get_stats(gss_stats *out);

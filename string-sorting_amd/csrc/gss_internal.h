@@ -36,8 +36,21 @@ struct Tile {
   uint32_t seg;    // index into the pass's BigSeg list
   uint32_t begin;  // absolute position of the first string
   uint32_t len;    // strings in this tile (<= kTile)
-  uint32_t first;  // 1 for the first tile of its segment
+  uint32_t tidx;   // index of this tile inside its segment
 };
+
+// Where a BigSeg's digit-major histogram sits in the pass's flat count
+// array: entry flat_base + d * ntiles + t counts digit d in tile t, for the
+// nbu digit values the segment can take (0 when its keys are all equal).
+struct SegAux {
+  uint32_t flat_base;
+  uint32_t ntiles;
+  uint32_t nbu;
+  uint32_t pad;
+};
+
+// Entries per block of the flat exclusive scan.
+constexpr uint32_t kScanBlock = 4096;
 
 // A segment finished inside one workgroup.
 struct LocalSeg {
@@ -71,10 +84,12 @@ struct Counters {
   uint32_t big_next;     // BigSegs emitted for the next pass
   uint32_t local[3];     // LocalSegs per size class
   uint32_t retire;       // Retire chunks
-  uint32_t boundary;     // Boundaries of the current pass
-  uint32_t tile_ticket;  // dynamic tile ids, monotonic over the whole sort
-  uint32_t error;        // nonzero: a bounded wait gave up or a list overflowed
+  uint32_t boundary;     // Boundaries emitted so far
+  uint32_t flat_len;     // entries of the current pass's flat count array
+  uint32_t error;        // nonzero: a list overflowed or a loop bound was hit
   uint32_t local_strings[3];  // strings handed to each local size class
+  uint32_t local_rounds_max;  // most key rounds any local segment needed
+  unsigned long long local_round_strings[3];  // sum over local rounds of active strings
 };
 
 // Size classes of the local kernel: (threads, capacity).
@@ -91,10 +106,12 @@ struct Plan {
   uint64_t* keys;        // n cached keys (global passes)
   BigSeg* big[2];        // BigSeg lists (current, next)
   uint64_t* segmm;       // per BigSeg: min, max key
-  uint32_t* seghist;     // per BigSeg: 2^radix_bits digit counts
-  uint32_t* bstart;      // per BigSeg: digit bucket starts | final flag
+  uint32_t* bstart;      // per BigSeg and digit: 1 when the bucket is final
+  uint32_t* alpha;       // 256-bit set of byte values seen in the pass's keys
   Tile* tiles;
-  uint64_t* status;      // look-back words, tiles * 2^radix_bits
+  SegAux* aux;           // per BigSeg: layout of its counts in `flat`
+  uint32_t* flat;        // digit-major per-tile counts, scanned in place (+1 total)
+  uint32_t* flat_blk;    // block sums of the flat scan
   LocalSeg* local[3];
   Retire* retire;
   Boundary* bounds;
@@ -107,6 +124,29 @@ struct Plan {
 // ---------------------------------------------------------------------------
 // Device primitives
 
+// Aligned text word `w` (little-endian), 0 beyond the text.  The address is
+// clamped instead of branched around, so a run of these loads issues back to
+// back and waits once.  nwords >= 1.
+__device__ __forceinline__ uint64_t text_word(const uint64_t* __restrict__ text64,
+                                              uint64_t nwords, uint64_t w) {
+  const uint64_t v = text64[w < nwords ? w : nwords - 1];
+  return w < nwords ? v : 0ull;
+}
+
+// Clears every byte after the first NUL of little-endian `v`; *ended tells
+// whether `v` holds a NUL.  The lowest flagged bit of the zero-byte test is
+// the top bit of the first zero byte (false positives only sit above it).
+__device__ __forceinline__ uint64_t cut_at_nul(uint64_t v, bool* ended) {
+  const uint64_t z = (v - 0x0101010101010101ull) & ~v & 0x8080808080808080ull;
+  *ended = z != 0;
+  return z ? (v & ((z & (0ull - z)) - 1ull)) : v;
+}
+
+// Little-endian 8 bytes at byte `sh/8` of the 16 bytes lo:hi (sh in 0..56).
+__device__ __forceinline__ uint64_t funnel(uint64_t lo, uint64_t hi, uint32_t sh) {
+  return (lo >> sh) | ((hi << 1) << (63 - sh));
+}
+
 // 8 bytes of the text starting at byte `pos`, big-endian (first byte most
 // significant), zero after the first NUL; words beyond the text read as 0.
 // Same packing as fill_cache in src/multikey_cache.cpp:107-125.
@@ -114,20 +154,31 @@ __device__ __forceinline__ uint64_t load_key(const uint64_t* __restrict__ text64
                                              uint64_t nwords, uint64_t pos) {
   const uint64_t w = pos >> 3;
   const uint32_t sh = (uint32_t)(pos & 7) * 8;
-  const uint64_t lo = w < nwords ? text64[w] : 0;
-  uint64_t v = lo;
-  if (sh) {
-    const uint64_t hi = (w + 1 < nwords) ? text64[w + 1] : 0;
-    v = (lo >> sh) | (hi << (64 - sh));
+  const uint64_t lo = text_word(text64, nwords, w);
+  const uint64_t hi = text_word(text64, nwords, w + 1);
+  bool ended;
+  return __builtin_bswap64(cut_at_nul(funnel(lo, hi, sh), &ended));
+}
+
+// KW consecutive keys of the string bytes at `pos`: key q covers bytes
+// [pos+8q, pos+8q+8); every byte after the first NUL is zero, so the keys of
+// a string that ended are 0.
+template <int KW>
+__device__ __forceinline__ void load_keys(const uint64_t* __restrict__ text64, uint64_t nwords,
+                                          uint64_t pos, uint64_t (&out)[KW]) {
+  const uint64_t w = pos >> 3;
+  const uint32_t sh = (uint32_t)(pos & 7) * 8;
+  uint64_t raw[KW + 1];
+#pragma unroll
+  for (int q = 0; q <= KW; ++q) raw[q] = text_word(text64, nwords, w + q);
+  bool done = false;
+#pragma unroll
+  for (int q = 0; q < KW; ++q) {
+    bool e;
+    const uint64_t v = cut_at_nul(funnel(raw[q], raw[q + 1], sh), &e);
+    out[q] = done ? 0ull : __builtin_bswap64(v);
+    done = done || e;
   }
-  // Little-endian v: byte i is text[pos+i].  The lowest flagged byte of the
-  // zero-byte test is the first NUL (false positives only sit above it).
-  const uint64_t z = (v - 0x0101010101010101ull) & ~v & 0x8080808080808080ull;
-  if (z) {
-    const uint32_t zb = (uint32_t)__builtin_ctzll(z) >> 3;
-    v = zb ? (v & (~0ull >> (64 - 8 * zb))) : 0;
-  }
-  return __builtin_bswap64(v);
 }
 
 // Number of string bytes before the NUL in a key whose low byte is zero.
@@ -258,18 +309,18 @@ __device__ __forceinline__ uint32_t block_max(uint32_t v, uint32_t* scratch) {
 // ---------------------------------------------------------------------------
 // Launch wrappers (kernels.hip).  All return hipError_t of the launch.
 
-hipError_t launch_init_segs(const Plan& p, uint32_t nsegs, uint32_t nbins,
-                            hipStream_t s);
+hipError_t launch_init_segs(const Plan& p, uint32_t nsegs, hipStream_t s);
 hipError_t launch_gather_minmax(const Plan& p, const BigSeg* segs, uint32_t ntiles,
                                 hipStream_t s);
-hipError_t launch_histogram(const Plan& p, const BigSeg* segs, uint32_t ntiles,
-                            uint32_t radix_bits, hipStream_t s);
+hipError_t launch_layout(const Plan& p, uint32_t nsegs, uint32_t radix_bits, hipStream_t s);
+hipError_t launch_histogram(const Plan& p, uint32_t ntiles, uint32_t radix_bits,
+                            hipStream_t s);
+// Scans flat[0, ctr->flat_len) in place; max_len bounds the launch size.
+hipError_t launch_flat_scan(const Plan& p, uint64_t max_len, hipStream_t s);
 hipError_t launch_segscan(const Plan& p, const BigSeg* segs, BigSeg* next,
-                          uint32_t nsegs, uint32_t pass, uint32_t radix_bits,
-                          hipStream_t s);
+                          uint32_t nsegs, uint32_t radix_bits, hipStream_t s);
 hipError_t launch_scatter(const Plan& p, const BigSeg* segs, uint32_t ntiles,
-                          uint32_t pass, uint32_t epoch, uint32_t radix_bits,
-                          hipStream_t s);
+                          uint32_t radix_bits, hipStream_t s);
 hipError_t launch_boundary_lcp(const Plan& p, uint32_t first, hipStream_t s);
 hipError_t launch_retire(const Plan& p, uint32_t nretire, hipStream_t s);
 hipError_t launch_local(const Plan& p, int cls, uint32_t nsegs, hipStream_t s);

@@ -1,28 +1,30 @@
-// HIP kernels of the MI355X MSD radix string sort (gfx950, wave64).
+// HIP kernels of the global MSD passes of the MI355X string sort (gfx950,
+// wave64), plus small helpers.
 //
-// A sort runs as global MSD passes over segments too large for one
-// workgroup, then finishes every remaining segment inside one workgroup.
+// A sort runs global MSD passes over segments too large for one workgroup,
+// then finishes every remaining segment inside one workgroup
+// (local_sort.hip).  One global pass over the BigSeg list, tiles of kTile
+// strings that never span two segments:
 //
-// Global pass over the BigSeg list (one launch per step, tiles of kTile
-// strings, never spanning two segments):
 //   gather_minmax  key = the 8 bytes at each string's depth (big-endian, NUL
-//                  padded), cached in `keys`; per-segment min/max key.
-//   histogram      digit = the radix_bits bits that follow the common prefix
-//                  of the segment's min and max key; per-segment histogram.
-//   segscan        one workgroup per segment: bucket starts, child segments
+//                  padded), cached in `keys`; per-segment min/max key; the
+//                  pass alphabet (set of byte values seen).
+//   layout         per segment: its digit (first differing byte + k ranked
+//                  characters, see digit_spec) and the place of its
+//                  digit-major count matrix in the flat array.
+//   histogram      per tile digit counts -> flat[base + d*ntiles + t].
+//   scan (x3)      exclusive scan of the whole flat array: entry (d, t) now
+//                  holds where tile t's strings of digit d go.
+//   segscan        one workgroup per segment: bucket sizes, child segments
 //                  (next pass, local, or final), retire runs, boundaries.
-//   scatter        stable partition of (offset) by digit: wave match-ranking
-//                  in LDS + decoupled look-back across the segment's tiles,
-//                  staged in LDS so the global writes are contiguous runs.
+//   scatter        stable partition of the offsets by digit: wave
+//                  match-ranking in LDS for the in-tile rank, staged in LDS
+//                  so the global writes are contiguous runs.
 //   boundary_lcp   LCP of each new adjacent pair of child buckets.
+//
 // A segment whose keys are all equal costs the gather only: it keeps its
 // place and buffer and moves 8 bytes deeper (or retires if the key holds the
 // NUL, i.e. all its strings are identical).
-//
-// local<BLOCK,CAP>: one workgroup per segment of <= CAP strings, rounds of
-// (gather 8-byte keys, stable LSD radix sort of (sub-segment, key) in LDS on
-// the bytes that vary inside some sub-segment, split into equal-key groups,
-// retire singletons and NUL-terminated groups, emit LCPs) until done.
 //
 // The reference's byte-at-a-time MSD (msd_CE2, src/msd_ce.cpp:143-178) and
 // the 8-byte cache rules of multikey_cache (src/multikey_cache.cpp:107-125,
@@ -35,12 +37,11 @@ namespace gss {
 // ---------------------------------------------------------------------------
 // Small helpers
 
-__global__ void k_init_segs(uint64_t* segmm, uint32_t* seghist, uint32_t nsegs,
-                            uint32_t nbins) {
+__global__ void k_init_segs(uint64_t* segmm, uint32_t* alpha, uint32_t nsegs) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t0 < 8) alpha[t0] = 0;
   for (uint64_t i = t0; i < 2ull * nsegs; i += stride) segmm[i] = (i & 1) ? 0ull : ~0ull;
-  for (uint64_t i = t0; i < (uint64_t)nsegs * nbins; i += stride) seghist[i] = 0;
 }
 
 __global__ void k_fill_u64(uint64_t* dst, uint64_t n, uint64_t v) {
@@ -86,17 +87,100 @@ __global__ void k_off_to_orig_ptr(const uint32_t* __restrict__ off,
   }
 }
 
-// Digit of `key` for a segment whose keys share their first `lzb` bits.
+// ---------------------------------------------------------------------------
+// Digits of a global pass.
+//
+// A pass records which byte values occur in its keys (the alphabet, A
+// symbols) and ranks them; rank order is byte order, so a digit formed as
+// the mixed-radix number of k consecutive ranks orders strings exactly as
+// their bytes do.  k is the largest count with A^k <= 2^radix_bits, e.g. 2
+// characters of [a-z0-9./:-] or 4 of {A,C,G,T} per 11-bit digit.  A segment's
+// digit starts at the first key byte where its min and max key differ.
+
+struct DigitSpec {
+  uint32_t fb0;  // first key byte that differs inside the segment
+  uint32_t k;    // characters in the digit
+  uint32_t A;    // alphabet size of the pass
+  uint32_t nbu;  // digit values in use: A^k
+};
+
+__device__ __forceinline__ uint32_t alpha_size(const uint32_t* __restrict__ alpha) {
+  uint32_t total = 0;
+#pragma unroll
+  for (int w = 0; w < 8; ++w) total += __popc(alpha[w]);
+  return total;
+}
+
+// Fills rank[0..256) from the pass alphabet and returns its size.  All
+// threads of the block call it; rank is ready after the next barrier.
+__device__ __forceinline__ uint32_t build_rank(const uint32_t* __restrict__ alpha,
+                                               uint8_t* rank) {
+  uint32_t words[8];
+  uint32_t total = 0;
+#pragma unroll
+  for (int w = 0; w < 8; ++w) {
+    words[w] = alpha[w];
+    total += __popc(words[w]);
+  }
+  for (uint32_t t = threadIdx.x; t < 256; t += blockDim.x) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int w = 0; w < 8; ++w)
+      if ((uint32_t)w < (t >> 5)) r += __popc(words[w]);
+    r += __popc(words[t >> 5] & ((1u << (t & 31)) - 1u));
+    rank[t] = (uint8_t)r;
+  }
+  return total;
+}
+
 template <int RB>
-__device__ __forceinline__ uint32_t seg_digit(uint64_t key, uint32_t lzb) {
-  return (uint32_t)((key << lzb) >> (64 - RB));
+__device__ __forceinline__ DigitSpec digit_spec(uint64_t mn, uint64_t mx, uint32_t A) {
+  DigitSpec ds;
+  ds.fb0 = (uint32_t)__builtin_clzll(mn ^ mx) >> 3;
+  ds.A = A < 2 ? 2 : A;
+  uint32_t k = 1, v = ds.A;
+  while (k < 8 && (uint64_t)v * ds.A <= (1ull << RB)) {
+    v *= ds.A;
+    ++k;
+  }
+  ds.k = min(k, 8 - ds.fb0);
+  uint32_t nbu = 1;
+  for (uint32_t c = 0; c < ds.k; ++c) nbu *= ds.A;
+  ds.nbu = nbu;
+  return ds;
+}
+
+__device__ __forceinline__ uint32_t digit_of(uint64_t key, const DigitSpec& ds,
+                                             const uint8_t* rank) {
+  uint32_t d = 0;
+  for (uint32_t c = 0; c < ds.k; ++c)
+    d = d * ds.A + rank[(uint32_t)(key >> (56 - 8 * (ds.fb0 + c))) & 0xFFu];
+  return d;
+}
+
+// Bits needed to tell digits below nbu apart.
+__device__ __forceinline__ uint32_t digit_bits(uint32_t nbu) {
+  return nbu <= 1 ? 1u : 32u - (uint32_t)__builtin_clz(nbu - 1);
+}
+
+// Lanes of the wave whose digit equals this lane's, among `live`; the digit
+// has `bits` significant bits (wave-uniform).
+__device__ __forceinline__ uint64_t match_digit_n(uint32_t d, uint64_t live, uint32_t bits) {
+  uint64_t m = live;
+  for (uint32_t b = 0; b < bits; ++b) {
+    const uint64_t bal = __ballot((d >> b) & 1u);
+    m &= ((d >> b) & 1u) ? bal : ~bal;
+  }
+  return m;
 }
 
 // ---------------------------------------------------------------------------
-// Global pass, step 1: gather keys, per-segment min/max.
+// Global pass, step 1: gather keys, per-segment min/max, pass alphabet.
 
 __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const BigSeg* segs) {
   __shared__ uint64_t s_mn[kTileThreads / kWave], s_mx[kTileThreads / kWave];
+  __shared__ uint8_t s_present[256];
+  if (threadIdx.x < 256) s_present[threadIdx.x] = 0;
   const Tile t = p.tiles[blockIdx.x];
   const BigSeg sg = segs[t.seg];
   const uint32_t* __restrict__ src = p.idx[sg.buf];
@@ -108,13 +192,12 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const Bi
     o[j] = i < t.len ? src[t.begin + i] : 0u;
   }
   uint64_t mn = ~0ull, mx = 0ull;
+  // Unconditional (out-of-tile slots read offset 0) so all loads issue
+  // before the first wait.
 #pragma unroll
-  for (int j = 0; j < kTileItems; ++j) {
-    const uint32_t i = j * kTileThreads + threadIdx.x;
-    if (i < t.len) {
-      k[j] = load_key(p.text64, p.nwords, (uint64_t)o[j] + sg.depth);
-    }
-  }
+  for (int j = 0; j < kTileItems; ++j)
+    k[j] = load_key(p.text64, p.nwords, (uint64_t)o[j] + sg.depth);
+  __syncthreads();  // s_present cleared
 #pragma unroll
   for (int j = 0; j < kTileItems; ++j) {
     const uint32_t i = j * kTileThreads + threadIdx.x;
@@ -122,6 +205,8 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const Bi
       p.keys[t.begin + i] = k[j];
       mn = k[j] < mn ? k[j] : mn;
       mx = k[j] > mx ? k[j] : mx;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) s_present[(uint32_t)(k[j] >> (8 * b)) & 0xFFu] = 1;
     }
   }
   mn = wave_min64(mn);
@@ -132,6 +217,15 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const Bi
     s_mx[wid] = mx;
   }
   __syncthreads();
+  // The pass alphabet: 256 presence flags folded into 8 words by ballots.
+  if (threadIdx.x < 256) {
+    const uint64_t bal = __ballot(s_present[threadIdx.x] != 0);
+    if (lane_id() == 0) {
+      const uint32_t w = threadIdx.x / 32;
+      if ((uint32_t)bal) atomicOr(&p.alpha[w], (uint32_t)bal);
+      if ((uint32_t)(bal >> 32)) atomicOr(&p.alpha[w + 1], (uint32_t)(bal >> 32));
+    }
+  }
   if (threadIdx.x == 0) {
     for (int w = 1; w < kTileThreads / kWave; ++w) {
       mn = s_mn[w] < mn ? s_mn[w] : mn;
@@ -143,31 +237,122 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const Bi
 }
 
 // ---------------------------------------------------------------------------
-// Global pass, step 2: per-segment digit histogram.
+// Global pass, step 2: digit ranges and the flat layout of the counts.
+
+template <int RB>
+__global__ __launch_bounds__(1024) void k_layout(Plan p, uint32_t nsegs) {
+  __shared__ uint32_t s_scan[1024 / kWave + 1];
+  const uint32_t A = alpha_size(p.alpha);
+  uint32_t carry = 0;
+  for (uint32_t s0 = 0; s0 < nsegs; s0 += 1024) {
+    const uint32_t s = s0 + threadIdx.x;
+    uint32_t entries = 0, nbu = 0;
+    if (s < nsegs) {
+      const uint64_t mn = p.segmm[2 * s], mx = p.segmm[2 * s + 1];
+      nbu = mn == mx ? 0u : digit_spec<RB>(mn, mx, A).nbu;
+      entries = nbu * p.aux[s].ntiles;
+    }
+    uint32_t total;
+    const uint32_t pre = block_excl_scan<1024>(entries, s_scan, &total);
+    if (s < nsegs) {
+      p.aux[s].flat_base = carry + pre;
+      p.aux[s].nbu = nbu;
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) p.ctr->flat_len = carry;
+}
+
+// ---------------------------------------------------------------------------
+// Global pass, step 3: per-tile digit counts, digit-major per segment.
 
 template <int RB>
 __global__ __launch_bounds__(kTileThreads) void k_histogram(Plan p) {
   constexpr uint32_t NB = 1u << RB;
   __shared__ uint32_t h[NB];
+  __shared__ uint8_t rank[256];
   const Tile t = p.tiles[blockIdx.x];
   const uint64_t mn = p.segmm[2 * t.seg], mx = p.segmm[2 * t.seg + 1];
   if (mn == mx) return;
-  const uint32_t lzb = (uint32_t)__builtin_clzll(mn ^ mx);
-  for (uint32_t d = threadIdx.x; d < NB; d += kTileThreads) h[d] = 0;
+  const DigitSpec ds = digit_spec<RB>(mn, mx, build_rank(p.alpha, rank));
+  for (uint32_t d = threadIdx.x; d < ds.nbu; d += kTileThreads) h[d] = 0;
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < kTileItems; ++j) {
     const uint32_t i = j * kTileThreads + threadIdx.x;
-    if (i < t.len) atomicAdd(&h[seg_digit<RB>(p.keys[t.begin + i], lzb)], 1u);
+    if (i < t.len) atomicAdd(&h[digit_of(p.keys[t.begin + i], ds, rank)], 1u);
   }
   __syncthreads();
-  uint32_t* gh = p.seghist + (uint64_t)t.seg * NB;
-  for (uint32_t d = threadIdx.x; d < NB; d += kTileThreads)
-    if (h[d]) atomicAdd(&gh[d], h[d]);
+  const SegAux ax = p.aux[t.seg];
+  uint32_t* col = p.flat + ax.flat_base + t.tidx;
+  for (uint32_t d = threadIdx.x; d < ds.nbu; d += kTileThreads) col[(uint64_t)d * ax.ntiles] = h[d];
 }
 
 // ---------------------------------------------------------------------------
-// Global pass, step 3: per-segment bucket starts and child classification.
+// Global pass, step 4: exclusive scan of the flat count array in place
+// (block sums, scan of the block sums, block-local scans plus offsets).
+// flat[len] receives the total.
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = kScanBlock / kScanThreads;
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(Plan p) {
+  __shared__ uint32_t s_scan[kScanThreads / kWave + 1];
+  const uint32_t len = p.ctr->flat_len;
+  const uint64_t b0 = (uint64_t)blockIdx.x * kScanBlock;
+  if (b0 >= len) return;
+  uint32_t sum = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const uint64_t i = b0 + j * kScanThreads + threadIdx.x;
+    if (i < len) sum += p.flat[i];
+  }
+  uint32_t total;
+  (void)block_excl_scan<kScanThreads>(sum, s_scan, &total);
+  if (threadIdx.x == 0) p.flat_blk[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_blocks(Plan p) {
+  __shared__ uint32_t s_scan[1024 / kWave + 1];
+  const uint32_t len = p.ctr->flat_len;
+  const uint32_t nblk = (len + kScanBlock - 1) / kScanBlock;
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < nblk; b0 += 1024) {
+    const uint32_t b = b0 + threadIdx.x;
+    const uint32_t v = b < nblk ? p.flat_blk[b] : 0u;
+    uint32_t total;
+    const uint32_t pre = block_excl_scan<1024>(v, s_scan, &total);
+    if (b < nblk) p.flat_blk[b] = carry + pre;
+    carry += total;
+  }
+  if (threadIdx.x == 0) p.flat[len] = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(Plan p) {
+  __shared__ uint32_t s_scan[kScanThreads / kWave + 1];
+  const uint32_t len = p.ctr->flat_len;
+  const uint64_t b0 = (uint64_t)blockIdx.x * kScanBlock;
+  if (b0 >= len) return;
+  // Thread t owns kScanItems consecutive entries.
+  const uint64_t first = b0 + (uint64_t)threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    v[j] = first + j < len ? p.flat[first + j] : 0u;
+    sum += v[j];
+  }
+  uint32_t total;
+  uint32_t run = block_excl_scan<kScanThreads>(sum, s_scan, &total) + p.flat_blk[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    if (first + j < len) p.flat[first + j] = run;
+    run += v[j];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Global pass, step 5: per-segment buckets and child classification.
 
 __device__ __forceinline__ void push_local(Plan& p, uint32_t begin, uint32_t size,
                                            uint32_t depth, uint32_t buf) {
@@ -214,7 +399,6 @@ template <int RB>
 __global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, BigSeg* next) {
   constexpr uint32_t NB = 1u << RB;
   constexpr uint32_t DPT = NB / 256;
-  __shared__ uint32_t s_scan[256 / kWave + 1];
   __shared__ uint32_t s_first;
   const uint32_t si = blockIdx.x;
   const BigSeg sg = segs[si];
@@ -229,52 +413,60 @@ __global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, Big
     return;
   }
   const uint32_t nbuf = sg.buf == 1 ? 2u : 1u;
-  const uint32_t lzb = (uint32_t)__builtin_clzll(mn ^ mx);
-  const uint32_t nreal = min((uint32_t)RB, 64u - lzb);  // digit bits inside the key
-  const uint32_t shared_bits = lzb + nreal;
-  const uint32_t fb = shared_bits / 8;  // bytes every child shares in full
-  const uint64_t hi = lzb ? (mn & (~0ull << (64 - lzb))) : 0ull;
-  const uint32_t* gh = p.seghist + (uint64_t)si * NB;
+  const bool has_nul = p.alpha[0] & 1u;  // NUL, if present, has rank 0
+  const DigitSpec ds = digit_spec<RB>(mn, mx, alpha_size(p.alpha));
+  const uint32_t fb = ds.fb0 + ds.k;  // bytes every child shares in full
+  const SegAux ax = p.aux[si];
+  const uint32_t* col0 = p.flat + ax.flat_base;
+  const uint32_t base0 = col0[0];
+  // Bucket of digit d: [start(d), start(d+1)), start(nbu) = segment size.
+  auto start = [&](uint32_t d) -> uint32_t {
+    return d < ds.nbu ? col0[(uint64_t)d * ax.ntiles] - base0 : sg.size;
+  };
 
-  uint32_t cnt[DPT];
-  uint32_t sum = 0, firstd = NB;
+  uint32_t st[DPT], cnt[DPT];
+  uint32_t firstd = NB;
 #pragma unroll
   for (uint32_t q = 0; q < DPT; ++q) {
     const uint32_t d = threadIdx.x * DPT + q;
-    cnt[q] = gh[d];
-    sum += cnt[q];
-    if (cnt[q] && firstd == NB) firstd = d;
+    st[q] = 0;
+    cnt[q] = 0;
+    if (d < ds.nbu) {
+      st[q] = start(d);
+      cnt[q] = start(d + 1) - st[q];
+      if (cnt[q] && firstd == NB) firstd = d;
+    }
   }
   if (threadIdx.x == 0) s_first = NB;
   __syncthreads();
   if (firstd < NB) atomicMin(&s_first, firstd);
-  uint32_t total;
-  uint32_t run = block_excl_scan<256>(sum, s_scan, &total);
+  __syncthreads();
   const uint32_t dfirst = s_first;
   uint32_t* bs = p.bstart + (uint64_t)si * NB;
 #pragma unroll
   for (uint32_t q = 0; q < DPT; ++q) {
     const uint32_t d = threadIdx.x * DPT + q;
     const uint32_t c = cnt[q];
-    const uint32_t st = run;
-    run += c;
-    if (c == 0) {
-      bs[d] = st;
-      continue;
-    }
-    const uint64_t pre = hi | ((uint64_t)(d >> (RB - nreal)) << (64 - shared_bits));
+    if (d >= ds.nbu || c == 0) continue;
+    // The bucket's strings ended if one of its digit characters is NUL (the
+    // common bytes before fb0 hold none: a NUL there would make min == max).
     bool ended = false;
     uint32_t nul = 0;
-    for (uint32_t j = 0; j < fb; ++j) {
-      if (((pre >> (56 - 8 * j)) & 0xFFu) == 0) {
-        ended = true;
-        nul = j;
-        break;
+    if (has_nul) {
+      uint32_t rest = d, scale = ds.nbu;
+      for (uint32_t ch = 0; ch < ds.k; ++ch) {
+        scale /= ds.A;
+        if (rest / scale == 0) {
+          ended = true;
+          nul = ds.fb0 + ch;
+          break;
+        }
+        rest %= scale;
       }
     }
     const bool fin = ended || c == 1;
-    bs[d] = st | (fin ? 0x80000000u : 0u);
-    const uint32_t cb = sg.begin + st;
+    bs[d] = fin ? 1u : 0u;
+    const uint32_t cb = sg.begin + st[q];
     if (d != dfirst) {
       const uint32_t slot = atomicAdd(&p.ctr->boundary, 1u);
       if (slot < p.cap_bounds)
@@ -291,17 +483,12 @@ __global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, Big
 }
 
 // ---------------------------------------------------------------------------
-// Global pass, step 4: stable scatter with decoupled look-back.
-
-constexpr uint64_t kFlagA = 1ull << 46;  // aggregate of this tile only
-constexpr uint64_t kFlagP = 2ull << 46;  // inclusive prefix up to this tile
-constexpr uint64_t kValMask = (1ull << 46) - 1;
-constexpr uint32_t kSpinLimit = 1u << 22;
+// Global pass, step 6: stable scatter.  The in-tile rank of each string
+// among the tile's strings of its digit comes from wave match-ranking; the
+// scanned flat array gives where the tile's strings of each digit start.
 
 template <int RB>
-__global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* segs,
-                                                          uint32_t ticket_base,
-                                                          uint32_t epoch) {
+__global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* segs) {
   constexpr uint32_t NB = 1u << RB;
   constexpr int W = kTileThreads / kWave;
   constexpr int IPW = kTile / W;  // items per wave
@@ -312,16 +499,14 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
   __shared__ uint32_t s_off[kTile];
   __shared__ uint16_t s_dig[kTile];
   __shared__ uint32_t s_scan[W + 1];
-  __shared__ uint32_t s_vt;
+  __shared__ uint8_t rank[256];
 
-  if (threadIdx.x == 0) s_vt = atomicAdd(&p.ctr->tile_ticket, 1u) - ticket_base;
-  __syncthreads();
-  const uint32_t vt = s_vt;
-  const Tile t = p.tiles[vt];
+  const Tile t = p.tiles[blockIdx.x];
   const BigSeg sg = segs[t.seg];
   const uint64_t mn = p.segmm[2 * t.seg], mx = p.segmm[2 * t.seg + 1];
-  if (mn == mx) return;  // whole segment skipped, no look-back partner
-  const uint32_t lzb = (uint32_t)__builtin_clzll(mn ^ mx);
+  if (mn == mx) return;  // whole segment stays where it is
+  const DigitSpec ds = digit_spec<RB>(mn, mx, build_rank(p.alpha, rank));
+  const uint32_t nbits = digit_bits(ds.nbu);
   const uint32_t* __restrict__ src = p.idx[sg.buf];
   uint32_t* __restrict__ dst = p.idx[sg.buf == 1 ? 2 : 1];
 
@@ -340,11 +525,10 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
     off[j] = 0;
     if (v) {
       off[j] = src[t.begin + i];
-      d = seg_digit<RB>(p.keys[t.begin + i], lzb);
+      d = digit_of(p.keys[t.begin + i], ds, rank);
     }
     dig[j] = d;
-    const uint64_t live = __ballot(v);
-    const uint64_t m = match_digit<RB>(d, live);
+    const uint64_t m = match_digit_n(d, __ballot(v), nbits);
     rnk[j] = 0;
     if (v) {
       const uint32_t base = s_whist[wid][d];
@@ -361,7 +545,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
   for (uint32_t q = 0; q < DPT; ++q) {
     const uint32_t d = threadIdx.x * DPT + q;
     uint32_t r = 0;
-    if (d < NB) {
+    if (d < ds.nbu) {
       for (int w = 0; w < W; ++w) {
         const uint32_t c = s_whist[w][d];
         s_whist[w][d] = (uint16_t)r;
@@ -373,43 +557,17 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
   }
   uint32_t tsum;
   uint32_t run = block_excl_scan<kTileThreads>(sum, s_scan, &tsum);
-  const uint32_t* bs = p.bstart + (uint64_t)t.seg * NB;
+  const SegAux ax = p.aux[t.seg];
+  const uint32_t* col = p.flat + ax.flat_base;
+  const uint32_t base0 = col[0];
 #pragma unroll
   for (uint32_t q = 0; q < DPT; ++q) {
     const uint32_t d = threadIdx.x * DPT + q;
-    if (d >= NB) continue;
-    s_tstart[d] = run;
-    const uint32_t c = tot[q];
-    run += c;
-    // Decoupled look-back over the earlier tiles of this segment.
-    uint64_t* st = p.status + (uint64_t)vt * NB + d;
-    const uint64_t tag = (uint64_t)epoch << 48;
-    uint64_t prefix = 0;
-    if (t.first) {
-      __hip_atomic_store(st, tag | kFlagP | c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      __hip_atomic_store(st, tag | kFlagA | c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      uint64_t* look = st - NB;
-      uint32_t spins = 0;
-      while (true) {
-        const uint64_t w = __hip_atomic_load(look, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const bool ready = (w >> 48) == epoch && (w & (kFlagA | kFlagP));
-        if (!ready) {
-          if (++spins > kSpinLimit) {
-            atomicOr(&p.ctr->error, 1u);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-          continue;
-        }
-        prefix += w & kValMask;
-        if (w & kFlagP) break;
-        look -= NB;
-      }
-      __hip_atomic_store(st, tag | kFlagP | (prefix + c), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+    if (d < ds.nbu) {
+      s_tstart[d] = run;
+      s_gbase[d] = sg.begin + (col[(uint64_t)d * ax.ntiles + t.tidx] - base0);
     }
-    s_gbase[d] = sg.begin + (bs[d] & 0x7FFFFFFFu) + (uint32_t)prefix;
+    run += tot[q];
   }
   __syncthreads();
 #pragma unroll
@@ -423,20 +581,22 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
     }
   }
   __syncthreads();
+  const uint32_t* bs = p.bstart + (uint64_t)t.seg * NB;
   for (uint32_t i = threadIdx.x; i < t.len; i += kTileThreads) {
     const uint32_t d = s_dig[i];
     const uint32_t pos = s_gbase[d] + (i - s_tstart[d]);
     const uint32_t o = s_off[i];
-    // Positions stay inside the segment unless a look-back wait gave up
-    // (error flagged above); never write outside it.
-    if (pos - sg.begin >= sg.size) continue;
+    if (pos - sg.begin >= sg.size) {  // counts disagree: never write outside
+      atomicOr(&p.ctr->error, 64u);
+      continue;
+    }
     dst[pos] = o;
-    if (bs[d] & 0x80000000u) p.out[pos] = o;
+    if (bs[d]) p.out[pos] = o;
   }
 }
 
 // ---------------------------------------------------------------------------
-// Global pass, step 5: LCP of pairs split by this pass.
+// Global pass, step 7: LCP of pairs split by this pass.
 
 __global__ void k_boundary_lcp(Plan p, uint32_t first) {
   const uint32_t n = min(p.ctr->boundary, p.cap_bounds);
@@ -465,261 +625,6 @@ __global__ void k_retire(Plan p, uint32_t count) {
 }
 
 // ---------------------------------------------------------------------------
-// Local sort: one workgroup finishes one segment of at most CAP strings.
-
-template <int BLOCK, int CAP>
-struct LocalSmem {
-  static constexpr int W = BLOCK / kWave;
-  uint64_t key[CAP];
-  uint32_t off[CAP];
-  uint16_t sid[CAP];
-  uint16_t ss_start[CAP / 2 + 2];  // sub-segment start in the active list
-  uint16_t ss_base[CAP / 2 + 2];   // sub-segment start in the final order
-  uint16_t whist[W][256];
-  uint32_t scan[W + 1];
-  uint64_t red[W];
-};
-
-// One stable LSD pass over the active list by an 8-bit digit: key byte
-// `which` (0 = least significant) or, for which = 8 / 9, the low / high byte
-// of the sub-segment id.
-template <int BLOCK, int CAP>
-__device__ void local_radix_pass(LocalSmem<BLOCK, CAP>& s, uint32_t nact, int which) {
-  constexpr int W = BLOCK / kWave;
-  constexpr int IPW = CAP / W;
-  constexpr int IPT = IPW / kWave;
-  const int wid = threadIdx.x / kWave;
-  const int lane = lane_id();
-  const uint64_t lt = lanemask_lt();
-
-  for (int i = threadIdx.x; i < W * 256; i += BLOCK) (&s.whist[0][0])[i] = 0;
-  __syncthreads();
-
-  uint64_t k[IPT];
-  uint32_t o[IPT], sd[IPT], dg[IPT], rk[IPT];
-#pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const uint32_t i = wid * IPW + j * kWave + lane;
-    const bool v = i < nact;
-    k[j] = 0;
-    o[j] = 0;
-    sd[j] = 0;
-    uint32_t d = 0;
-    if (v) {
-      k[j] = s.key[i];
-      o[j] = s.off[i];
-      sd[j] = s.sid[i];
-      d = which < 8 ? (uint32_t)(k[j] >> (8 * which)) & 0xFFu
-                    : (which == 8 ? sd[j] & 0xFFu : sd[j] >> 8);
-    }
-    dg[j] = d;
-    rk[j] = 0;
-    if ((uint32_t)(wid * IPW + j * kWave) < nact) {  // wave-uniform
-      const uint64_t live = __ballot(v);
-      const uint64_t m = match_digit<8>(d, live);
-      if (v) {
-        const uint32_t base = s.whist[wid][d];
-        rk[j] = base + (uint32_t)__popcll(m & lt);
-        if (lane == __builtin_ctzll(m)) s.whist[wid][d] = (uint16_t)(base + __popcll(m));
-      }
-    }
-  }
-  __syncthreads();
-
-  constexpr int DPT = 256 >= BLOCK ? 256 / BLOCK : 1;
-  uint32_t tot[DPT];
-  uint32_t sum = 0;
-#pragma unroll
-  for (int q = 0; q < DPT; ++q) {
-    const int d = threadIdx.x * DPT + q;
-    uint32_t r = 0;
-    if (d < 256) {
-      for (int w = 0; w < W; ++w) {
-        const uint32_t c = s.whist[w][d];
-        s.whist[w][d] = (uint16_t)r;
-        r += c;
-      }
-    }
-    tot[q] = r;
-    sum += r;
-  }
-  uint32_t total;
-  uint32_t run = block_excl_scan<BLOCK>(sum, s.scan, &total);
-#pragma unroll
-  for (int q = 0; q < DPT; ++q) {
-    const int d = threadIdx.x * DPT + q;
-    if (d < 256) {
-      for (int w = 0; w < W; ++w) s.whist[w][d] = (uint16_t)(s.whist[w][d] + run);
-    }
-    run += tot[q];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const uint32_t i = wid * IPW + j * kWave + lane;
-    if (i < nact) {
-      const uint32_t dst = s.whist[wid][dg[j]] + rk[j];
-      s.key[dst] = k[j];
-      s.off[dst] = o[j];
-      s.sid[dst] = (uint16_t)sd[j];
-    }
-  }
-  __syncthreads();
-}
-
-template <int BLOCK, int CAP>
-__global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
-  constexpr int IPT = CAP / BLOCK;
-  __shared__ LocalSmem<BLOCK, CAP> s;
-  const LocalSeg sg = segs[blockIdx.x];
-  const uint32_t n = sg.size;
-  const uint32_t gb = sg.begin;
-  const uint32_t* __restrict__ src = p.idx[sg.buf];
-  uint64_t* __restrict__ lcp = p.lcp;
-  uint32_t* __restrict__ out = p.out;
-
-  for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
-    s.off[i] = src[gb + i];
-    s.sid[i] = 0;
-  }
-  if (threadIdx.x == 0) {
-    s.ss_start[0] = 0;
-    s.ss_start[1] = (uint16_t)n;
-    s.ss_base[0] = 0;
-  }
-  uint32_t nact = n, nss = 1, maxss = n, depth = sg.depth;
-  __syncthreads();
-
-  while (true) {
-    // 1. Gather the next 8 bytes of every active string.
-    {
-      uint32_t o[IPT];
-      uint64_t k[IPT];
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) {
-        const uint32_t i = j * BLOCK + threadIdx.x;
-        o[j] = i < nact ? s.off[i] : 0u;
-      }
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) {
-        const uint32_t i = j * BLOCK + threadIdx.x;
-        if (i < nact) k[j] = load_key(p.text64, p.nwords, (uint64_t)o[j] + depth);
-      }
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) {
-        const uint32_t i = j * BLOCK + threadIdx.x;
-        if (i < nact) s.key[i] = k[j];
-      }
-    }
-    __syncthreads();
-
-    // 2. Which key bytes vary inside some sub-segment.
-    uint64_t vm = 0;
-    for (uint32_t i = threadIdx.x; i < nact; i += BLOCK)
-      vm |= s.key[i] ^ s.key[s.ss_start[s.sid[i]]];
-    vm = block_or<BLOCK>(vm, s.red);
-
-    // 3. Sort every sub-segment by key (stable).
-    if (vm) {
-      if (maxss <= 16) {
-        for (uint32_t g = threadIdx.x; g < nss; g += BLOCK) {
-          const uint32_t a = s.ss_start[g], e = s.ss_start[g + 1];
-          for (uint32_t i = a + 1; i < e; ++i) {
-            const uint64_t kk = s.key[i];
-            const uint32_t oo = s.off[i];
-            uint32_t j = i;
-            while (j > a && s.key[j - 1] > kk) {
-              s.key[j] = s.key[j - 1];
-              s.off[j] = s.off[j - 1];
-              --j;
-            }
-            s.key[j] = kk;
-            s.off[j] = oo;
-          }
-        }
-        __syncthreads();
-      } else {
-        for (int b = 0; b < 8; ++b)
-          if ((vm >> (8 * b)) & 0xFFu) local_radix_pass<BLOCK, CAP>(s, nact, b);
-        if (nss > 1) local_radix_pass<BLOCK, CAP>(s, nact, 8);
-        if (nss > 256) local_radix_pass<BLOCK, CAP>(s, nact, 9);
-      }
-    }
-
-    // 4. Split into equal-key groups; retire singletons and groups whose key
-    //    holds the NUL; emit the LCP of every pair split here.
-    uint32_t keep_off[IPT], keep_pos[IPT];
-    uint32_t flags = 0;  // bit 2j: member of a continuing group, bit 2j+1: its head
-    uint32_t nmem = 0, nhead = 0;
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const uint32_t i = threadIdx.x * IPT + j;
-      keep_off[j] = 0;
-      keep_pos[j] = 0;
-      if (i < nact) {
-        const uint64_t k = s.key[i];
-        const uint32_t g = s.sid[i];
-        const uint32_t a = s.ss_start[g], e = s.ss_start[g + 1];
-        const bool has_prev = i > a;
-        const uint64_t kp = has_prev ? s.key[i - 1] : 0;
-        const bool first = !has_prev || kp != k;
-        const bool last = i + 1 >= e || s.key[i + 1] != k;
-        const uint32_t pos = s.ss_base[g] + (i - a);
-        const uint32_t o = s.off[i];
-        if (lcp && has_prev && kp != k) lcp[gb + pos - 1] = (uint64_t)depth + key_lcp(kp, k);
-        const bool ended = (k & 0xFFu) == 0;
-        if (ended || (first && last)) {
-          out[gb + pos] = o;
-          if (lcp && ended && !first) lcp[gb + pos - 1] = (uint64_t)depth + key_len(k);
-        } else {
-          flags |= 1u << (2 * j);
-          nmem++;
-          if (first) {
-            flags |= 2u << (2 * j);
-            nhead++;
-          }
-        }
-        keep_off[j] = o;
-        keep_pos[j] = pos;
-      }
-    }
-    uint32_t tmem, thead;
-    uint32_t m = block_excl_scan<BLOCK>(nmem, s.scan, &tmem);
-    uint32_t h = block_excl_scan<BLOCK>(nhead, s.scan, &thead);
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      if (flags & (1u << (2 * j))) {
-        if (flags & (2u << (2 * j))) {
-          s.ss_start[h] = (uint16_t)m;
-          s.ss_base[h] = (uint16_t)keep_pos[j];
-          ++h;
-        }
-        s.off[m] = keep_off[j];
-        s.sid[m] = (uint16_t)(h - 1);
-        ++m;
-      }
-    }
-    nact = tmem;
-    nss = thead;
-    if (threadIdx.x == 0) s.ss_start[nss] = (uint16_t)nact;
-    __syncthreads();
-    if (nact == 0) break;
-    uint32_t mx = 0;
-    for (uint32_t g = threadIdx.x; g < nss; g += BLOCK)
-      mx = max(mx, (uint32_t)(s.ss_start[g + 1] - s.ss_start[g]));
-    maxss = block_max<BLOCK>(mx, s.scan);
-    depth += 8;
-    // Every string ends inside the text (reads past it return NUL), so no
-    // group can stay active beyond the text length; stop rather than spin.
-    if ((uint64_t)depth > p.nwords * 8 + 8) {
-      if (threadIdx.x == 0) atomicOr(&p.ctr->error, 32u);
-      break;
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Launch wrappers
 
 static inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 4096) {
@@ -729,9 +634,8 @@ static inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 4096)
   return (uint32_t)g;
 }
 
-hipError_t launch_init_segs(const Plan& p, uint32_t nsegs, uint32_t nbins, hipStream_t s) {
-  k_init_segs<<<grid_for((uint64_t)nsegs * nbins, 256), 256, 0, s>>>(p.segmm, p.seghist,
-                                                                      nsegs, nbins);
+hipError_t launch_init_segs(const Plan& p, uint32_t nsegs, hipStream_t s) {
+  k_init_segs<<<grid_for(2ull * nsegs, 256), 256, 0, s>>>(p.segmm, p.alpha, nsegs);
   return hipGetLastError();
 }
 
@@ -741,9 +645,17 @@ hipError_t launch_gather_minmax(const Plan& p, const BigSeg* segs, uint32_t ntil
   return hipGetLastError();
 }
 
-hipError_t launch_histogram(const Plan& p, const BigSeg* segs, uint32_t ntiles,
-                            uint32_t radix_bits, hipStream_t s) {
-  (void)segs;
+hipError_t launch_layout(const Plan& p, uint32_t nsegs, uint32_t radix_bits, hipStream_t s) {
+  if (radix_bits == 8)
+    k_layout<8><<<1, 1024, 0, s>>>(p, nsegs);
+  else if (radix_bits == 11)
+    k_layout<11><<<1, 1024, 0, s>>>(p, nsegs);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_histogram(const Plan& p, uint32_t ntiles, uint32_t radix_bits, hipStream_t s) {
   if (radix_bits == 8)
     k_histogram<8><<<ntiles, kTileThreads, 0, s>>>(p);
   else if (radix_bits == 11)
@@ -753,9 +665,16 @@ hipError_t launch_histogram(const Plan& p, const BigSeg* segs, uint32_t ntiles,
   return hipGetLastError();
 }
 
+hipError_t launch_flat_scan(const Plan& p, uint64_t max_len, hipStream_t s) {
+  const uint32_t blocks = (uint32_t)((max_len + kScanBlock - 1) / kScanBlock);
+  if (blocks) k_scan_reduce<<<blocks, kScanThreads, 0, s>>>(p);
+  k_scan_blocks<<<1, 1024, 0, s>>>(p);
+  if (blocks) k_scan_down<<<blocks, kScanThreads, 0, s>>>(p);
+  return hipGetLastError();
+}
+
 hipError_t launch_segscan(const Plan& p, const BigSeg* segs, BigSeg* next, uint32_t nsegs,
-                          uint32_t pass, uint32_t radix_bits, hipStream_t s) {
-  (void)pass;
+                          uint32_t radix_bits, hipStream_t s) {
   if (radix_bits == 8)
     k_segscan<8><<<nsegs, 256, 0, s>>>(p, segs, next);
   else if (radix_bits == 11)
@@ -765,13 +684,12 @@ hipError_t launch_segscan(const Plan& p, const BigSeg* segs, BigSeg* next, uint3
   return hipGetLastError();
 }
 
-hipError_t launch_scatter(const Plan& p, const BigSeg* segs, uint32_t ntiles, uint32_t pass,
-                          uint32_t epoch, uint32_t radix_bits, hipStream_t s) {
-  // `pass` carries the ticket base: tickets grow monotonically over a sort.
+hipError_t launch_scatter(const Plan& p, const BigSeg* segs, uint32_t ntiles,
+                          uint32_t radix_bits, hipStream_t s) {
   if (radix_bits == 8)
-    k_scatter<8><<<ntiles, kTileThreads, 0, s>>>(p, segs, pass, epoch);
+    k_scatter<8><<<ntiles, kTileThreads, 0, s>>>(p, segs);
   else if (radix_bits == 11)
-    k_scatter<11><<<ntiles, kTileThreads, 0, s>>>(p, segs, pass, epoch);
+    k_scatter<11><<<ntiles, kTileThreads, 0, s>>>(p, segs);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -785,24 +703,6 @@ hipError_t launch_boundary_lcp(const Plan& p, uint32_t first, hipStream_t s) {
 hipError_t launch_retire(const Plan& p, uint32_t nretire, hipStream_t s) {
   if (!nretire) return hipSuccess;
   k_retire<<<grid_for(nretire, 1, 8192), 256, 0, s>>>(p, nretire);
-  return hipGetLastError();
-}
-
-hipError_t launch_local(const Plan& p, int cls, uint32_t nsegs, hipStream_t s) {
-  if (!nsegs) return hipSuccess;
-  switch (cls) {
-    case 0:
-      k_local<64, 64><<<nsegs, 64, 0, s>>>(p, p.local[0]);
-      break;
-    case 1:
-      k_local<256, 1024><<<nsegs, 256, 0, s>>>(p, p.local[1]);
-      break;
-    case 2:
-      k_local<1024, 8192><<<nsegs, 1024, 0, s>>>(p, p.local[2]);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
   return hipGetLastError();
 }
 

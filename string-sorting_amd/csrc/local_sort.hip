@@ -1,0 +1,434 @@
+// Workgroup-local finish of the MI355X string sort (gfx950, wave64).
+//
+// One workgroup takes one segment of at most CAP strings (offsets into the
+// text, all equal on their first `depth` bytes) and sorts it completely in
+// LDS.  The segment is kept as an ordered list of active sub-segments (groups
+// of strings still equal so far), each with its own depth, and rounds repeat:
+//
+//   1. gather  the next kLocalWords 8-byte keys (32 bytes) of every active
+//              string at its sub-segment's depth, big-endian and NUL-padded
+//              like multikey_cache's cache (src/multikey_cache.cpp:107-125);
+//   2. choose  per sub-segment the first of those keys that differs between
+//              two members or holds a NUL; keys before it are equal for the
+//              whole group and are skipped.  A group equal on all 32 bytes
+//              with no NUL just moves 32 bytes deeper;
+//   3. sort    every sub-segment by its chosen key, stably: LSD radix passes
+//              over the key bytes that vary inside some sub-segment, then
+//              over the sub-segment id (wave match-ranking with per-wave LDS
+//              digit counters); when no sub-segment exceeds 16 strings each is
+//              insertion-sorted by one thread;
+//   4. split   into equal-key groups.  A singleton, or a group whose key holds
+//              the NUL (identical strings), is final -- the retirement rule
+//              of multikey_cache (src/multikey_cache.cpp:207-211) and of
+//              msd_CE2's bucket 0 (src/msd_ce.cpp:168-173).  Every pair split
+//              here gets its LCP (src/mergesort_lcp.cpp:88-100); survivors are
+//              compacted in order into the next round's list.
+//
+// Items are laid out per wave: each round the nact active strings are spread
+// as wave w -> [w*ipw, (w+1)*ipw), lane l, slot j -> w*ipw + 64*j + l, so all
+// waves share the work and per-slot LDS addresses are immediate offsets.
+
+#include "gss_internal.h"
+
+namespace gss {
+
+constexpr int kLocalWords = 4;
+
+template <int BLOCK, int CAP>
+struct LocalSmem {
+  static constexpr int W = BLOCK / kWave;
+  static constexpr int SS = CAP / 2 + 2;
+  uint64_t key[CAP];
+  uint32_t off[CAP];
+  uint16_t sid[CAP];
+  uint32_t ss_depth[SS];  // byte depth the sub-segment's next key starts at
+  uint16_t ss_start[SS];  // start in the active list (entry nss = nact)
+  uint16_t ss_base[SS];   // start in the segment's final order
+  uint8_t ss_stop[(SS + 3) & ~3];  // bit q: key q varies or holds a NUL in the group
+  uint16_t whist[W][256];
+  uint64_t edge[W][kLocalWords];  // keys of each wave's last item
+  uint32_t wvm[W];   // per wave: varying key bytes | 0x100 if a group is > 16
+  uint32_t wtot[W];  // per wave: survivors | their group heads << 16
+  uint32_t scan[W + 1];
+};
+
+// Lane i receives lane i-1's value, lane 0 receives 0 (DPP wave_shr:1).
+__device__ __forceinline__ uint64_t wave_shr1_64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_update_dpp(0u, (uint32_t)v, 0x138, 0xF, 0xF, false);
+  const uint32_t hi =
+      __builtin_amdgcn_update_dpp(0u, (uint32_t)(v >> 32), 0x138, 0xF, 0xF, false);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t lane63_64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, kWave - 1);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), kWave - 1);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Bit b set when byte b (0 = least significant) of x is nonzero.
+__device__ __forceinline__ uint32_t nonzero_bytes(uint64_t x) {
+  const uint64_t t = ((x & 0x7F7F7F7F7F7F7F7Full) + 0x7F7F7F7F7F7F7F7Full) | x;
+  const uint64_t h = t & 0x8080808080808080ull;  // top bit of each nonzero byte
+  return (uint32_t)((h * 0x0002040810204081ull) >> 56);
+}
+
+// One stable LSD pass over the active list by an 8-bit digit: key byte
+// `which` (0 = least significant) or, for which = 8 / 9, the low / high byte
+// of the sub-segment id.
+template <int BLOCK, int CAP>
+__device__ __forceinline__ void local_radix_pass(LocalSmem<BLOCK, CAP>& s, uint32_t nact,
+                                                 uint32_t base, uint32_t jn, int which) {
+  constexpr int W = BLOCK / kWave;
+  constexpr int IPT = CAP / BLOCK;
+  const int wid = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const uint64_t lt = lanemask_lt();
+
+  for (int i = threadIdx.x; i < W * 256; i += BLOCK) (&s.whist[0][0])[i] = 0;
+  __syncthreads();
+
+  uint64_t k[IPT];
+  uint32_t o[IPT], sd[IPT], rk[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    k[j] = 0;
+    o[j] = 0;
+    sd[j] = 0;
+    rk[j] = 0;
+    if ((uint32_t)j < jn) {
+      const uint32_t i = base + j * kWave;
+      const bool v = i < nact;
+      uint32_t d = 0;
+      if (v) {
+        k[j] = s.key[i];
+        o[j] = s.off[i];
+        sd[j] = s.sid[i];
+        d = which < 8 ? (uint32_t)(k[j] >> (8 * which)) & 0xFFu
+                      : (which == 8 ? sd[j] & 0xFFu : sd[j] >> 8);
+      }
+      const uint64_t m = match_digit<8>(d, __ballot(v));
+      if (v) {
+        const uint32_t b = s.whist[wid][d];
+        rk[j] = (d << 16) | (b + (uint32_t)__popcll(m & lt));
+        if (lane == __builtin_ctzll(m)) s.whist[wid][d] = (uint16_t)(b + __popcll(m));
+      }
+    }
+  }
+  __syncthreads();
+
+  // Digit-major, then wave order: exclusive starts of every (digit, wave).
+  constexpr int DPT = 256 >= BLOCK ? 256 / BLOCK : 1;
+  uint32_t tot[DPT];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int q = 0; q < DPT; ++q) {
+    const int d = threadIdx.x * DPT + q;
+    uint32_t r = 0;
+    if (d < 256) {
+      for (int w = 0; w < W; ++w) {
+        const uint32_t c = s.whist[w][d];
+        s.whist[w][d] = (uint16_t)r;
+        r += c;
+      }
+    }
+    tot[q] = r;
+    sum += r;
+  }
+  uint32_t total;
+  uint32_t run = block_excl_scan<BLOCK>(sum, s.scan, &total);
+#pragma unroll
+  for (int q = 0; q < DPT; ++q) {
+    const int d = threadIdx.x * DPT + q;
+    if (d < 256) {
+      for (int w = 0; w < W; ++w) s.whist[w][d] = (uint16_t)(s.whist[w][d] + run);
+    }
+    run += tot[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    if ((uint32_t)j < jn && base + j * kWave < nact) {
+      const uint32_t dst = s.whist[wid][rk[j] >> 16] + (rk[j] & 0xFFFFu);
+      s.key[dst] = k[j];
+      s.off[dst] = o[j];
+      s.sid[dst] = (uint16_t)sd[j];
+    }
+  }
+  __syncthreads();
+}
+
+template <int BLOCK, int CAP>
+__global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
+  constexpr int W = BLOCK / kWave;
+  constexpr int IPT = CAP / BLOCK;
+  constexpr int KW = kLocalWords;
+  constexpr int kClass = CAP <= kLocalClassCap[0] ? 0 : (CAP <= kLocalClassCap[1] ? 1 : 2);
+  constexpr uint64_t kSkipKey = ~0ull;  // key of a group that only moves deeper
+  static_assert(CAP % BLOCK == 0 && (CAP / W) % kWave == 0, "layout");
+  __shared__ LocalSmem<BLOCK, CAP> s;
+  const LocalSeg sg = segs[blockIdx.x];
+  const uint32_t n = sg.size;
+  const uint32_t gb = sg.begin;
+  const uint32_t* __restrict__ src = p.idx[sg.buf];
+  uint64_t* __restrict__ lcp = p.lcp;
+  uint32_t* __restrict__ out = p.out;
+  const uint32_t wid = threadIdx.x / kWave;
+  const uint32_t lane = lane_id();
+  const uint64_t lt = lanemask_lt();
+
+  for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+    s.off[i] = src[gb + i];
+    s.sid[i] = 0;
+  }
+  if (threadIdx.x == 0) {
+    s.ss_start[0] = 0;
+    s.ss_start[1] = (uint16_t)n;
+    s.ss_base[0] = 0;
+    s.ss_depth[0] = sg.depth;
+    s.ss_stop[0] = 0;
+  }
+  uint32_t nact = n, nss = 1, rounds = 0;
+  __syncthreads();
+
+  while (true) {
+    ++rounds;
+    if (threadIdx.x == 0)
+      atomicAdd(&p.ctr->local_round_strings[kClass], (unsigned long long)nact);
+    // Per-wave share of this round's active strings, a multiple of 64.
+    const uint32_t ipw = (((nact + W - 1) / W) + kWave - 1) & ~(uint32_t)(kWave - 1);
+    const uint32_t jn = ipw / kWave;
+    const uint32_t base = wid * ipw + lane;
+
+    // 1. Gather the next KW keys of every active string.
+    uint64_t kw[IPT][KW];
+    uint32_t gv[IPT];  // sub-segment of the item (0xFFFF when inactive)
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+#pragma unroll
+      for (int q = 0; q < KW; ++q) kw[j][q] = 0;
+      gv[j] = 0xFFFFu;
+      if ((uint32_t)j < jn) {
+        const uint32_t i = base + j * kWave;
+        const bool v = i < nact;
+        const uint32_t g = v ? s.sid[i] : 0u;
+        const uint32_t o = v ? s.off[i] : 0u;
+        gv[j] = v ? g : 0xFFFFu;
+        load_keys<KW>(p.text64, p.nwords, (uint64_t)o + s.ss_depth[g], kw[j]);
+      }
+    }
+    if (lane == kWave - 1) {
+#pragma unroll
+      for (int j = 0; j < IPT; ++j)
+        if ((uint32_t)j + 1 == jn) {
+#pragma unroll
+          for (int q = 0; q < KW; ++q) s.edge[wid][q] = kw[j][q];
+        }
+    }
+    __syncthreads();
+
+    // 2. Per sub-segment: which keys differ between adjacent members (two
+    //    members differ somewhere only if some adjacent pair does) or hold a
+    //    NUL.  Also note whether any sub-segment exceeds 16 strings.
+    bool big = false;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      if ((uint32_t)j < jn) {
+        const uint32_t i = base + j * kWave;
+        uint32_t stop = 0;
+        const bool same = gv[j] != 0xFFFFu && i > 0 && s.sid[i > 0 ? i - 1 : 0] == gv[j];
+#pragma unroll
+        for (int q = 0; q < KW; ++q) {
+          const uint64_t up = wave_shr1_64(kw[j][q]);
+          const uint64_t wrap = j > 0 ? lane63_64(kw[j > 0 ? j - 1 : 0][q])
+                                      : (wid > 0 ? s.edge[wid > 0 ? wid - 1 : 0][q] : 0ull);
+          const uint64_t prev = lane ? up : wrap;
+          if ((same && kw[j][q] != prev) || (kw[j][q] & 0xFFu) == 0) stop |= 1u << q;
+        }
+        if (gv[j] != 0xFFFFu) {
+          if (stop) atomicOr((uint32_t*)&s.ss_stop[0] + (gv[j] >> 2), stop << (8 * (gv[j] & 3)));
+          big = big || (s.ss_start[gv[j] + 1] - s.ss_start[gv[j]]) > 16;
+        }
+      }
+    }
+    const bool wave_big = __ballot(big) != 0;
+    __syncthreads();
+
+    // 3. Each string's key: the first key its group stops at, reloaded (its
+    //    lines were just fetched, so this hits the cache); a group that does
+    //    not stop gets kSkipKey.
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      if ((uint32_t)j < jn && gv[j] != 0xFFFFu) {
+        const uint32_t i = base + j * kWave;
+        const uint32_t g = gv[j];
+        const uint32_t e = __builtin_ctz(s.ss_stop[g] | (1u << KW));
+        uint64_t k = kSkipKey;
+        if (e < (uint32_t)KW)
+          k = load_key(p.text64, p.nwords, (uint64_t)s.off[i] + s.ss_depth[g] + 8 * e);
+        s.key[i] = k;
+      }
+    }
+    __syncthreads();
+
+    // 4. Bytes of the chosen keys that vary inside some sub-segment.
+    uint32_t vm = 0;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      if ((uint32_t)j < jn) {
+        const uint32_t i = base + j * kWave;
+        uint32_t nz = 0;
+        if (gv[j] != 0xFFFFu && i > 0 && s.sid[i - 1] == gv[j])
+          nz = nonzero_bytes(s.key[i] ^ s.key[i - 1]);
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+          if (__ballot((nz >> b) & 1u)) vm |= 1u << b;
+      }
+    }
+    if (lane == 0) s.wvm[wid] = vm | (wave_big ? 0x100u : 0u);
+    __syncthreads();
+    vm = 0;
+    for (int w = 0; w < W; ++w) vm |= s.wvm[w];
+    big = (vm & 0x100u) != 0;
+    vm &= 0xFFu;
+
+    // 5. Sort every sub-segment by its key (stable).
+    if (vm) {
+      if (!big) {
+        for (uint32_t g = threadIdx.x; g < nss; g += BLOCK) {
+          const uint32_t a = s.ss_start[g], en = s.ss_start[g + 1];
+          for (uint32_t i = a + 1; i < en; ++i) {
+            const uint64_t kk = s.key[i];
+            const uint32_t oo = s.off[i];
+            uint32_t j = i;
+            while (j > a && s.key[j - 1] > kk) {
+              s.key[j] = s.key[j - 1];
+              s.off[j] = s.off[j - 1];
+              --j;
+            }
+            s.key[j] = kk;
+            s.off[j] = oo;
+          }
+        }
+        __syncthreads();
+      } else {
+        for (int b = 0; b < 8; ++b)
+          if ((vm >> b) & 1u) local_radix_pass<BLOCK, CAP>(s, nact, base, jn, b);
+        if (nss > 1) local_radix_pass<BLOCK, CAP>(s, nact, base, jn, 8);
+        if (nss > 256) local_radix_pass<BLOCK, CAP>(s, nact, base, jn, 9);
+      }
+    }
+
+    // 6. Split into equal-key groups; retire singletons and groups whose key
+    //    holds the NUL; emit the LCP of every pair split here.  Survivors are
+    //    compacted in order: wave-local ranks from ballots, wave bases from
+    //    the per-wave totals.
+    uint32_t keep_off[IPT], keep_pos[IPT], keep_dep[IPT], rank[IPT];
+    uint32_t wm = 0, wh = 0;  // members / heads of this wave so far
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      keep_off[j] = 0;
+      keep_pos[j] = 0;
+      keep_dep[j] = 0;
+      rank[j] = 0x8000u;
+      if ((uint32_t)j < jn) {
+        const uint32_t i = base + j * kWave;
+        bool member = false, head = false;
+        if (i < nact) {
+          const uint64_t k = s.key[i];
+          const uint32_t g = s.sid[i];
+          const uint32_t a = s.ss_start[g], en = s.ss_start[g + 1];
+          const uint32_t e = __builtin_ctz(s.ss_stop[g] | (1u << KW));
+          const uint32_t dep = s.ss_depth[g] + 8 * e;  // depth of key k
+          const bool has_prev = i > a;
+          const uint64_t kp = has_prev ? s.key[i - 1] : 0;
+          const bool first = !has_prev || kp != k;
+          const bool last = i + 1 >= en || s.key[i + 1] != k;
+          const uint32_t pos = s.ss_base[g] + (i - a);
+          const uint32_t o = s.off[i];
+          if (lcp && has_prev && kp != k) lcp[gb + pos - 1] = (uint64_t)dep + key_lcp(kp, k);
+          const bool ended = e < (uint32_t)KW && (k & 0xFFu) == 0;
+          if (ended || (first && last)) {
+            out[gb + pos] = o;
+            if (lcp && ended && !first) lcp[gb + pos - 1] = (uint64_t)dep + key_len(k);
+          } else {
+            member = true;
+            head = first;
+          }
+          keep_off[j] = o;
+          keep_pos[j] = pos;
+          keep_dep[j] = e < (uint32_t)KW ? dep + 8 : dep;  // e == KW: dep = depth + 32
+        }
+        const uint64_t bm = __ballot(member);
+        const uint64_t bh = __ballot(head);
+        // Heads of this wave up to and including this item's group head.
+        const uint32_t hr = wh + (uint32_t)__popcll(bh & lt) + (head ? 1u : 0u);
+        rank[j] = (wm + (uint32_t)__popcll(bm & lt)) | (head ? 0x4000u : 0u) |
+                  (member ? 0u : 0x8000u) | (hr << 16);
+        wm += (uint32_t)__popcll(bm);
+        wh += (uint32_t)__popcll(bh);
+      }
+    }
+    if (lane == 0) s.wtot[wid] = wm | (wh << 16);
+    __syncthreads();
+    uint32_t mbase = 0, tot = 0;
+    for (uint32_t w = 0; w < (uint32_t)W; ++w) {
+      const uint32_t t = s.wtot[w];
+      if (w < wid) mbase += t;
+      tot += t;
+    }
+    const uint32_t hbase = mbase >> 16;
+    mbase &= 0xFFFFu;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      if ((uint32_t)j < jn && !(rank[j] & 0x8000u)) {
+        const uint32_t mi = mbase + (rank[j] & 0x3FFFu);
+        // A member whose head sits in an earlier wave has hr == 0: its group
+        // is the last one started before this wave (index hbase - 1).
+        const uint32_t g = hbase + (rank[j] >> 16) - 1;
+        s.off[mi] = keep_off[j];
+        s.sid[mi] = (uint16_t)g;
+        if (rank[j] & 0x4000u) {
+          s.ss_start[g] = (uint16_t)mi;
+          s.ss_base[g] = (uint16_t)keep_pos[j];
+          s.ss_depth[g] = keep_dep[j];
+          s.ss_stop[g] = 0;
+        }
+      }
+    }
+    nact = tot & 0xFFFFu;
+    nss = tot >> 16;
+    if (threadIdx.x == 0) s.ss_start[nss] = (uint16_t)nact;
+    __syncthreads();
+    if (nact == 0) {
+      if (threadIdx.x == 0) atomicMax(&p.ctr->local_rounds_max, rounds);
+      break;
+    }
+    // Every round moves each group >= 8 bytes deeper and no string reaches
+    // past the text, so the rounds are bounded; stop rather than spin.
+    if ((uint64_t)rounds > p.nwords + 2) {
+      if (threadIdx.x == 0) atomicOr(&p.ctr->error, 32u);
+      break;
+    }
+  }
+}
+
+hipError_t launch_local(const Plan& p, int cls, uint32_t nsegs, hipStream_t s) {
+  if (!nsegs) return hipSuccess;
+  switch (cls) {
+    case 0:
+      k_local<64, 64><<<nsegs, 64, 0, s>>>(p, p.local[0]);
+      break;
+    case 1:
+      k_local<256, 1024><<<nsegs, 256, 0, s>>>(p, p.local[1]);
+      break;
+    case 2:
+      k_local<1024, 8192><<<nsegs, 1024, 0, s>>>(p, p.local[2]);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace gss

@@ -33,6 +33,7 @@ struct Workspace {
   Counters* h_ctr = nullptr;
   BigSeg* h_big = nullptr;
   Tile* h_tiles = nullptr;
+  SegAux* h_aux = nullptr;
   LocalSeg* h_local = nullptr;
 };
 
@@ -40,7 +41,6 @@ struct DeviceCtx {
   bool ready = false;
   hipStream_t stream = nullptr;
   Workspace ws;
-  uint32_t epoch = 0;
 };
 
 struct ProfTotal {
@@ -57,7 +57,7 @@ struct Pending {
 
 std::recursive_mutex g_mu;
 DeviceCtx g_dev[kMaxDevices];
-gss_tuning g_tuning = {8192u, 8u, kTile};
+gss_tuning g_tuning = {8192u, 11u, kTile};
 gss_stats g_stats;
 bool g_prof = false;
 std::map<std::string, ProfTotal> g_prof_totals;
@@ -103,10 +103,12 @@ void free_workspace(Workspace& w) {
   if (w.base) (void)hipFree(w.base);
   if (w.h_big) (void)hipHostFree(w.h_big);
   if (w.h_tiles) (void)hipHostFree(w.h_tiles);
+  if (w.h_aux) (void)hipHostFree(w.h_aux);
   if (w.h_local) (void)hipHostFree(w.h_local);
   w.base = nullptr;
   w.h_big = nullptr;
   w.h_tiles = nullptr;
+  w.h_aux = nullptr;
   w.h_local = nullptr;
   w.bytes = 0;
   w.n_cap = 0;
@@ -143,10 +145,12 @@ int ensure_workspace(DeviceCtx& c, uint64_t n, uint32_t rb, uint32_t t) {
       {(void**)&p.big[0], sizeof(BigSeg) * cap_big},
       {(void**)&p.big[1], sizeof(BigSeg) * cap_big},
       {(void**)&p.segmm, 16 * cap_big},
-      {(void**)&p.seghist, 4 * cap_big * nb},
       {(void**)&p.bstart, 4 * cap_big * nb},
+      {(void**)&p.alpha, 32},
       {(void**)&p.tiles, sizeof(Tile) * cap_tiles},
-      {(void**)&p.status, 8 * cap_tiles * nb},
+      {(void**)&p.aux, sizeof(SegAux) * cap_big},
+      {(void**)&p.flat, 4 * (cap_tiles * nb + 1)},
+      {(void**)&p.flat_blk, 4 * ((cap_tiles * nb) / kScanBlock + 2)},
       {(void**)&p.local[0], sizeof(LocalSeg) * cap_local[0]},
       {(void**)&p.local[1], sizeof(LocalSeg) * cap_local[1]},
       {(void**)&p.local[2], sizeof(LocalSeg) * cap_local[2]},
@@ -179,11 +183,8 @@ int ensure_workspace(DeviceCtx& c, uint64_t n, uint32_t rb, uint32_t t) {
   for (int i = 0; i < 3; ++i) w.cap_local[i] = p.cap_local[i] = (uint32_t)cap_local[i];
   GSS_HIP(hipHostMalloc((void**)&w.h_big, sizeof(BigSeg) * cap_big, hipHostMallocDefault));
   GSS_HIP(hipHostMalloc((void**)&w.h_tiles, sizeof(Tile) * cap_tiles, hipHostMallocDefault));
+  GSS_HIP(hipHostMalloc((void**)&w.h_aux, sizeof(SegAux) * cap_big, hipHostMallocDefault));
   GSS_HIP(hipHostMalloc((void**)&w.h_local, sizeof(LocalSeg) * 4, hipHostMallocDefault));
-  // Look-back words carry a pass epoch; start from a clean array.
-  GSS_HIP(launch_fill_u64(p.status, cap_tiles * nb, 0, c.stream));
-  GSS_HIP(hipStreamSynchronize(c.stream));
-  c.epoch = 0;
   return GSS_OK;
 }
 
@@ -299,7 +300,7 @@ int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t
   } else {
     w.h_big[0] = BigSeg{0, (uint32_t)n, 0, 0};
     GSS_HIP(hipMemcpyAsync(p.big[0], w.h_big, sizeof(BigSeg), hipMemcpyHostToDevice, st));
-    uint32_t nbig = 1, cur = 0, ticket = 0, bound_first = 0, pass = 0;
+    uint32_t nbig = 1, cur = 0, bound_first = 0, pass = 0;
     // Each pass either splits a segment or moves it >= 1 byte deeper, and no
     // string reaches past the text, so the pass count is bounded.
     const uint64_t max_passes = p.nwords * 8 + 64;
@@ -310,29 +311,30 @@ int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t
       for (uint32_t i = 0; i < nbig; ++i) {
         const BigSeg& sg = w.h_big[i];
         active += sg.size;
+        const uint32_t first = ntiles;
         for (uint32_t o = 0; o < sg.size; o += kTile) {
           if (ntiles >= p.cap_tiles) return fail(GSS_E_INTERNAL, "tile table overflow");
-          w.h_tiles[ntiles++] = Tile{i, sg.begin + o, std::min<uint32_t>(kTile, sg.size - o),
-                                     o == 0 ? 1u : 0u};
+          w.h_tiles[ntiles] = Tile{i, sg.begin + o, std::min<uint32_t>(kTile, sg.size - o),
+                                   ntiles - first};
+          ++ntiles;
         }
+        w.h_aux[i] = SegAux{0, ntiles - first, 0, 0};
       }
       if (pass < GSS_MAX_PASSES) g_stats.active[pass] = active;
       GSS_HIP(hipMemcpyAsync(p.tiles, w.h_tiles, sizeof(Tile) * ntiles, hipMemcpyHostToDevice,
                              st));
-      if (c->epoch == 65535u) {  // epochs wrap: clear stale look-back words
-        GSS_HIP(launch_fill_u64(p.status, (uint64_t)p.cap_tiles * nb, 0, st));
-        c->epoch = 0;
-      }
-      c->epoch += 1;
+      GSS_HIP(hipMemcpyAsync(p.aux, w.h_aux, sizeof(SegAux) * nbig, hipMemcpyHostToDevice, st));
       const BigSeg* segs = p.big[cur];
       BigSeg* next = p.big[cur ^ 1];
-      GSS_LAUNCH("init_segs", nbig, launch_init_segs(p, nbig, nb, st));
+      GSS_LAUNCH("init_segs", nbig, launch_init_segs(p, nbig, st));
       GSS_LAUNCH("gather", active, launch_gather_minmax(p, segs, ntiles, st));
-      GSS_LAUNCH("histogram", active, launch_histogram(p, segs, ntiles, rb, st));
+      GSS_LAUNCH("layout", nbig, launch_layout(p, nbig, rb, st));
+      GSS_LAUNCH("histogram", active, launch_histogram(p, ntiles, rb, st));
+      GSS_LAUNCH("flat_scan", (uint64_t)ntiles * nb,
+                 launch_flat_scan(p, (uint64_t)ntiles * nb, st));
       GSS_HIP(hipMemsetAsync(&p.ctr->big_next, 0, sizeof(uint32_t), st));
-      GSS_LAUNCH("segscan", nbig, launch_segscan(p, segs, next, nbig, pass, rb, st));
-      GSS_LAUNCH("scatter", active, launch_scatter(p, segs, ntiles, ticket, c->epoch, rb, st));
-      ticket += ntiles;
+      GSS_LAUNCH("segscan", nbig, launch_segscan(p, segs, next, nbig, rb, st));
+      GSS_LAUNCH("scatter", active, launch_scatter(p, segs, ntiles, rb, st));
       if (lcp) GSS_LAUNCH("boundary", 0, launch_boundary_lcp(p, bound_first, st));
       GSS_HIP(hipMemcpyAsync(w.h_ctr, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
       GSS_HIP(hipMemcpyAsync(w.h_big, next, sizeof(BigSeg) * p.cap_big, hipMemcpyDeviceToHost,
@@ -355,16 +357,26 @@ int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t
   }
   if (nretire) GSS_LAUNCH("retire", nretire, launch_retire(p, nretire, st));
   static const char* kLocalNames[3] = {"local64", "local1024", "local8192"};
+  long local_tok[3] = {-1, -1, -1};  // pending profile entries of the local launches
   for (int cls = 2; cls >= 0; --cls) {
     if (nlocal[cls] > w.cap_local[cls])
       return fail(GSS_E_INTERNAL, "local segment list overflow");
-    if (nlocal[cls])
-      GSS_LAUNCH(kLocalNames[cls], local_strings[cls], launch_local(p, cls, nlocal[cls], st));
+    if (nlocal[cls]) {
+      const size_t mark = g_pending.size();
+      GSS_LAUNCH(kLocalNames[cls], 0, launch_local(p, cls, nlocal[cls], st));
+      if (g_pending.size() > mark) local_tok[cls] = (long)mark;
+    }
     g_stats.local_segments += nlocal[cls];
     g_stats.local_strings += local_strings[cls];
   }
   GSS_HIP(hipMemcpyAsync(w.h_ctr, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
   GSS_HIP(hipStreamSynchronize(st));
+  // A local launch's unit of work is one active string in one key round.
+  for (int cls = 0; cls < 3; ++cls) {
+    g_stats.local_round_strings += w.h_ctr->local_round_strings[cls];
+    if (local_tok[cls] >= 0) g_pending[local_tok[cls]].items = w.h_ctr->local_round_strings[cls];
+  }
+  g_stats.local_rounds_max = w.h_ctr->local_rounds_max;
   prof_collect();
   if (w.h_ctr->error)
     return fail(GSS_E_INTERNAL,

@@ -42,6 +42,7 @@ class Stats(C.Structure):
         ("active", C.c_uint64 * GSS_MAX_PASSES),
         ("local_strings", C.c_uint64),
         ("local_rounds_max", C.c_uint32),
+        ("local_round_strings", C.c_uint64),
     ]
 
 
