@@ -46,7 +46,7 @@ struct SegAux {
   uint32_t flat_base;
   uint32_t ntiles;
   uint32_t nbu;
-  uint32_t pad;
+  uint32_t tile0;  // index of the segment's first tile in the pass
 };
 
 // Entries per block of the flat exclusive scan.
@@ -88,8 +88,16 @@ struct Counters {
   uint32_t flat_len;     // entries of the current pass's flat count array
   uint32_t error;        // nonzero: a list overflowed or a loop bound was hit
   uint32_t local_strings[3];  // strings handed to each local size class
-  uint32_t local_rounds_max;  // most key rounds any local segment needed
-  unsigned long long local_round_strings[3];  // sum over local rounds of active strings
+};
+
+// Statistics of the local kernels, spread over kStatSlots slots (block b
+// updates slot b % kStatSlots, each slot in its own 256-byte line) so that
+// thousands of blocks do not serialise on one address; the host combines
+// the slots.  v[slot][c] for c < 3: sum over class-c rounds of the active
+// strings; v[slot][3]: most rounds one segment needed.
+constexpr int kStatSlots = 64;
+struct LocalStats {
+  unsigned long long v[kStatSlots][32];
 };
 
 // Size classes of the local kernel: (threads, capacity).
@@ -116,6 +124,10 @@ struct Plan {
   Retire* retire;
   Boundary* bounds;
   Counters* ctr;
+  LocalStats* lstats;
+  uint64_t* tile_mm;     // per tile: min, max key
+  uint32_t* tile_alpha;  // per tile: 256-bit byte-presence set
+  uint32_t* seg_alpha;   // per BigSeg: 256-bit byte-presence set
   uint32_t local_threshold;
   uint32_t cap_big, cap_retire, cap_bounds, cap_tiles;
   uint32_t cap_local[3];
@@ -147,17 +159,40 @@ __device__ __forceinline__ uint64_t funnel(uint64_t lo, uint64_t hi, uint32_t sh
   return (lo >> sh) | ((hi << 1) << (63 - sh));
 }
 
+// Two text words as one 16-byte load (8-byte alignment suffices on gfx950).
+struct __attribute__((aligned(8))) TextPair {
+  uint64_t lo, hi;
+};
+
+// Aligned text words [w, w+NW): 16-byte loads when all lie inside the text,
+// word by word (0 beyond the end) only in the last NW words of the text.
+template <int NW>
+__device__ __forceinline__ void text_words(const uint64_t* __restrict__ text64, uint64_t nwords,
+                                           uint64_t w, uint64_t (&raw)[NW]) {
+  if (w + NW <= nwords) {
+#pragma unroll
+    for (int q = 0; q + 1 < NW; q += 2) {
+      const TextPair pr = *reinterpret_cast<const TextPair*>(text64 + w + q);
+      raw[q] = pr.lo;
+      raw[q + 1] = pr.hi;
+    }
+    if (NW & 1) raw[NW - 1] = text64[w + NW - 1];
+  } else {
+#pragma unroll
+    for (int q = 0; q < NW; ++q) raw[q] = text_word(text64, nwords, w + q);
+  }
+}
+
 // 8 bytes of the text starting at byte `pos`, big-endian (first byte most
 // significant), zero after the first NUL; words beyond the text read as 0.
 // Same packing as fill_cache in src/multikey_cache.cpp:107-125.
 __device__ __forceinline__ uint64_t load_key(const uint64_t* __restrict__ text64,
                                              uint64_t nwords, uint64_t pos) {
-  const uint64_t w = pos >> 3;
   const uint32_t sh = (uint32_t)(pos & 7) * 8;
-  const uint64_t lo = text_word(text64, nwords, w);
-  const uint64_t hi = text_word(text64, nwords, w + 1);
+  uint64_t raw[2];
+  text_words<2>(text64, nwords, pos >> 3, raw);
   bool ended;
-  return __builtin_bswap64(cut_at_nul(funnel(lo, hi, sh), &ended));
+  return __builtin_bswap64(cut_at_nul(funnel(raw[0], raw[1], sh), &ended));
 }
 
 // KW consecutive keys of the string bytes at `pos`: key q covers bytes
@@ -166,11 +201,9 @@ __device__ __forceinline__ uint64_t load_key(const uint64_t* __restrict__ text64
 template <int KW>
 __device__ __forceinline__ void load_keys(const uint64_t* __restrict__ text64, uint64_t nwords,
                                           uint64_t pos, uint64_t (&out)[KW]) {
-  const uint64_t w = pos >> 3;
   const uint32_t sh = (uint32_t)(pos & 7) * 8;
   uint64_t raw[KW + 1];
-#pragma unroll
-  for (int q = 0; q <= KW; ++q) raw[q] = text_word(text64, nwords, w + q);
+  text_words<KW + 1>(text64, nwords, pos >> 3, raw);
   bool done = false;
 #pragma unroll
   for (int q = 0; q < KW; ++q) {
@@ -309,9 +342,9 @@ __device__ __forceinline__ uint32_t block_max(uint32_t v, uint32_t* scratch) {
 // ---------------------------------------------------------------------------
 // Launch wrappers (kernels.hip).  All return hipError_t of the launch.
 
-hipError_t launch_init_segs(const Plan& p, uint32_t nsegs, hipStream_t s);
 hipError_t launch_gather_minmax(const Plan& p, const BigSeg* segs, uint32_t ntiles,
                                 hipStream_t s);
+hipError_t launch_segreduce(const Plan& p, uint32_t nsegs, hipStream_t s);
 hipError_t launch_layout(const Plan& p, uint32_t nsegs, uint32_t radix_bits, hipStream_t s);
 hipError_t launch_histogram(const Plan& p, uint32_t ntiles, uint32_t radix_bits,
                             hipStream_t s);

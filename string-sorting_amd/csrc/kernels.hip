@@ -37,13 +37,6 @@ namespace gss {
 // ---------------------------------------------------------------------------
 // Small helpers
 
-__global__ void k_init_segs(uint64_t* segmm, uint32_t* alpha, uint32_t nsegs) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t0 < 8) alpha[t0] = 0;
-  for (uint64_t i = t0; i < 2ull * nsegs; i += stride) segmm[i] = (i & 1) ? 0ull : ~0ull;
-}
-
 __global__ void k_fill_u64(uint64_t* dst, uint64_t n, uint64_t v) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -175,7 +168,9 @@ __device__ __forceinline__ uint64_t match_digit_n(uint32_t d, uint64_t live, uin
 }
 
 // ---------------------------------------------------------------------------
-// Global pass, step 1: gather keys, per-segment min/max, pass alphabet.
+// Global pass, step 1: gather keys; per tile the min/max key and the set of
+// byte values seen (no atomics: thousands of tiles would serialise on one
+// segment's words).
 
 __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const BigSeg* segs) {
   __shared__ uint64_t s_mn[kTileThreads / kWave], s_mx[kTileThreads / kWave];
@@ -217,13 +212,13 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const Bi
     s_mx[wid] = mx;
   }
   __syncthreads();
-  // The pass alphabet: 256 presence flags folded into 8 words by ballots.
+  // 256 presence flags folded into 8 words by ballots.
   if (threadIdx.x < 256) {
     const uint64_t bal = __ballot(s_present[threadIdx.x] != 0);
     if (lane_id() == 0) {
-      const uint32_t w = threadIdx.x / 32;
-      if ((uint32_t)bal) atomicOr(&p.alpha[w], (uint32_t)bal);
-      if ((uint32_t)(bal >> 32)) atomicOr(&p.alpha[w + 1], (uint32_t)(bal >> 32));
+      uint32_t* ta = p.tile_alpha + 8ull * blockIdx.x + threadIdx.x / 32;
+      ta[0] = (uint32_t)bal;
+      ta[1] = (uint32_t)(bal >> 32);
     }
   }
   if (threadIdx.x == 0) {
@@ -231,8 +226,44 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const Bi
       mn = s_mn[w] < mn ? s_mn[w] : mn;
       mx = s_mx[w] > mx ? s_mx[w] : mx;
     }
-    atomicMin((unsigned long long*)&p.segmm[2 * t.seg], (unsigned long long)mn);
-    atomicMax((unsigned long long*)&p.segmm[2 * t.seg + 1], (unsigned long long)mx);
+    p.tile_mm[2ull * blockIdx.x] = mn;
+    p.tile_mm[2ull * blockIdx.x + 1] = mx;
+  }
+}
+
+// One workgroup per segment: min/max key and byte set over its tiles.
+__global__ __launch_bounds__(256) void k_segreduce(Plan p) {
+  __shared__ uint64_t s_mn[256 / kWave], s_mx[256 / kWave];
+  __shared__ uint32_t s_al[8];
+  const SegAux ax = p.aux[blockIdx.x];
+  if (threadIdx.x < 8) s_al[threadIdx.x] = 0;
+  uint64_t mn = ~0ull, mx = 0ull;
+  uint32_t al = 0;
+  for (uint32_t t = threadIdx.x; t < ax.ntiles; t += 256) {
+    const uint64_t a = p.tile_mm[2ull * (ax.tile0 + t)], b = p.tile_mm[2ull * (ax.tile0 + t) + 1];
+    mn = a < mn ? a : mn;
+    mx = b > mx ? b : mx;
+  }
+  // Byte sets: thread (t, w) ORs word w of tiles t, t+32, ...
+  for (uint32_t i = threadIdx.x; i < 8 * ax.ntiles; i += 256) al |= p.tile_alpha[8ull * ax.tile0 + i];
+  mn = wave_min64(mn);
+  mx = wave_max64(mx);
+  const int wid = threadIdx.x / kWave;
+  if (lane_id() == 0) {
+    s_mn[wid] = mn;
+    s_mx[wid] = mx;
+  }
+  __syncthreads();
+  if (al) atomicOr(&s_al[threadIdx.x & 7], al);  // 256 % 8 == 0: word index = i & 7
+  __syncthreads();
+  if (threadIdx.x < 8) p.seg_alpha[8ull * blockIdx.x + threadIdx.x] = s_al[threadIdx.x];
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 256 / kWave; ++w) {
+      mn = s_mn[w] < mn ? s_mn[w] : mn;
+      mx = s_mx[w] > mx ? s_mx[w] : mx;
+    }
+    p.segmm[2ull * blockIdx.x] = mn;
+    p.segmm[2ull * blockIdx.x + 1] = mx;
   }
 }
 
@@ -242,7 +273,16 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const Bi
 template <int RB>
 __global__ __launch_bounds__(1024) void k_layout(Plan p, uint32_t nsegs) {
   __shared__ uint32_t s_scan[1024 / kWave + 1];
-  const uint32_t A = alpha_size(p.alpha);
+  __shared__ uint32_t s_al[8];
+  // The pass alphabet: union of the segments' byte sets.
+  if (threadIdx.x < 8) s_al[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t al = 0;
+  for (uint32_t i = threadIdx.x; i < 8 * nsegs; i += 1024) al |= p.seg_alpha[i];
+  if (al) atomicOr(&s_al[threadIdx.x & 7], al);  // 1024 % 8 == 0: word index = i & 7
+  __syncthreads();
+  if (threadIdx.x < 8) p.alpha[threadIdx.x] = s_al[threadIdx.x];
+  const uint32_t A = alpha_size(s_al);
   uint32_t carry = 0;
   for (uint32_t s0 = 0; s0 < nsegs; s0 += 1024) {
     const uint32_t s = s0 + threadIdx.x;
@@ -443,22 +483,28 @@ __global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, Big
   __syncthreads();
   const uint32_t dfirst = s_first;
   uint32_t* bs = p.bstart + (uint64_t)si * NB;
+
+  // Classify every bucket.  kind: 0 final or empty, 1..3 local class 0..2,
+  // 4 next pass, 5 run of identical strings (nul = where they end).
+  uint32_t kind[DPT], nulp[DPT];
+  uint32_t nb = 0, nr = 0, ng = 0, nl[3] = {0, 0, 0}, sl[3] = {0, 0, 0};
 #pragma unroll
   for (uint32_t q = 0; q < DPT; ++q) {
     const uint32_t d = threadIdx.x * DPT + q;
     const uint32_t c = cnt[q];
+    kind[q] = 0;
+    nulp[q] = 0;
     if (d >= ds.nbu || c == 0) continue;
     // The bucket's strings ended if one of its digit characters is NUL (the
     // common bytes before fb0 hold none: a NUL there would make min == max).
     bool ended = false;
-    uint32_t nul = 0;
     if (has_nul) {
       uint32_t rest = d, scale = ds.nbu;
       for (uint32_t ch = 0; ch < ds.k; ++ch) {
         scale /= ds.A;
         if (rest / scale == 0) {
           ended = true;
-          nul = ds.fb0 + ch;
+          nulp[q] = ds.fb0 + ch;
           break;
         }
         rest %= scale;
@@ -466,18 +512,87 @@ __global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, Big
     }
     const bool fin = ended || c == 1;
     bs[d] = fin ? 1u : 0u;
+    if (d != dfirst) ++nb;
+    if (ended) {
+      if (c > 1) {
+        kind[q] = 5;
+        nr += (c + kRetireChunk - 1) / kRetireChunk;
+      }
+    } else if (!fin) {
+      if (c <= p.local_threshold) {
+        const uint32_t cls = c <= kLocalClassCap[0] ? 0 : (c <= kLocalClassCap[1] ? 1 : 2);
+        kind[q] = 1 + cls;
+        ++nl[cls];
+        sl[cls] += c;
+      } else {
+        kind[q] = 4;
+        ++ng;
+      }
+    }
+  }
+
+  // One reservation per list for the whole segment: same-address atomics
+  // from every bucket would serialise.
+  __shared__ uint32_t s_scan[256 / kWave + 1];
+  __shared__ uint32_t s_base[6];
+  uint32_t tb, tr, tg, t0, t1, t2;
+  uint32_t ob = block_excl_scan<256>(nb, s_scan, &tb);
+  uint32_t orr = block_excl_scan<256>(nr, s_scan, &tr);
+  uint32_t og = block_excl_scan<256>(ng, s_scan, &tg);
+  uint32_t ol[3];
+  ol[0] = block_excl_scan<256>(nl[0], s_scan, &t0);
+  ol[1] = block_excl_scan<256>(nl[1], s_scan, &t1);
+  ol[2] = block_excl_scan<256>(nl[2], s_scan, &t2);
+  uint32_t ts0, ts1, ts2;
+  (void)block_excl_scan<256>(sl[0], s_scan, &ts0);
+  (void)block_excl_scan<256>(sl[1], s_scan, &ts1);
+  (void)block_excl_scan<256>(sl[2], s_scan, &ts2);
+  if (threadIdx.x == 0) {
+    const uint32_t tl[3] = {t0, t1, t2}, ts[3] = {ts0, ts1, ts2};
+    s_base[0] = tb ? atomicAdd(&p.ctr->boundary, tb) : 0u;
+    s_base[1] = tr ? atomicAdd(&p.ctr->retire, tr) : 0u;
+    s_base[5] = tg ? atomicAdd(&p.ctr->big_next, tg) : 0u;
+    for (int c = 0; c < 3; ++c) {
+      s_base[2 + c] = tl[c] ? atomicAdd(&p.ctr->local[c], tl[c]) : 0u;
+      if (ts[c]) atomicAdd(&p.ctr->local_strings[c], ts[c]);
+    }
+    if (s_base[0] + tb > p.cap_bounds) atomicOr(&p.ctr->error, 16u);
+    if (s_base[1] + tr > p.cap_retire) atomicOr(&p.ctr->error, 8u);
+    if (s_base[5] + tg > p.cap_big) atomicOr(&p.ctr->error, 4u);
+    for (int c = 0; c < 3; ++c)
+      if (s_base[2 + c] + tl[c] > p.cap_local[c]) atomicOr(&p.ctr->error, 2u);
+  }
+  __syncthreads();
+  ob += s_base[0];
+  orr += s_base[1];
+  og += s_base[5];
+  for (int c = 0; c < 3; ++c) ol[c] += s_base[2 + c];
+#pragma unroll
+  for (uint32_t q = 0; q < DPT; ++q) {
+    const uint32_t d = threadIdx.x * DPT + q;
+    const uint32_t c = cnt[q];
+    if (d >= ds.nbu || c == 0) continue;
     const uint32_t cb = sg.begin + st[q];
     if (d != dfirst) {
-      const uint32_t slot = atomicAdd(&p.ctr->boundary, 1u);
-      if (slot < p.cap_bounds)
-        p.bounds[slot] = Boundary{cb - 1, sg.depth, nbuf, 0u};
-      else
-        atomicOr(&p.ctr->error, 16u);
+      if (ob < p.cap_bounds) p.bounds[ob] = Boundary{cb - 1, sg.depth, nbuf, 0u};
+      ++ob;
     }
-    if (ended) {
-      if (c > 1) push_retire(p, cb, c, nbuf, sg.depth + nul, 0u);
-    } else if (!fin) {
-      push_child(p, next, cb, c, sg.depth + fb, nbuf);
+    const uint32_t kd = kind[q];
+    if (kd == 5) {
+      const uint32_t nch = (c + kRetireChunk - 1) / kRetireChunk;
+      for (uint32_t ch = 0; ch < nch; ++ch, ++orr) {
+        if (orr >= p.cap_retire) continue;
+        const uint32_t b = cb + ch * kRetireChunk;
+        p.retire[orr] = Retire{b, min(kRetireChunk, c - ch * kRetireChunk), cb + c - 1, nbuf,
+                               sg.depth + nulp[q], 0u};
+      }
+    } else if (kd == 4) {
+      if (og < p.cap_big) next[og] = BigSeg{cb, c, sg.depth + fb, nbuf};
+      ++og;
+    } else if (kd >= 1) {
+      const uint32_t slot = kd == 1 ? ol[0]++ : (kd == 2 ? ol[1]++ : ol[2]++);
+      if (slot < p.cap_local[kd - 1])
+        p.local[kd - 1][slot] = LocalSeg{cb, c, sg.depth + fb, nbuf};
     }
   }
 }
@@ -634,8 +749,8 @@ static inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 4096)
   return (uint32_t)g;
 }
 
-hipError_t launch_init_segs(const Plan& p, uint32_t nsegs, hipStream_t s) {
-  k_init_segs<<<grid_for(2ull * nsegs, 256), 256, 0, s>>>(p.segmm, p.alpha, nsegs);
+hipError_t launch_segreduce(const Plan& p, uint32_t nsegs, hipStream_t s) {
+  k_segreduce<<<nsegs, 256, 0, s>>>(p);
   return hipGetLastError();
 }
 

@@ -189,12 +189,12 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
     s.ss_stop[0] = 0;
   }
   uint32_t nact = n, nss = 1, rounds = 0;
+  unsigned long long round_strings = 0;
   __syncthreads();
 
   while (true) {
     ++rounds;
-    if (threadIdx.x == 0)
-      atomicAdd(&p.ctr->local_round_strings[kClass], (unsigned long long)nact);
+    round_strings += nact;
     // Per-wave share of this round's active strings, a multiple of 64.
     const uint32_t ipw = (((nact + W - 1) / W) + kWave - 1) & ~(uint32_t)(kWave - 1);
     const uint32_t jn = ipw / kWave;
@@ -400,16 +400,18 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
     nss = tot >> 16;
     if (threadIdx.x == 0) s.ss_start[nss] = (uint16_t)nact;
     __syncthreads();
-    if (nact == 0) {
-      if (threadIdx.x == 0) atomicMax(&p.ctr->local_rounds_max, rounds);
-      break;
-    }
+    if (nact == 0) break;
     // Every round moves each group >= 8 bytes deeper and no string reaches
     // past the text, so the rounds are bounded; stop rather than spin.
     if ((uint64_t)rounds > p.nwords + 2) {
       if (threadIdx.x == 0) atomicOr(&p.ctr->error, 32u);
       break;
     }
+  }
+  if (threadIdx.x == 0) {
+    unsigned long long* slot = p.lstats->v[blockIdx.x % kStatSlots];
+    atomicAdd(&slot[kClass], round_strings);
+    atomicMax(&slot[3], (unsigned long long)rounds);
   }
 }
 

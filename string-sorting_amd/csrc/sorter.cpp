@@ -35,6 +35,7 @@ struct Workspace {
   Tile* h_tiles = nullptr;
   SegAux* h_aux = nullptr;
   LocalSeg* h_local = nullptr;
+  LocalStats* h_lstats = nullptr;
 };
 
 struct DeviceCtx {
@@ -105,6 +106,8 @@ void free_workspace(Workspace& w) {
   if (w.h_tiles) (void)hipHostFree(w.h_tiles);
   if (w.h_aux) (void)hipHostFree(w.h_aux);
   if (w.h_local) (void)hipHostFree(w.h_local);
+  if (w.h_lstats) (void)hipHostFree(w.h_lstats);
+  w.h_lstats = nullptr;
   w.base = nullptr;
   w.h_big = nullptr;
   w.h_tiles = nullptr;
@@ -157,6 +160,10 @@ int ensure_workspace(DeviceCtx& c, uint64_t n, uint32_t rb, uint32_t t) {
       {(void**)&p.retire, sizeof(Retire) * cap_retire},
       {(void**)&p.bounds, sizeof(Boundary) * cap_bounds},
       {(void**)&p.ctr, sizeof(Counters)},
+      {(void**)&p.lstats, sizeof(LocalStats)},
+      {(void**)&p.tile_mm, 16 * cap_tiles},
+      {(void**)&p.tile_alpha, 32 * cap_tiles},
+      {(void**)&p.seg_alpha, 32 * cap_big},
   };
   size_t total = 0;
   for (auto& it : items) total += align_up(it.bytes, 256);
@@ -185,6 +192,7 @@ int ensure_workspace(DeviceCtx& c, uint64_t n, uint32_t rb, uint32_t t) {
   GSS_HIP(hipHostMalloc((void**)&w.h_tiles, sizeof(Tile) * cap_tiles, hipHostMallocDefault));
   GSS_HIP(hipHostMalloc((void**)&w.h_aux, sizeof(SegAux) * cap_big, hipHostMallocDefault));
   GSS_HIP(hipHostMalloc((void**)&w.h_local, sizeof(LocalSeg) * 4, hipHostMallocDefault));
+  GSS_HIP(hipHostMalloc((void**)&w.h_lstats, sizeof(LocalStats), hipHostMallocDefault));
   return GSS_OK;
 }
 
@@ -286,6 +294,7 @@ int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t
     p.idx[0] = w.idx_copy;
   }
   GSS_LAUNCH("init", 0, launch_zero_counters(p, st));
+  GSS_HIP(launch_fill_u64(reinterpret_cast<uint64_t*>(p.lstats), sizeof(LocalStats) / 8, 0, st));
   const uint32_t nb = 1u << rb;
 
   uint32_t nlocal[3] = {0, 0, 0};
@@ -318,7 +327,7 @@ int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t
                                    ntiles - first};
           ++ntiles;
         }
-        w.h_aux[i] = SegAux{0, ntiles - first, 0, 0};
+        w.h_aux[i] = SegAux{0, ntiles - first, 0, first};
       }
       if (pass < GSS_MAX_PASSES) g_stats.active[pass] = active;
       GSS_HIP(hipMemcpyAsync(p.tiles, w.h_tiles, sizeof(Tile) * ntiles, hipMemcpyHostToDevice,
@@ -326,8 +335,8 @@ int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t
       GSS_HIP(hipMemcpyAsync(p.aux, w.h_aux, sizeof(SegAux) * nbig, hipMemcpyHostToDevice, st));
       const BigSeg* segs = p.big[cur];
       BigSeg* next = p.big[cur ^ 1];
-      GSS_LAUNCH("init_segs", nbig, launch_init_segs(p, nbig, st));
       GSS_LAUNCH("gather", active, launch_gather_minmax(p, segs, ntiles, st));
+      GSS_LAUNCH("segreduce", nbig, launch_segreduce(p, nbig, st));
       GSS_LAUNCH("layout", nbig, launch_layout(p, nbig, rb, st));
       GSS_LAUNCH("histogram", active, launch_histogram(p, ntiles, rb, st));
       GSS_LAUNCH("flat_scan", (uint64_t)ntiles * nb,
@@ -370,13 +379,19 @@ int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t
     g_stats.local_strings += local_strings[cls];
   }
   GSS_HIP(hipMemcpyAsync(w.h_ctr, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
+  GSS_HIP(hipMemcpyAsync(w.h_lstats, p.lstats, sizeof(LocalStats), hipMemcpyDeviceToHost, st));
   GSS_HIP(hipStreamSynchronize(st));
   // A local launch's unit of work is one active string in one key round.
-  for (int cls = 0; cls < 3; ++cls) {
-    g_stats.local_round_strings += w.h_ctr->local_round_strings[cls];
-    if (local_tok[cls] >= 0) g_pending[local_tok[cls]].items = w.h_ctr->local_round_strings[cls];
+  uint64_t round_strings[3] = {0, 0, 0}, rounds_max = 0;
+  for (int slot = 0; slot < kStatSlots; ++slot) {
+    for (int cls = 0; cls < 3; ++cls) round_strings[cls] += w.h_lstats->v[slot][cls];
+    rounds_max = std::max<uint64_t>(rounds_max, w.h_lstats->v[slot][3]);
   }
-  g_stats.local_rounds_max = w.h_ctr->local_rounds_max;
+  for (int cls = 0; cls < 3; ++cls) {
+    g_stats.local_round_strings += round_strings[cls];
+    if (local_tok[cls] >= 0) g_pending[local_tok[cls]].items = round_strings[cls];
+  }
+  g_stats.local_rounds_max = (uint32_t)rounds_max;
   prof_collect();
   if (w.h_ctr->error)
     return fail(GSS_E_INTERNAL,
