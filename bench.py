@@ -44,6 +44,29 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# Library profile names -> device symbol fragments in the rocprofv3 tables.
+SYMBOLS = {"gather": "k_gather_minmax", "histogram": "k_histogram", "scatter": "k_scatter",
+           "segscan": "k_segscan", "local64": "k_local<64, 64>",
+           "local1024": "k_local<256, 1024>", "local8192": "k_local<1024, 8192>"}
+PROFILE = ROOT / "profiles" / "url_like_10M_kernels.md"
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` (calibrated FETCH_SIZE + WRITE_SIZE)
+    from the committed rocprofv3 summary, or None when it has no row."""
+    sym = SYMBOLS.get(kernel)
+    if not sym or not PROFILE.exists():
+        return None
+    for line in PROFILE.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 4 and sym in cells[0]:
+            try:
+                return (float(cells[2]) + float(cells[3])) * 2**20
+            except ValueError:
+                return None
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,6 +77,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-e2e", action="store_true", help="skip the pointer-path timing")
     ap.add_argument("--cpu-runs", type=int, default=2)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed steps, copy 1 GiB device to device twice: the "
+                         "reference dispatch that scales FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
 
 
@@ -135,6 +161,13 @@ def main():
     dt = time.perf_counter() - t0
     gss.profile_enable(False)
     st = gss.stats()
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     ms_per_step = dt / args.steps * 1e3
     value = ss.n * args.steps / dt / 1e6
@@ -155,8 +188,11 @@ def main():
     d = prof[dominant]
     per_launch_bytes = BYTES_PER_ROUND_STRING * d["items"] / d["launches"]
     achieved = per_launch_bytes / (d["ms_avg"] * 1e-3) / 1e9
+    traffic = measured_traffic(dominant)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                "traffic_source": str(PROFILE.relative_to(ROOT)) if traffic else None,
+                "traffic_gbps": traffic / (d["ms_avg"] * 1e-3) / 1e9 if traffic else None,
                 "alg_bytes_per_launch": per_launch_bytes, "avg_launch_ms": d["ms_avg"]}
 
     res = {

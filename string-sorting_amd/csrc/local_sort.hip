@@ -254,18 +254,18 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
     const bool wave_big = __ballot(big) != 0;
     __syncthreads();
 
-    // 3. Each string's key: the first key its group stops at, reloaded (its
-    //    lines were just fetched, so this hits the cache); a group that does
-    //    not stop gets kSkipKey.
+    // 3. Each string's key: the first key its group stops at, still in
+    //    registers (reloading it would miss L2: a round's lines do not stay
+    //    there); a group that does not stop gets kSkipKey.
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
       if ((uint32_t)j < jn && gv[j] != 0xFFFFu) {
         const uint32_t i = base + j * kWave;
-        const uint32_t g = gv[j];
-        const uint32_t e = __builtin_ctz(s.ss_stop[g] | (1u << KW));
+        const uint32_t e = __builtin_ctz(s.ss_stop[gv[j]] | (1u << KW));
         uint64_t k = kSkipKey;
-        if (e < (uint32_t)KW)
-          k = load_key(p.text64, p.nwords, (uint64_t)s.off[i] + s.ss_depth[g] + 8 * e);
+#pragma unroll
+        for (int q = 0; q < KW; ++q)
+          if ((uint32_t)q == e) k = kw[j][q];
         s.key[i] = k;
       }
     }
