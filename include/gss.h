@@ -66,6 +66,38 @@ int gss_sort_device32(const unsigned char *d_text, uint64_t text_bytes,
 		      uint64_t *d_lcp, uint64_t n, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Batched sort (no reference counterpart: the reference sorts one input per
+ * call).  nreq independent requests share one text and one offset array:
+ * request r owns offsets [req_begin[r], req_begin[r+1]) (host array of
+ * nreq+1 non-decreasing entries, req_begin[0] = 0, req_begin[nreq] = total
+ * strings) and is sorted on its own, exactly as gss_sort_device32 would sort
+ * it alone; all requests go through the same kernel launches.  d_lcp, when
+ * given, receives each request's LCPs at its positions; the entry between
+ * two requests (index req_begin[r+1]-1) is left untouched. */
+int gss_sort_batch_device32(const unsigned char *d_text, uint64_t text_bytes,
+			    const uint32_t *d_off_in, uint32_t *d_off_out,
+			    uint64_t *d_lcp, const uint64_t *req_begin,
+			    uint32_t nreq, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Sort contexts (no reference counterpart).  The entry points above use one
+ * default context per device, so their calls are serialised.  A context owns
+ * a stream and a device workspace; calls on different contexts run
+ * concurrently (for example from two host threads), which lets a server keep
+ * several batches on the GPU at once.  A context belongs to the device that
+ * was current when it was created. */
+typedef struct gss_ctx gss_ctx;
+
+gss_ctx *gss_ctx_create(void);
+void gss_ctx_destroy(gss_ctx *ctx);
+void *gss_ctx_stream(gss_ctx *ctx); /* the context's own hipStream_t */
+int gss_ctx_sort_batch_device32(gss_ctx *ctx, const unsigned char *d_text,
+				uint64_t text_bytes, const uint32_t *d_off_in,
+				uint32_t *d_off_out, uint64_t *d_lcp,
+				const uint64_t *req_begin, uint32_t nreq,
+				void *stream);
+
+/* ------------------------------------------------------------------------
  * Tuning knobs (no reference counterpart: the reference fixes its
  * thresholds at compile time, e.g. src/msd_ce.cpp:79 n < 32 and :216
  * n >= 0x10000).  Values outside the supported range are rejected with
@@ -98,7 +130,8 @@ typedef struct gss_stats {
 	uint64_t local_round_strings; /* sum over those rounds of active strings */
 } gss_stats;
 
-int gss_get_stats(gss_stats *out);
+int gss_get_stats(gss_stats *out);            /* last sort on this thread */
+int gss_ctx_get_stats(gss_ctx *ctx, gss_stats *out); /* last sort of ctx */
 
 /* Kernel timing: when enabled, every kernel launch is bracketed by HIP events
  * recorded on the stream the kernel runs on; totals per kernel accumulate

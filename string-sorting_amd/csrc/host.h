@@ -11,23 +11,12 @@ namespace gss {
 // Records `msg` as the calling thread's last error and returns `code`.
 int fail(int code, const std::string& msg);
 
-// Sorts n 32-bit offsets into `text` (device memory) into `out`; see
-// gss_sort_device32 in gss.h for the contract.  Thread-safe (serialised).
+// Sorts n 32-bit offsets into `text` (device memory) into `out` on the
+// current device's default context; see gss_sort_device32 in gss.h.
 int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
                   uint32_t* out, uint64_t* lcp, uint64_t n, hipStream_t stream);
 
-// The library's own stream on the current device (created on first use).
+// The default context's stream on the current device (created on first use).
 int library_stream(hipStream_t* out);
-
-// Profiling hooks: bracket a launch with events on `stream` when enabled.
-struct ProfToken {
-  hipEvent_t a = nullptr;
-  const char* name = nullptr;
-  uint64_t items = 0;
-};
-ProfToken prof_begin(const char* name, uint64_t items, hipStream_t stream);
-void prof_end(ProfToken& t, hipStream_t stream);
-// Collects the elapsed times of finished brackets (call after a sync).
-void prof_collect();
 
 }  // namespace gss

@@ -1,8 +1,10 @@
-// Device-resident MSD radix string sort: workspace, pass loop, statistics,
-// profiling and the C entry points of gss.h that do not touch host strings.
+// Device-resident MSD radix string sort: sort contexts (stream, workspace,
+// statistics), the pass loop, profiling, and the C entry points of gss.h
+// that do not touch host strings.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -28,20 +30,14 @@ struct Workspace {
   uint32_t threshold = 0;
   Plan plan{};
   uint32_t* idx_copy = nullptr;
-  uint32_t cap_local[3] = {0, 0, 0};
   // Pinned host mirrors.
   Counters* h_ctr = nullptr;
   BigSeg* h_big = nullptr;
   Tile* h_tiles = nullptr;
   SegAux* h_aux = nullptr;
-  LocalSeg* h_local = nullptr;
   LocalStats* h_lstats = nullptr;
-};
-
-struct DeviceCtx {
-  bool ready = false;
-  hipStream_t stream = nullptr;
-  Workspace ws;
+  LocalSeg* h_local = nullptr;  // initial local segments, 3 classes of cap_init
+  uint32_t cap_init = 0;
 };
 
 struct ProfTotal {
@@ -56,17 +52,44 @@ struct Pending {
   uint64_t items;
 };
 
-std::recursive_mutex g_mu;
-DeviceCtx g_dev[kMaxDevices];
+std::mutex g_mu;  // tuning, profile totals, default contexts
 gss_tuning g_tuning = {8192u, 11u, kTile};
-gss_stats g_stats;
-bool g_prof = false;
+std::atomic<bool> g_prof{false};
 std::map<std::string, ProfTotal> g_prof_totals;
-std::vector<Pending> g_pending;
-std::vector<hipEvent_t> g_event_pool;
 thread_local std::string g_last_error;
+thread_local gss_stats g_last_stats;
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+int local_class(uint64_t size) {
+  return size <= kLocalClassCap[0] ? 0 : (size <= kLocalClassCap[1] ? 1 : 2);
+}
+
+}  // namespace
+
+// One independent sorter: its own stream and workspace.  Calls on one
+// context are serialised; different contexts run concurrently.
+struct Context {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  Workspace ws;
+  std::mutex mu;
+  gss_stats stats{};
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> events;
+};
+
+}  // namespace gss
+
+struct gss_ctx {
+  gss::Context c;
+};
+
+namespace gss {
+
+namespace {
+
+gss_ctx* g_default[kMaxDevices] = {};
 
 #define GSS_HIP(call)                                                              \
   do {                                                                             \
@@ -75,24 +98,35 @@ size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
       return fail(GSS_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-int current_ctx(DeviceCtx** out) {
-  int dev = 0;
-  GSS_HIP(hipGetDevice(&dev));
-  if (dev < 0 || dev >= kMaxDevices) return fail(GSS_E_DEVICE, "device id out of range");
-  DeviceCtx& c = g_dev[dev];
-  if (!c.ready) {
-    GSS_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-    GSS_HIP(hipHostMalloc((void**)&c.ws.h_ctr, sizeof(Counters), hipHostMallocDefault));
-    c.ready = true;
-  }
-  *out = &c;
+int init_context(Context& c) {
+  GSS_HIP(hipGetDevice(&c.device));
+  GSS_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  GSS_HIP(hipHostMalloc((void**)&c.ws.h_ctr, sizeof(Counters), hipHostMallocDefault));
   return GSS_OK;
 }
 
-hipEvent_t take_event() {
-  if (!g_event_pool.empty()) {
-    hipEvent_t e = g_event_pool.back();
-    g_event_pool.pop_back();
+int default_context(Context** out) {
+  int dev = 0;
+  GSS_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(GSS_E_DEVICE, "device id out of range");
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_default[dev]) {
+    gss_ctx* h = new gss_ctx;
+    const int rc = init_context(h->c);
+    if (rc) {
+      delete h;
+      return rc;
+    }
+    g_default[dev] = h;
+  }
+  *out = &g_default[dev]->c;
+  return GSS_OK;
+}
+
+hipEvent_t take_event(Context& c) {
+  if (!c.events.empty()) {
+    hipEvent_t e = c.events.back();
+    c.events.pop_back();
     return e;
   }
   hipEvent_t e = nullptr;
@@ -100,26 +134,66 @@ hipEvent_t take_event() {
   return e;
 }
 
+struct ProfToken {
+  hipEvent_t a = nullptr;
+  const char* name = nullptr;
+  uint64_t items = 0;
+};
+
+ProfToken prof_begin(Context& c, const char* name, uint64_t items, hipStream_t stream) {
+  ProfToken t;
+  if (!g_prof.load(std::memory_order_relaxed)) return t;
+  t.a = take_event(c);
+  t.name = name;
+  t.items = items;
+  if (t.a) (void)hipEventRecord(t.a, stream);
+  return t;
+}
+
+void prof_end(Context& c, ProfToken& t, hipStream_t stream) {
+  if (!t.a) return;
+  hipEvent_t b = take_event(c);
+  if (!b) return;
+  (void)hipEventRecord(b, stream);
+  c.pending.push_back(Pending{t.a, b, t.name, t.items});
+}
+
+// Folds the finished brackets into the global totals (after a sync).
+void prof_collect(Context& c) {
+  if (c.pending.empty()) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto& p : c.pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      ProfTotal& tot = g_prof_totals[p.name];
+      tot.launches += 1;
+      tot.ms += ms;
+      tot.items += p.items;
+    }
+    c.events.push_back(p.a);
+    c.events.push_back(p.b);
+  }
+  c.pending.clear();
+}
+
 void free_workspace(Workspace& w) {
   if (w.base) (void)hipFree(w.base);
   if (w.h_big) (void)hipHostFree(w.h_big);
   if (w.h_tiles) (void)hipHostFree(w.h_tiles);
   if (w.h_aux) (void)hipHostFree(w.h_aux);
-  if (w.h_local) (void)hipHostFree(w.h_local);
   if (w.h_lstats) (void)hipHostFree(w.h_lstats);
-  w.h_lstats = nullptr;
   w.base = nullptr;
   w.h_big = nullptr;
   w.h_tiles = nullptr;
   w.h_aux = nullptr;
-  w.h_local = nullptr;
+  w.h_lstats = nullptr;
   w.bytes = 0;
   w.n_cap = 0;
 }
 
 // Sizes every device array for up to n strings, 2^rb digits and local
 // threshold t, and carves them out of one allocation.
-int ensure_workspace(DeviceCtx& c, uint64_t n, uint32_t rb, uint32_t t) {
+int ensure_workspace(Context& c, uint64_t n, uint32_t rb, uint32_t t) {
   Workspace& w = c.ws;
   if (w.base && n <= w.n_cap && rb == w.rb && t == w.threshold) return GSS_OK;
   GSS_HIP(hipStreamSynchronize(c.stream));
@@ -131,7 +205,7 @@ int ensure_workspace(DeviceCtx& c, uint64_t n, uint32_t rb, uint32_t t) {
   const uint64_t cap_tiles = ncap / kTile + cap_big + 2;
   const uint64_t cap_local[3] = {ncap / 2 + 2, ncap / (kLocalClassCap[0] + 1) + 2,
                                  ncap / (kLocalClassCap[1] + 1) + 2};
-  const uint64_t cap_retire = ncap / 2 + ncap / kRetireChunk + 8;
+  const uint64_t cap_retire = ncap + ncap / kRetireChunk + 8;  // single-string requests too
   const uint64_t cap_bounds = ncap + 2;
 
   struct Item {
@@ -187,101 +261,73 @@ int ensure_workspace(DeviceCtx& c, uint64_t n, uint32_t rb, uint32_t t) {
   p.cap_tiles = (uint32_t)cap_tiles;
   p.cap_retire = (uint32_t)cap_retire;
   p.cap_bounds = (uint32_t)cap_bounds;
-  for (int i = 0; i < 3; ++i) w.cap_local[i] = p.cap_local[i] = (uint32_t)cap_local[i];
+  for (int i = 0; i < 3; ++i) p.cap_local[i] = (uint32_t)cap_local[i];
   GSS_HIP(hipHostMalloc((void**)&w.h_big, sizeof(BigSeg) * cap_big, hipHostMallocDefault));
   GSS_HIP(hipHostMalloc((void**)&w.h_tiles, sizeof(Tile) * cap_tiles, hipHostMallocDefault));
   GSS_HIP(hipHostMalloc((void**)&w.h_aux, sizeof(SegAux) * cap_big, hipHostMallocDefault));
-  GSS_HIP(hipHostMalloc((void**)&w.h_local, sizeof(LocalSeg) * 4, hipHostMallocDefault));
   GSS_HIP(hipHostMalloc((void**)&w.h_lstats, sizeof(LocalStats), hipHostMallocDefault));
   return GSS_OK;
 }
 
-int local_class(uint64_t size) {
-  return size <= kLocalClassCap[0] ? 0 : (size <= kLocalClassCap[1] ? 1 : 2);
-}
-
-}  // namespace
-
-int fail(int code, const std::string& msg) {
-  g_last_error = msg;
-  return code;
-}
-
-int library_stream(hipStream_t* out) {
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
-  DeviceCtx* c = nullptr;
-  int rc = current_ctx(&c);
-  if (rc) return rc;
-  *out = c->stream;
+int ensure_init_lists(Workspace& w, uint32_t nreq) {
+  if (nreq <= w.cap_init) return GSS_OK;
+  if (w.h_local) (void)hipHostFree(w.h_local);
+  w.h_local = nullptr;
+  const uint32_t cap = std::max<uint32_t>(nreq, 64);
+  // Three classes of initial local segments, then the initial retire runs.
+  static_assert(sizeof(Retire) <= 2 * sizeof(LocalSeg), "layout of the initial lists");
+  GSS_HIP(hipHostMalloc((void**)&w.h_local, 5 * sizeof(LocalSeg) * cap, hipHostMallocDefault));
+  w.cap_init = cap;
   return GSS_OK;
 }
 
-ProfToken prof_begin(const char* name, uint64_t items, hipStream_t stream) {
-  ProfToken t;
-  if (!g_prof) return t;
-  t.a = take_event();
-  t.name = name;
-  t.items = items;
-  if (t.a) (void)hipEventRecord(t.a, stream);
-  return t;
-}
-
-void prof_end(ProfToken& t, hipStream_t stream) {
-  if (!g_prof || !t.a) return;
-  hipEvent_t b = take_event();
-  if (!b) return;
-  (void)hipEventRecord(b, stream);
-  g_pending.push_back(Pending{t.a, b, t.name, t.items});
-}
-
-void prof_collect() {
-  for (auto& p : g_pending) {
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
-      ProfTotal& tot = g_prof_totals[p.name];
-      tot.launches += 1;
-      tot.ms += ms;
-      tot.items += p.items;
-    }
-    g_event_pool.push_back(p.a);
-    g_event_pool.push_back(p.b);
-  }
-  g_pending.clear();
-}
-
-#define GSS_LAUNCH(name, items, call)            \
-  do {                                           \
-    ProfToken tok_ = prof_begin(name, items, st); \
-    GSS_HIP(call);                               \
-    prof_end(tok_, st);                          \
+#define GSS_LAUNCH(name, items, call)                \
+  do {                                               \
+    ProfToken tok_ = prof_begin(c, name, items, st); \
+    GSS_HIP(call);                                   \
+    prof_end(c, tok_, st);                           \
   } while (0)
 
-int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
-                  uint32_t* out, uint64_t* lcp, uint64_t n, hipStream_t stream) {
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
-  memset(&g_stats, 0, sizeof(g_stats));
-  g_stats.n = n;
-  g_stats.text_bytes = text_bytes;
+// Sorts the requests [req[r], req[r+1]) of the offset array independently
+// (req == nullptr: one request of n strings).  Called with c.mu held.
+int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
+             uint32_t* out, uint64_t* lcp, uint64_t n, const uint64_t* req, uint32_t nreq,
+             hipStream_t stream) {
+  gss_stats& stats = c.stats;
+  memset(&stats, 0, sizeof(stats));
+  stats.n = n;
+  stats.text_bytes = text_bytes;
   if (n == 0) return GSS_OK;
   if (!text || !in || !out) return fail(GSS_E_INVALID, "NULL text, input or output pointer");
   if (reinterpret_cast<uintptr_t>(text) & 7)
     return fail(GSS_E_INVALID, "the device text must be 8-byte aligned");
   if (n >= (1ull << 31))
     return fail(GSS_E_TOO_LARGE, "at most 2^31-1 strings per sort on the 32-bit path");
-  DeviceCtx* c = nullptr;
-  int rc = current_ctx(&c);
-  if (rc) return rc;
-  hipStream_t st = stream ? stream : c->stream;
-  if (n == 1) {
-    GSS_HIP(hipMemcpyAsync(out, in, 4, hipMemcpyDeviceToDevice, st));
-    GSS_HIP(hipStreamSynchronize(st));
-    return GSS_OK;
+  int dev = -1;
+  GSS_HIP(hipGetDevice(&dev));
+  if (dev != c.device) return fail(GSS_E_INVALID, "the context belongs to another device");
+  const uint64_t one[2] = {0, n};
+  if (!req) {
+    req = one;
+    nreq = 1;
   }
-  const uint32_t rb = g_tuning.radix_bits;
-  const uint32_t T = g_tuning.local_threshold;
-  rc = ensure_workspace(*c, n, rb, T);
+  if (req[0] != 0 || req[nreq] != n) return fail(GSS_E_INVALID, "requests must cover [0, n)");
+  for (uint32_t r = 0; r < nreq; ++r)
+    if (req[r + 1] < req[r]) return fail(GSS_E_INVALID, "request bounds must not decrease");
+  hipStream_t st = stream ? stream : c.stream;
+
+  gss_tuning tun;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    tun = g_tuning;
+  }
+  const uint32_t rb = tun.radix_bits;
+  const uint32_t T = tun.local_threshold;
+  int rc = ensure_workspace(c, n, rb, T);
   if (rc) return rc;
-  Workspace& w = c->ws;
+  rc = ensure_init_lists(c.ws, nreq);
+  if (rc) return rc;
+  Workspace& w = c.ws;
   Plan p = w.plan;
   p.text64 = reinterpret_cast<const uint64_t*>(text);
   p.nwords = (text_bytes + 7) / 8;
@@ -293,23 +339,46 @@ int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t
     GSS_HIP(hipMemcpyAsync(w.idx_copy, in, 4 * n, hipMemcpyDeviceToDevice, st));
     p.idx[0] = w.idx_copy;
   }
-  GSS_LAUNCH("init", 0, launch_zero_counters(p, st));
-  GSS_HIP(launch_fill_u64(reinterpret_cast<uint64_t*>(p.lstats), sizeof(LocalStats) / 8, 0, st));
   const uint32_t nb = 1u << rb;
 
-  uint32_t nlocal[3] = {0, 0, 0};
-  uint32_t local_strings[3] = {0, 0, 0};
-  uint32_t nretire = 0;
-  if (n <= T) {
-    const int cls = local_class(n);
-    w.h_local[0] = LocalSeg{0, (uint32_t)n, 0, 0};
-    GSS_HIP(hipMemcpyAsync(p.local[cls], w.h_local, sizeof(LocalSeg), hipMemcpyHostToDevice, st));
-    nlocal[cls] = 1;
-    local_strings[cls] = (uint32_t)n;
-  } else {
-    w.h_big[0] = BigSeg{0, (uint32_t)n, 0, 0};
-    GSS_HIP(hipMemcpyAsync(p.big[0], w.h_big, sizeof(BigSeg), hipMemcpyHostToDevice, st));
-    uint32_t nbig = 1, cur = 0, bound_first = 0, pass = 0;
+  // Initial segments: one per request, finished locally when small enough;
+  // a single string is final as it is (a retire run of one, nothing to fill).
+  Counters& h0 = *w.h_ctr;
+  memset(&h0, 0, sizeof(h0));
+  LocalSeg* init[3] = {w.h_local, w.h_local + w.cap_init, w.h_local + 2 * w.cap_init};
+  Retire* init_retire = reinterpret_cast<Retire*>(w.h_local + 3 * w.cap_init);
+  uint32_t nbig = 0;
+  for (uint32_t r = 0; r < nreq; ++r) {
+    const uint32_t b = (uint32_t)req[r], size = (uint32_t)(req[r + 1] - req[r]);
+    if (size == 0) continue;
+    if (size == 1) {
+      init_retire[h0.retire++] = Retire{b, 1, b, 0, 0, 1};
+    } else if (size <= T) {
+      const int cls = local_class(size);
+      init[cls][h0.local[cls]++] = LocalSeg{b, size, 0, 0};
+      h0.local_strings[cls] += size;
+    } else {
+      if (nbig >= p.cap_big) return fail(GSS_E_INTERNAL, "segment table overflow");
+      w.h_big[nbig++] = BigSeg{b, size, 0, 0};
+    }
+  }
+  for (int cls = 0; cls < 3; ++cls)
+    if (h0.local[cls])
+      GSS_HIP(hipMemcpyAsync(p.local[cls], init[cls], sizeof(LocalSeg) * h0.local[cls],
+                             hipMemcpyHostToDevice, st));
+  if (h0.retire)
+    GSS_HIP(hipMemcpyAsync(p.retire, init_retire, sizeof(Retire) * h0.retire,
+                           hipMemcpyHostToDevice, st));
+  if (nbig)
+    GSS_HIP(hipMemcpyAsync(p.big[0], w.h_big, sizeof(BigSeg) * nbig, hipMemcpyHostToDevice, st));
+  GSS_HIP(hipMemcpyAsync(p.ctr, &h0, sizeof(Counters), hipMemcpyHostToDevice, st));
+  GSS_HIP(launch_fill_u64(reinterpret_cast<uint64_t*>(p.lstats), sizeof(LocalStats) / 8, 0, st));
+
+  uint32_t nlocal[3] = {h0.local[0], h0.local[1], h0.local[2]};
+  uint32_t local_strings[3] = {h0.local_strings[0], h0.local_strings[1], h0.local_strings[2]};
+  uint32_t nretire = h0.retire;
+  if (nbig) {
+    uint32_t cur = 0, bound_first = 0, pass = 0;
     // Each pass either splits a segment or moves it >= 1 byte deeper, and no
     // string reaches past the text, so the pass count is bounded.
     const uint64_t max_passes = p.nwords * 8 + 64;
@@ -329,7 +398,7 @@ int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t
         }
         w.h_aux[i] = SegAux{0, ntiles - first, 0, first};
       }
-      if (pass < GSS_MAX_PASSES) g_stats.active[pass] = active;
+      if (pass < GSS_MAX_PASSES) stats.active[pass] = active;
       GSS_HIP(hipMemcpyAsync(p.tiles, w.h_tiles, sizeof(Tile) * ntiles, hipMemcpyHostToDevice,
                              st));
       GSS_HIP(hipMemcpyAsync(p.aux, w.h_aux, sizeof(SegAux) * nbig, hipMemcpyHostToDevice, st));
@@ -357,7 +426,7 @@ int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t
       cur ^= 1;
       ++pass;
     }
-    g_stats.global_passes = pass;
+    stats.global_passes = pass;
     for (int i = 0; i < 3; ++i) {
       nlocal[i] = w.h_ctr->local[i];
       local_strings[i] = w.h_ctr->local_strings[i];
@@ -368,15 +437,14 @@ int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t
   static const char* kLocalNames[3] = {"local64", "local1024", "local8192"};
   long local_tok[3] = {-1, -1, -1};  // pending profile entries of the local launches
   for (int cls = 2; cls >= 0; --cls) {
-    if (nlocal[cls] > w.cap_local[cls])
-      return fail(GSS_E_INTERNAL, "local segment list overflow");
+    if (nlocal[cls] > p.cap_local[cls]) return fail(GSS_E_INTERNAL, "local segment list overflow");
     if (nlocal[cls]) {
-      const size_t mark = g_pending.size();
+      const size_t mark = c.pending.size();
       GSS_LAUNCH(kLocalNames[cls], 0, launch_local(p, cls, nlocal[cls], st));
-      if (g_pending.size() > mark) local_tok[cls] = (long)mark;
+      if (c.pending.size() > mark) local_tok[cls] = (long)mark;
     }
-    g_stats.local_segments += nlocal[cls];
-    g_stats.local_strings += local_strings[cls];
+    stats.local_segments += nlocal[cls];
+    stats.local_strings += local_strings[cls];
   }
   GSS_HIP(hipMemcpyAsync(w.h_ctr, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
   GSS_HIP(hipMemcpyAsync(w.h_lstats, p.lstats, sizeof(LocalStats), hipMemcpyDeviceToHost, st));
@@ -388,15 +456,47 @@ int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t
     rounds_max = std::max<uint64_t>(rounds_max, w.h_lstats->v[slot][3]);
   }
   for (int cls = 0; cls < 3; ++cls) {
-    g_stats.local_round_strings += round_strings[cls];
-    if (local_tok[cls] >= 0) g_pending[local_tok[cls]].items = round_strings[cls];
+    stats.local_round_strings += round_strings[cls];
+    if (local_tok[cls] >= 0) c.pending[local_tok[cls]].items = round_strings[cls];
   }
-  g_stats.local_rounds_max = (uint32_t)rounds_max;
-  prof_collect();
+  stats.local_rounds_max = (uint32_t)rounds_max;
+  prof_collect(c);
   if (w.h_ctr->error)
     return fail(GSS_E_INTERNAL,
                 "device sort failed (code " + std::to_string(w.h_ctr->error) + ")");
   return GSS_OK;
+}
+
+int run_locked(Context& c, const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
+               uint32_t* out, uint64_t* lcp, uint64_t n, const uint64_t* req, uint32_t nreq,
+               hipStream_t stream) {
+  std::lock_guard<std::mutex> lk(c.mu);
+  const int rc = run_sort(c, text, text_bytes, in, out, lcp, n, req, nreq, stream);
+  g_last_stats = c.stats;
+  return rc;
+}
+
+}  // namespace
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int library_stream(hipStream_t* out) {
+  Context* c = nullptr;
+  const int rc = default_context(&c);
+  if (rc) return rc;
+  *out = c->stream;
+  return GSS_OK;
+}
+
+int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
+                  uint32_t* out, uint64_t* lcp, uint64_t n, hipStream_t stream) {
+  Context* c = nullptr;
+  const int rc = default_context(&c);
+  if (rc) return rc;
+  return run_locked(*c, text, text_bytes, in, out, lcp, n, nullptr, 0, stream);
 }
 
 }  // namespace gss
@@ -411,9 +511,61 @@ int gss_sort_device32(const unsigned char* d_text, uint64_t text_bytes, const ui
                        reinterpret_cast<hipStream_t>(stream));
 }
 
+int gss_sort_batch_device32(const unsigned char* d_text, uint64_t text_bytes,
+                            const uint32_t* d_off_in, uint32_t* d_off_out, uint64_t* d_lcp,
+                            const uint64_t* req_begin, uint32_t nreq, void* stream) {
+  if (!req_begin) return fail(GSS_E_INVALID, "NULL request table");
+  Context* c = nullptr;
+  const int rc = default_context(&c);
+  if (rc) return rc;
+  return run_locked(*c, d_text, text_bytes, d_off_in, d_off_out, d_lcp, req_begin[nreq],
+                    req_begin, nreq, reinterpret_cast<hipStream_t>(stream));
+}
+
+gss_ctx* gss_ctx_create(void) {
+  gss_ctx* h = new gss_ctx;
+  if (init_context(h->c)) {
+    delete h;
+    return nullptr;
+  }
+  return h;
+}
+
+void gss_ctx_destroy(gss_ctx* h) {
+  if (!h) return;
+  {
+    std::lock_guard<std::mutex> lk(h->c.mu);
+    (void)hipStreamSynchronize(h->c.stream);
+    free_workspace(h->c.ws);
+    if (h->c.ws.h_ctr) (void)hipHostFree(h->c.ws.h_ctr);
+    if (h->c.ws.h_local) (void)hipHostFree(h->c.ws.h_local);
+    for (auto e : h->c.events) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(h->c.stream);
+  }
+  delete h;
+}
+
+void* gss_ctx_stream(gss_ctx* h) { return h ? reinterpret_cast<void*>(h->c.stream) : nullptr; }
+
+int gss_ctx_sort_batch_device32(gss_ctx* h, const unsigned char* d_text, uint64_t text_bytes,
+                                const uint32_t* d_off_in, uint32_t* d_off_out, uint64_t* d_lcp,
+                                const uint64_t* req_begin, uint32_t nreq, void* stream) {
+  if (!h) return fail(GSS_E_INVALID, "NULL context");
+  if (!req_begin) return fail(GSS_E_INVALID, "NULL request table");
+  return run_locked(h->c, d_text, text_bytes, d_off_in, d_off_out, d_lcp, req_begin[nreq],
+                    req_begin, nreq, reinterpret_cast<hipStream_t>(stream));
+}
+
+int gss_ctx_get_stats(gss_ctx* h, gss_stats* out) {
+  if (!h || !out) return fail(GSS_E_INVALID, "NULL context or stats pointer");
+  std::lock_guard<std::mutex> lk(h->c.mu);
+  *out = h->c.stats;
+  return GSS_OK;
+}
+
 int gss_get_tuning(gss_tuning* out) {
   if (!out) return fail(GSS_E_INVALID, "NULL tuning pointer");
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  std::lock_guard<std::mutex> lk(g_mu);
   *out = g_tuning;
   return GSS_OK;
 }
@@ -425,33 +577,31 @@ int gss_set_tuning(const gss_tuning* in) {
   if (in->radix_bits != 8 && in->radix_bits != 11)
     return fail(GSS_E_INVALID, "radix_bits must be 8 or 11");
   if (in->tile != kTile) return fail(GSS_E_INVALID, "tile must be 4096");
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  std::lock_guard<std::mutex> lk(g_mu);
   g_tuning = *in;
   return GSS_OK;
 }
 
 int gss_get_stats(gss_stats* out) {
   if (!out) return fail(GSS_E_INVALID, "NULL stats pointer");
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
-  *out = g_stats;
+  *out = g_last_stats;
   return GSS_OK;
 }
 
 int gss_profile_enable(int on) {
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
-  g_prof = on != 0;
+  g_prof.store(on != 0);
   return GSS_OK;
 }
 
 int gss_profile_reset(void) {
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  std::lock_guard<std::mutex> lk(g_mu);
   g_prof_totals.clear();
   return GSS_OK;
 }
 
 int gss_profile_get(const char* name, uint64_t* launches, double* total_ms, uint64_t* items) {
   if (!name) return fail(GSS_E_INVALID, "NULL kernel name");
-  std::lock_guard<std::recursive_mutex> lk(g_mu);
+  std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_prof_totals.find(name);
   ProfTotal t;
   if (it != g_prof_totals.end()) t = it->second;

@@ -52,6 +52,16 @@ _SIGNATURES = {
     "gss_msd_lcp": (None, [C.POINTER(C.c_void_p), C.c_size_t, C.POINTER(C.c_size_t)]),
     "gss_sort_device32": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_uint64, C.c_void_p]),
+    "gss_sort_batch_device32": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.POINTER(C.c_uint64), C.c_uint32,
+                                          C.c_void_p]),
+    "gss_ctx_create": (C.c_void_p, []),
+    "gss_ctx_destroy": (None, [C.c_void_p]),
+    "gss_ctx_stream": (C.c_void_p, [C.c_void_p]),
+    "gss_ctx_sort_batch_device32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64),
+                                              C.c_uint32, C.c_void_p]),
+    "gss_ctx_get_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
     "gss_get_tuning": (C.c_int, [C.POINTER(Tuning)]),
     "gss_set_tuning": (C.c_int, [C.POINTER(Tuning)]),
     "gss_get_stats": (C.c_int, [C.POINTER(Stats)]),
@@ -158,6 +168,75 @@ def sort_device(text, offsets, out=None, lcp=None, text_bytes=None, stream=None)
                                  out.data_ptr(), lp if lp else None, n, stream.cuda_stream)
     _check(rc, "gss_sort_device32")
     return out
+
+
+def _req_table(req_begin):
+    arr = (C.c_uint64 * len(req_begin))(*[int(x) for x in req_begin])
+    return arr
+
+
+def sort_batch(text, offsets, req_begin, out=None, lcp=None, text_bytes=None, stream=None,
+               ctx=None):
+    """Sorts independent requests in one batch: request r owns offsets
+    [req_begin[r], req_begin[r+1]) of the uint32 `offsets` tensor (all into
+    `text`).  Each request's sorted offsets land at its own positions of
+    `out`.  `ctx` is a Context (None = the default context)."""
+    import torch
+
+    if not (text.is_cuda and offsets.is_cuda):
+        raise GssError("text and offsets must be device tensors")
+    n = offsets.numel()
+    nreq = len(req_begin) - 1
+    if nreq < 0 or int(req_begin[0]) != 0 or int(req_begin[-1]) != n:
+        raise GssError("req_begin must run from 0 to the number of offsets")
+    if out is None:
+        out = torch.empty_like(offsets)
+    if text_bytes is None:
+        text_bytes = text.numel()
+    lp = lcp.data_ptr() if lcp is not None else None
+    table = _req_table(req_begin)
+    if ctx is None:
+        if stream is None:
+            stream = torch.cuda.current_stream(text.device)
+        rc = lib().gss_sort_batch_device32(text.data_ptr(), int(text_bytes), offsets.data_ptr(),
+                                           out.data_ptr(), lp, table, nreq, stream.cuda_stream)
+    else:
+        st = stream.cuda_stream if stream is not None else None
+        rc = lib().gss_ctx_sort_batch_device32(ctx.handle, text.data_ptr(), int(text_bytes),
+                                               offsets.data_ptr(), out.data_ptr(), lp, table,
+                                               nreq, st)
+    _check(rc, "gss_sort_batch_device32")
+    return out
+
+
+class Context:
+    """A sort context: its own HIP stream and device workspace, so sorts on
+    different contexts can run at the same time (gss_ctx_* in gss.h)."""
+
+    def __init__(self):
+        self.handle = lib().gss_ctx_create()
+        if not self.handle:
+            raise GssError(f"gss_ctx_create failed: {last_error()}")
+
+    @property
+    def stream_handle(self):
+        return lib().gss_ctx_stream(self.handle)
+
+    def stats(self) -> "Stats":
+        s = Stats()
+        _check(lib().gss_ctx_get_stats(self.handle, C.byref(s)), "gss_ctx_get_stats")
+        return s
+
+    def close(self):
+        if self.handle:
+            lib().gss_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def get_tuning() -> Tuning:

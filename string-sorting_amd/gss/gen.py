@@ -219,6 +219,25 @@ def zipf_words(n: int = SIZES[3], seed: int = SEEDS[3], vocab: int = 2_000_000,
     return _assemble(n, lens, fill, "zipf_words")
 
 
+def concat(sets, name: str = "batch"):
+    """One text holding several sets one after another; returns the joined
+    StringSet and the request table (set r owns offsets [req[r], req[r+1]))."""
+    parts, offs, req = [], [], [0]
+    base = 0
+    for s in sets:
+        parts.append(s.text[: s.text_bytes])
+        offs.append(s.offsets.astype(np.int64) + base)
+        base += s.text_bytes
+        req.append(req[-1] + s.n)
+    if base + PAD >= 2**32:
+        raise ValueError("text too large for 32-bit offsets")
+    text = np.zeros(base + PAD, dtype=np.uint8)
+    if parts:
+        text[:base] = np.concatenate(parts)
+    off = np.concatenate(offs).astype(np.uint32) if offs else np.zeros(0, np.uint32)
+    return StringSet(text, off, name), req
+
+
 GENERATORS = {0: uniform_az, 1: url_like, 2: dna, 3: zipf_words}
 
 
