@@ -80,7 +80,141 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device twice: the "
                          "reference dispatch that scales FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--stream-requests", type=int, default=2000,
+                    help="requests of the served-stream comparison (0 = skip it)")
+    ap.add_argument("--stream-only", action="store_true",
+                    help="run only the served-stream comparison")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-budget-gib", type=float, default=16.0)
+    ap.add_argument("--serve-max-batch", type=int, default=1024)
+    ap.add_argument("--rates", type=str, default="0.5,1,2,4",
+                    help="swept Poisson rates, as multiples of the plain path's capacity")
     return ap.parse_args()
+
+
+def _pctl(x, q):
+    """Nearest-rank percentile (the smallest sample with >= q% at or below)."""
+    return float(np.percentile(np.asarray(x), q, method="inverted_cdf")) if len(x) else 0.0
+
+
+def served_stream(args, dev):
+    """Request-stream comparison: the served loop (continuous batching over
+    gss_server_*) against one plain gss_sort_device32 call per request, on
+    the same requests and arrival times.  Inputs are device-resident pools;
+    each request's sorted offsets land in its own range of an output arena,
+    one arena per mode, and the two arenas must be identical."""
+    import torch
+
+    import gss
+    import gss.gen as gen
+
+    t0 = time.time()
+    rs = gen.request_stream(nreq=args.stream_requests)
+    nreq = rs.nreq
+    sizes = rs.size.astype(np.int64)
+    firsts = np.array([rs.first(i) for i in range(nreq)], dtype=np.int64)
+    pos = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    total = int(sizes.sum())
+    log(f"request stream: {nreq} requests, {total} strings, pools {rs.pools.n} strings / "
+        f"{rs.pools.text_bytes} bytes, built in {time.time() - t0:.1f}s")
+    tb = rs.pools.text_bytes
+    text = torch.from_numpy(rs.pools.text).to(dev)
+    off = torch.from_numpy(rs.pools.offsets.view(np.int32)).to(dev)
+    arena = {m: torch.empty(total, dtype=torch.int32, device=dev) for m in ("plain", "served")}
+    stream = torch.cuda.current_stream(dev)
+    torch.cuda.synchronize()
+    L = gss.lib()
+    tp, op = text.data_ptr(), off.data_ptr()
+    in_ptr = [op + 4 * int(a) for a in firsts]
+    ns = [int(n) for n in sizes]
+    out_ptr = {m: [arena[m].data_ptr() + 4 * int(p) for p in pos] for m in arena}
+    st = stream.cuda_stream
+
+    # Warm-up: size the default context's workspace for the largest request.
+    big = int(np.argmax(sizes))
+    for _ in range(2):
+        rc = L.gss_sort_device32(tp, tb, in_ptr[big], out_ptr["plain"][big], None, ns[big], st)
+        if rc:
+            raise RuntimeError(f"plain sort failed: {gss.last_error()}")
+
+    def run_plain(due):
+        lat = np.empty(nreq)
+        start = time.perf_counter()
+        for i in range(nreq):
+            t_due = start + due[i]
+            while time.perf_counter() < t_due:
+                pass
+            rc = L.gss_sort_device32(tp, tb, in_ptr[i], out_ptr["plain"][i], None, ns[i], st)
+            if rc:
+                raise RuntimeError(f"plain sort of request {i} failed: {gss.last_error()}")
+            lat[i] = time.perf_counter() - t_due
+        return time.perf_counter() - start, lat * 1e3
+
+    budget = int(args.serve_budget_gib * 2**30)
+    srv = gss.Server(text, tb, hbm_budget=budget, slots=args.serve_slots,
+                     max_batch_requests=args.serve_max_batch)
+    w = srv.submit(off[firsts[big]:firsts[big] + ns[big]],
+                   arena["served"][pos[big]:pos[big] + ns[big]])
+    srv.wait(w)
+
+    def run_served(due):
+        ids = [0] * nreq
+        start = time.perf_counter()
+        for i in range(nreq):
+            t_due = start + due[i]
+            while time.perf_counter() < t_due:
+                pass
+            rid = L.gss_server_submit(srv.handle, in_ptr[i], out_ptr["served"][i], None, ns[i])
+            if rid < 0:
+                raise RuntimeError(f"submit of request {i} failed: {gss.last_error()}")
+            ids[i] = rid
+        lat = np.array([srv.wait(r) for r in ids])
+        return time.perf_counter() - start, lat
+
+    def identical():
+        torch.cuda.synchronize()
+        step = 1 << 28
+        return all(torch.equal(arena["plain"][a:a + step], arena["served"][a:a + step])
+                   for a in range(0, total, step))
+
+    def summary(wall, lat):
+        return {"wall_s": wall, "req_per_s": nreq / wall, "mstrings_per_s": total / wall / 1e6,
+                "p50_ms": _pctl(lat, 50), "p99_ms": _pctl(lat, 99), "max_ms": float(lat.max())}
+
+    zero = np.zeros(nreq)
+    log("plain, all requests at once ...")
+    plain = summary(*run_plain(zero))
+    log("served, all requests at once ...")
+    served = summary(*run_served(zero))
+    same = identical()
+    sst = srv.stats()
+    served.update({"batches": int(sst.batches) - 1, "batch_capacity": int(sst.batch_capacity),
+                   "max_batch_requests": int(sst.max_batch_requests_seen)})
+    sweep = []
+    for mult in [float(x) for x in args.rates.split(",") if x]:
+        rate = mult * plain["req_per_s"]
+        due = gen.arrivals(nreq, rate)
+        log(f"rate {rate:.0f}/s: plain ...")
+        p = summary(*run_plain(due))
+        log(f"rate {rate:.0f}/s: served ...")
+        s = summary(*run_served(due))
+        same = same and identical()
+        sweep.append({"rate_req_per_s": rate, "x_plain_capacity": mult,
+                      "plain": {k: p[k] for k in ("p50_ms", "p99_ms", "req_per_s")},
+                      "served": {k: s[k] for k in ("p50_ms", "p99_ms", "req_per_s")}})
+    srv.close()
+    return {
+        "workload": (f"first {nreq} of the {gen.STREAM_REQUESTS}-request stream (seed "
+                     f"{gen.STREAM_SEED}): sizes floor(exp(U[ln 1e3, ln 4e6])), generator "
+                     f"U{{0..3}}, payload = a contiguous slice of that generator's seeded "
+                     f"{gen.POOL_N}-string pool, pools device-resident"),
+        "requests": nreq, "strings": total,
+        "slots": args.serve_slots, "hbm_budget_bytes": budget,
+        "max_batch_requests": args.serve_max_batch,
+        "plain": plain, "served": served,
+        "throughput_ratio": served["req_per_s"] / plain["req_per_s"],
+        "sweep": sweep, "identical_output": bool(same),
+    }
 
 
 def cpu_baseline(ss, runs):
@@ -134,10 +268,13 @@ def main():
     import gss
     import gss.gen as gen
 
+    dev = torch.device("cuda:0")
+    if args.stream_only:
+        print(json.dumps({"batched": served_stream(args, dev)}), flush=True)
+        return
     t0 = time.time()
     ss = gen.url_like(args.n)
     log(f"generated {ss.n} strings, {ss.text_bytes} text bytes in {time.time() - t0:.1f}s")
-    dev = torch.device("cuda:0")
     text = torch.from_numpy(ss.text).to(dev)
     off = torch.from_numpy(ss.offsets.view(np.int32)).to(dev)
     out = torch.empty_like(off)
@@ -234,6 +371,11 @@ def main():
         e2e_ok = bool(np.array_equal((ptrs - np.uint64(base)).astype(np.uint32), got))
         res["e2e"] = {"entry": "gss_msd (host pointers, H2D text+pointers, D2H pointers)",
                       "seconds": te, "mstrings_per_s": ss.n / te / 1e6, "matches_device": e2e_ok}
+
+    if args.stream_requests > 0:
+        del text, off, out, lcp
+        torch.cuda.empty_cache()
+        res["batched"] = served_stream(args, dev)
 
     if not args.no_cpu:
         log("timing the CPU baseline ...")

@@ -96,6 +96,62 @@ int gss_ctx_sort_batch_device32(gss_ctx *ctx, const unsigned char *d_text,
 				uint32_t *d_off_out, uint64_t *d_lcp,
 				const uint64_t *req_begin, uint32_t nreq,
 				void *stream);
+/* Sizes the context's device workspace for sorts of up to n strings now, so
+ * that later calls allocate nothing until they exceed n. */
+int gss_ctx_reserve(gss_ctx *ctx, uint64_t n);
+
+/* ------------------------------------------------------------------------
+ * Served request loop (no reference counterpart).  A server is bound to one
+ * device-resident text; requests name strings of it by device offset arrays.
+ * Requests queue as they arrive; each of `slots` worker slots (a sort context
+ * and its batch buffers) repeatedly takes as many queued requests as fit its
+ * share of the HBM budget (at most max_batch_requests), gathers their
+ * offsets with one kernel, sorts them as one batch, scatters the results
+ * back and completes them.  Slots run concurrently, so requests arriving
+ * while others are on the GPU start as soon as a slot frees up.  Results are
+ * identical to sorting each request alone (gss_sort_device32). */
+typedef struct gss_server gss_server;
+
+typedef struct gss_server_config {
+	uint64_t hbm_budget;         /* device bytes for all slots' batches */
+	uint32_t slots;              /* batches on the GPU at once, 1..8 */
+	uint32_t max_batch_requests; /* requests folded into one batch, >= 1 */
+} gss_server_config;
+
+typedef struct gss_server_stats {
+	uint64_t submitted;
+	uint64_t completed;
+	uint64_t batches;
+	uint64_t strings;           /* strings of completed requests */
+	uint64_t batch_capacity;    /* strings one slot's batch may hold */
+	uint32_t max_batch_requests_seen;
+	double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> completion */
+} gss_server_stats;
+
+/* Creates a server on the current device for the text d_text (device, 8-byte
+ * aligned, text_bytes bytes, as for gss_sort_device32), which must stay valid
+ * until gss_server_destroy.  One slot's batch capacity is the largest string
+ * count whose buffers and workspace, times `slots`, fit hbm_budget (under the
+ * tuning current at creation).  Returns NULL on error (gss_last_error()). */
+gss_server *gss_server_create(const unsigned char *d_text, uint64_t text_bytes,
+			      const gss_server_config *cfg);
+/* Queues a sort of the n strings at device offsets d_off (into the server's
+ * text).  The sorted offsets go to d_out (device, n entries, may equal d_off)
+ * and, when d_lcp is not NULL, the n-1 LCPs to d_lcp.  The arrays must stay
+ * valid until the request completes.  Returns the request id (>= 0), or a
+ * negative GSS_E* code: GSS_E_TOO_LARGE when n exceeds one slot's batch
+ * capacity (that request is not admitted; sort it with gss_sort_device32). */
+int64_t gss_server_submit(gss_server *s, const uint32_t *d_off, uint32_t *d_out,
+			  uint64_t *d_lcp, uint64_t n);
+/* Waits for request `id` and releases its record; *latency_ms (optional) =
+ * completion - submission.  Returns the request's status (0 = sorted). */
+int gss_server_wait(gss_server *s, int64_t id, double *latency_ms);
+/* 1 once request `id` has completed, 0 before, GSS_E_INVALID for an id that
+ * is unknown or already waited for. */
+int gss_server_done(gss_server *s, int64_t id);
+int gss_server_get_stats(gss_server *s, gss_server_stats *out);
+/* Finishes every queued request, stops the slots and frees the server. */
+void gss_server_destroy(gss_server *s);
 
 /* ------------------------------------------------------------------------
  * Tuning knobs (no reference counterpart: the reference fixes its

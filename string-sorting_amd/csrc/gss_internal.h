@@ -367,4 +367,16 @@ hipError_t launch_off_to_orig_ptr(const uint32_t* off, const uint32_t* starts,
                                   const uint64_t* orig, uint64_t* ptrs, uint64_t n,
                                   hipStream_t s);
 
+// One piece of a multi-array copy: `words` 32-bit words from src to dst.
+// The served loop gathers its requests' offsets into one batch array and
+// scatters the results back with one launch each.
+constexpr uint32_t kChunkWords = 16384;
+struct CopyChunk {
+  const uint32_t* src;
+  uint32_t* dst;
+  uint64_t words;  // <= kChunkWords
+};
+// Copies chunks[0, nchunks) (device array), one workgroup per chunk.
+hipError_t launch_copy_chunks(const CopyChunk* chunks, uint32_t nchunks, hipStream_t s);
+
 }  // namespace gss

@@ -80,6 +80,11 @@ __global__ void k_off_to_orig_ptr(const uint32_t* __restrict__ off,
   }
 }
 
+__global__ __launch_bounds__(256) void k_copy_chunks(const CopyChunk* __restrict__ chunks) {
+  const CopyChunk c = chunks[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < c.words; i += 256) c.dst[i] = c.src[i];
+}
+
 // ---------------------------------------------------------------------------
 // Digits of a global pass.
 //
@@ -848,6 +853,12 @@ hipError_t launch_off_to_orig_ptr(const uint32_t* off, const uint32_t* starts,
                                   const uint64_t* orig, uint64_t* ptrs, uint64_t n,
                                   hipStream_t s) {
   k_off_to_orig_ptr<<<grid_for(n, 256), 256, 0, s>>>(off, starts, orig, ptrs, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_chunks(const CopyChunk* chunks, uint32_t nchunks, hipStream_t s) {
+  if (!nchunks) return hipSuccess;
+  k_copy_chunks<<<nchunks, 256, 0, s>>>(chunks);
   return hipGetLastError();
 }
 

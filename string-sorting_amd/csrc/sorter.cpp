@@ -191,56 +191,81 @@ void free_workspace(Workspace& w) {
   w.n_cap = 0;
 }
 
+// Capacities of the workspace tables for up to ncap strings and local
+// threshold t.
+struct Caps {
+  uint64_t ncap, big, tiles, local[3], retire, bounds;
+};
+
+Caps caps_for(uint64_t ncap, uint32_t t) {
+  Caps k;
+  k.ncap = std::max<uint64_t>(ncap, 4096);
+  k.big = k.ncap / (t + 1) + 2;
+  k.tiles = k.ncap / kTile + k.big + 2;
+  k.local[0] = k.ncap / 2 + 2;
+  k.local[1] = k.ncap / (kLocalClassCap[0] + 1) + 2;
+  k.local[2] = k.ncap / (kLocalClassCap[1] + 1) + 2;
+  k.retire = k.ncap + k.ncap / kRetireChunk + 8;  // single-string requests too
+  k.bounds = k.ncap + 2;
+  return k;
+}
+
+struct Item {
+  void** dst;
+  size_t bytes;
+};
+
+// The device arrays of a workspace, in allocation order.
+std::vector<Item> workspace_items(Plan& p, uint32_t** idx_copy, const Caps& k, uint32_t rb) {
+  const uint64_t nb = 1ull << rb;
+  return {
+      {(void**)&p.idx[1], 4 * k.ncap},
+      {(void**)&p.idx[2], 4 * k.ncap},
+      {(void**)idx_copy, 4 * k.ncap},
+      {(void**)&p.keys, 8 * k.ncap},
+      {(void**)&p.big[0], sizeof(BigSeg) * k.big},
+      {(void**)&p.big[1], sizeof(BigSeg) * k.big},
+      {(void**)&p.segmm, 16 * k.big},
+      {(void**)&p.bstart, 4 * k.big * nb},
+      {(void**)&p.alpha, 32},
+      {(void**)&p.tiles, sizeof(Tile) * k.tiles},
+      {(void**)&p.aux, sizeof(SegAux) * k.big},
+      {(void**)&p.flat, 4 * (k.tiles * nb + 1)},
+      {(void**)&p.flat_blk, 4 * ((k.tiles * nb) / kScanBlock + 2)},
+      {(void**)&p.local[0], sizeof(LocalSeg) * k.local[0]},
+      {(void**)&p.local[1], sizeof(LocalSeg) * k.local[1]},
+      {(void**)&p.local[2], sizeof(LocalSeg) * k.local[2]},
+      {(void**)&p.retire, sizeof(Retire) * k.retire},
+      {(void**)&p.bounds, sizeof(Boundary) * k.bounds},
+      {(void**)&p.ctr, sizeof(Counters)},
+      {(void**)&p.lstats, sizeof(LocalStats)},
+      {(void**)&p.tile_mm, 16 * k.tiles},
+      {(void**)&p.tile_alpha, 32 * k.tiles},
+      {(void**)&p.seg_alpha, 32 * k.big},
+  };
+}
+
+size_t items_bytes(const std::vector<Item>& items) {
+  size_t total = 0;
+  for (auto& it : items) total += align_up(it.bytes, 256);
+  return total;
+}
+
 // Sizes every device array for up to n strings, 2^rb digits and local
 // threshold t, and carves them out of one allocation.
 int ensure_workspace(Context& c, uint64_t n, uint32_t rb, uint32_t t) {
   Workspace& w = c.ws;
   if (w.base && n <= w.n_cap && rb == w.rb && t == w.threshold) return GSS_OK;
   GSS_HIP(hipStreamSynchronize(c.stream));
-  uint64_t ncap = std::max<uint64_t>(n, w.n_cap);
-  if (ncap < 4096) ncap = 4096;
+  const Caps k = caps_for(std::max<uint64_t>(n, w.n_cap), t);
   free_workspace(w);
-  const uint64_t nb = 1ull << rb;
-  const uint64_t cap_big = ncap / (t + 1) + 2;
-  const uint64_t cap_tiles = ncap / kTile + cap_big + 2;
-  const uint64_t cap_local[3] = {ncap / 2 + 2, ncap / (kLocalClassCap[0] + 1) + 2,
-                                 ncap / (kLocalClassCap[1] + 1) + 2};
-  const uint64_t cap_retire = ncap + ncap / kRetireChunk + 8;  // single-string requests too
-  const uint64_t cap_bounds = ncap + 2;
-
-  struct Item {
-    void** dst;
-    size_t bytes;
-  };
+  const uint64_t ncap = k.ncap, cap_big = k.big, cap_tiles = k.tiles, cap_retire = k.retire,
+                 cap_bounds = k.bounds;
+  const uint64_t* cap_local = k.local;
   Plan& p = w.plan;
   p = Plan{};
-  std::vector<Item> items = {
-      {(void**)&p.idx[1], 4 * ncap},
-      {(void**)&p.idx[2], 4 * ncap},
-      {(void**)&w.idx_copy, 4 * ncap},
-      {(void**)&p.keys, 8 * ncap},
-      {(void**)&p.big[0], sizeof(BigSeg) * cap_big},
-      {(void**)&p.big[1], sizeof(BigSeg) * cap_big},
-      {(void**)&p.segmm, 16 * cap_big},
-      {(void**)&p.bstart, 4 * cap_big * nb},
-      {(void**)&p.alpha, 32},
-      {(void**)&p.tiles, sizeof(Tile) * cap_tiles},
-      {(void**)&p.aux, sizeof(SegAux) * cap_big},
-      {(void**)&p.flat, 4 * (cap_tiles * nb + 1)},
-      {(void**)&p.flat_blk, 4 * ((cap_tiles * nb) / kScanBlock + 2)},
-      {(void**)&p.local[0], sizeof(LocalSeg) * cap_local[0]},
-      {(void**)&p.local[1], sizeof(LocalSeg) * cap_local[1]},
-      {(void**)&p.local[2], sizeof(LocalSeg) * cap_local[2]},
-      {(void**)&p.retire, sizeof(Retire) * cap_retire},
-      {(void**)&p.bounds, sizeof(Boundary) * cap_bounds},
-      {(void**)&p.ctr, sizeof(Counters)},
-      {(void**)&p.lstats, sizeof(LocalStats)},
-      {(void**)&p.tile_mm, 16 * cap_tiles},
-      {(void**)&p.tile_alpha, 32 * cap_tiles},
-      {(void**)&p.seg_alpha, 32 * cap_big},
-  };
-  size_t total = 0;
-  for (auto& it : items) total += align_up(it.bytes, 256);
+  std::vector<Item> items = workspace_items(p, &w.idx_copy, k, rb);
+  const size_t total = items_bytes(items);
   char* base = nullptr;
   if (hipMalloc((void**)&base, total) != hipSuccess) {
     (void)hipGetLastError();
@@ -499,6 +524,35 @@ int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t
   return run_locked(*c, text, text_bytes, in, out, lcp, n, nullptr, 0, stream);
 }
 
+size_t workspace_bytes(uint64_t n) {
+  gss_tuning tun;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    tun = g_tuning;
+  }
+  Plan p{};
+  uint32_t* idx_copy = nullptr;
+  return items_bytes(workspace_items(p, &idx_copy, caps_for(n, tun.local_threshold),
+                                     tun.radix_bits));
+}
+
+int ctx_reserve(gss_ctx* h, uint64_t n, uint32_t nreq) {
+  if (!h) return fail(GSS_E_INVALID, "NULL context");
+  Context& c = h->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  int dev = -1;
+  GSS_HIP(hipGetDevice(&dev));
+  if (dev != c.device) return fail(GSS_E_INVALID, "the context belongs to another device");
+  gss_tuning tun;
+  {
+    std::lock_guard<std::mutex> lk2(g_mu);
+    tun = g_tuning;
+  }
+  int rc = ensure_workspace(c, n, tun.radix_bits, tun.local_threshold);
+  if (rc) return rc;
+  return nreq ? ensure_init_lists(c.ws, nreq) : GSS_OK;
+}
+
 }  // namespace gss
 
 using namespace gss;
@@ -555,6 +609,8 @@ int gss_ctx_sort_batch_device32(gss_ctx* h, const unsigned char* d_text, uint64_
   return run_locked(h->c, d_text, text_bytes, d_off_in, d_off_out, d_lcp, req_begin[nreq],
                     req_begin, nreq, reinterpret_cast<hipStream_t>(stream));
 }
+
+int gss_ctx_reserve(gss_ctx* h, uint64_t n) { return ctx_reserve(h, n, 0); }
 
 int gss_ctx_get_stats(gss_ctx* h, gss_stats* out) {
   if (!h || !out) return fail(GSS_E_INVALID, "NULL context or stats pointer");

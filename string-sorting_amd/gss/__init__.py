@@ -46,6 +46,26 @@ class Stats(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("slots", C.c_uint32),
+                ("max_batch_requests", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64),
+        ("completed", C.c_uint64),
+        ("batches", C.c_uint64),
+        ("strings", C.c_uint64),
+        ("batch_capacity", C.c_uint64),
+        ("max_batch_requests_seen", C.c_uint32),
+        ("p50_ms", C.c_double),
+        ("p99_ms", C.c_double),
+        ("mean_ms", C.c_double),
+        ("max_ms", C.c_double),
+    ]
+
+
 # Every symbol include/gss.h declares, with its ctypes signature.
 _SIGNATURES = {
     "gss_msd": (None, [C.POINTER(C.c_void_p), C.c_size_t]),
@@ -62,6 +82,14 @@ _SIGNATURES = {
                                               C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64),
                                               C.c_uint32, C.c_void_p]),
     "gss_ctx_get_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gss_ctx_reserve": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "gss_server_create": (C.c_void_p, [C.c_void_p, C.c_uint64, C.POINTER(ServerConfig)]),
+    "gss_server_submit": (C.c_int64, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_uint64]),
+    "gss_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_double)]),
+    "gss_server_done": (C.c_int, [C.c_void_p, C.c_int64]),
+    "gss_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "gss_server_destroy": (None, [C.c_void_p]),
     "gss_get_tuning": (C.c_int, [C.POINTER(Tuning)]),
     "gss_set_tuning": (C.c_int, [C.POINTER(Tuning)]),
     "gss_get_stats": (C.c_int, [C.POINTER(Stats)]),
@@ -231,6 +259,79 @@ class Context:
         if self.handle:
             lib().gss_ctx_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The served request loop (gss_server_* in gss.h) over one device text.
+
+    ``submit`` queues a sort of a uint32 offset tensor (a contiguous device
+    tensor, e.g. a slice of a pool) into an output tensor of the same size;
+    slots take queued requests in order, fold as many as fit into one batch
+    and complete them together.  ``wait`` returns the request's latency in ms.
+    """
+
+    def __init__(self, text, text_bytes=None, hbm_budget=8 << 30, slots=2,
+                 max_batch_requests=4096):
+        if not text.is_cuda:
+            raise GssError("the server text must be a device tensor")
+        self.text = text  # kept alive for the server's lifetime
+        cfg = ServerConfig(int(hbm_budget), int(slots), int(max_batch_requests))
+        tb = text.numel() if text_bytes is None else int(text_bytes)
+        self.handle = lib().gss_server_create(text.data_ptr(), tb, C.byref(cfg))
+        if not self.handle:
+            raise GssError(f"gss_server_create failed: {last_error()}")
+        self._keep = {}
+
+    def submit(self, offsets, out, lcp=None) -> int:
+        n = offsets.numel()
+        if out.numel() < n:
+            raise GssError("output tensor too small")
+        if not (offsets.is_contiguous() and out.is_contiguous()):
+            raise GssError("offsets and out must be contiguous")
+        if lcp is not None and (lcp.numel() < max(0, n - 1) or lcp.element_size() != 8):
+            raise GssError("lcp must hold n-1 64-bit entries")
+        rid = lib().gss_server_submit(self.handle, offsets.data_ptr(), out.data_ptr(),
+                                      lcp.data_ptr() if lcp is not None else None, n)
+        if rid < 0:
+            raise GssError(f"gss_server_submit failed ({rid}): {last_error()}")
+        self._keep[rid] = (offsets, out, lcp)
+        return rid
+
+    def wait(self, rid: int) -> float:
+        ms = C.c_double()
+        rc = lib().gss_server_wait(self.handle, rid, C.byref(ms))
+        self._keep.pop(rid, None)
+        _check(rc, "gss_server_wait")
+        return ms.value
+
+    def done(self, rid: int) -> bool:
+        rc = lib().gss_server_done(self.handle, rid)
+        if rc < 0:
+            raise GssError(f"gss_server_done failed ({rc}): {last_error()}")
+        return rc == 1
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        _check(lib().gss_server_get_stats(self.handle, C.byref(s)), "gss_server_get_stats")
+        return s
+
+    def close(self):
+        if self.handle:
+            lib().gss_server_destroy(self.handle)
+            self.handle = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:

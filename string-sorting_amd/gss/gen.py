@@ -245,3 +245,69 @@ def make(config: int, n: int | None = None, seed: int | None = None) -> StringSe
     """Workload `config` (0..3) at its default size and seed unless given."""
     gen = GENERATORS[config]
     return gen(SIZES[config] if n is None else n, SEEDS[config] if seed is None else seed)
+
+
+# ---------------------------------------------------------------------------
+# Request stream: many independent sorts of uneven size.
+#
+# Payloads come from a seeded pool per generator: pool g is GENERATORS[g] at
+# POOL_N strings and its default seed.  Request i draws its generator id
+# uniformly from 0..3, its size as floor(exp(U[ln lo, ln hi])) and a uniform
+# start in its pool; its strings are the pool's strings [start, start+size)
+# in pool order.  Arrivals are a Poisson process: exponential gaps at `rate`
+# requests per second.
+
+STREAM_SEED = 20244
+STREAM_REQUESTS = 20_000
+POOL_N = 4_000_000
+STREAM_SIZES = (1_000, 4_000_000)
+
+
+@dataclass
+class RequestStream:
+    pools: StringSet        # the four pools joined (gen.concat order 0..3)
+    pool_begin: list        # pool g owns offsets [pool_begin[g], pool_begin[g+1])
+    gen_id: np.ndarray      # int64 per request
+    start: np.ndarray       # int64 start inside the pool
+    size: np.ndarray        # int64 strings per request
+
+    @property
+    def nreq(self) -> int:
+        return int(self.size.shape[0])
+
+    def first(self, i: int) -> int:
+        """Index of request i's first string in the joined offset array."""
+        return int(self.pool_begin[int(self.gen_id[i])] + self.start[i])
+
+    def offsets(self, i: int) -> np.ndarray:
+        a = self.first(i)
+        return self.pools.offsets[a:a + int(self.size[i])]
+
+
+def request_stream(nreq: int = STREAM_REQUESTS, seed: int = STREAM_SEED, pool_n: int = POOL_N,
+                   sizes=STREAM_SIZES, pools: StringSet | None = None,
+                   pool_begin=None) -> RequestStream:
+    """The first `nreq` requests of the seeded stream (pools are built unless
+    given, so several streams can share them)."""
+    if pools is None:
+        pools, pool_begin = concat([GENERATORS[g](pool_n, SEEDS[g]) for g in range(4)], "pools")
+    lo, hi = sizes
+    hi = min(hi, pool_n)
+    rng = np.random.default_rng(seed)
+    m = max(nreq, STREAM_REQUESTS)  # drawn in full, so a shorter stream is a prefix
+    gen_id = rng.integers(0, 4, size=m, dtype=np.int64)[:nreq]
+    size = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), size=m)))[:nreq].astype(np.int64)
+    size = np.clip(size, 1, pool_n)
+    start = np.floor(rng.random(m)[:nreq] * (pool_n - size + 1)).astype(np.int64)
+    return RequestStream(pools, list(pool_begin), gen_id, start, size)
+
+
+def arrivals(nreq: int, rate: float, seed: int = STREAM_SEED) -> np.ndarray:
+    """Poisson arrival times (seconds from the first arrival) at `rate`/s;
+    rate = inf gives all requests at time 0."""
+    if not np.isfinite(rate):
+        return np.zeros(nreq)
+    rng = np.random.default_rng(seed + 1)
+    gaps = rng.exponential(1.0 / rate, size=nreq)
+    gaps[0] = 0.0
+    return np.cumsum(gaps)

@@ -90,3 +90,29 @@ def test_stats_and_profile_without_device():
     assert s.global_passes >= 0
     gss.profile_reset()
     assert gss.profile_get("scatter") == (0, 0.0, 0)
+
+
+def test_server_create_validates_before_touching_a_device():
+    """Argument checks and HBM-budget admission of gss_server_create run on
+    the host: a budget below one slot's workspace is refused with a message,
+    as are slot counts outside [1, 8], before any device call."""
+    import ctypes as C
+
+    lib = gss.lib()
+    fake_text = C.c_void_p(1 << 20)  # never dereferenced: every case fails first
+    for budget, slots, maxreq, msg in [
+        (1 << 30, 0, 16, "slots"),
+        (1 << 30, 9, 16, "slots"),
+        (1 << 30, 2, 0, "max_batch_requests"),
+        (1000, 1, 16, "too small"),
+    ]:
+        cfg = gss.ServerConfig(budget, slots, maxreq)
+        assert not lib.gss_server_create(fake_text, 4096, C.byref(cfg))
+        assert msg in gss.last_error()
+    cfg = gss.ServerConfig(1 << 30, 1, 16)
+    assert not lib.gss_server_create(C.c_void_p(1 << 20 | 4), 4096, C.byref(cfg))
+    assert "aligned" in gss.last_error()
+    assert lib.gss_server_submit(None, None, None, None, 5) < 0
+    assert lib.gss_server_wait(None, 0, None) < 0
+    assert lib.gss_server_done(None, 0) < 0
+    lib.gss_server_destroy(None)

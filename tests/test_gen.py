@@ -65,3 +65,28 @@ def test_from_strings():
     ss = gen.from_strings([b"b", b"", b"a"])
     assert ss.strings() == [b"b", b"", b"a"]
     assert ss.offsets.tolist() == [0, 2, 3]
+
+
+def test_request_stream_is_seeded_and_prefix_stable():
+    """The stream's request table: sizes log-uniform in [1e3, pool], ids 0..3,
+    every slice inside its pool; a shorter stream is a prefix of the full one."""
+    pools, begin = gen.concat([gen.GENERATORS[g](3000, gen.SEEDS[g]) for g in range(4)], "pools")
+    a = gen.request_stream(nreq=50, pool_n=3000, sizes=(10, 3000), pools=pools, pool_begin=begin)
+    b = gen.request_stream(nreq=400, pool_n=3000, sizes=(10, 3000), pools=pools, pool_begin=begin)
+    assert np.array_equal(a.size, b.size[:50]) and np.array_equal(a.gen_id, b.gen_id[:50])
+    assert np.array_equal(a.start, b.start[:50])
+    assert b.size.min() >= 10 and b.size.max() <= 3000
+    assert set(np.unique(b.gen_id)) == {0, 1, 2, 3}
+    assert np.all(b.start >= 0) and np.all(b.start + b.size <= 3000)
+    for i in range(b.nreq):
+        o = b.offsets(i)
+        g = int(b.gen_id[i])
+        assert o.shape[0] == b.size[i]
+        assert np.array_equal(o, pools.offsets[begin[g] + b.start[i]: begin[g] + b.start[i] + b.size[i]])
+    # log-uniform: the median size is near the geometric mean of the range
+    med = np.median(b.size)
+    assert 100 < med < 300
+    t = gen.arrivals(1000, 500.0)
+    assert t[0] == 0 and np.all(np.diff(t) >= 0)
+    assert abs(t[-1] / 999 - 1 / 500.0) < 0.2 / 500.0
+    assert np.all(gen.arrivals(5, float("inf")) == 0)

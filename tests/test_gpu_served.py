@@ -1,0 +1,133 @@
+"""GPU: the served request loop (gss_server_*).  Requests of uneven size
+arrive while others are on the GPU, are folded into batches and must each
+come out exactly as when sorted alone (gss_sort_device32) and as the oracle's
+msd_CE2 sorts them, LCP arrays included."""
+
+import time
+
+import numpy as np
+import pytest
+
+from gpu_util import device, log
+
+pytestmark = pytest.mark.gpu
+
+import gss  # noqa: E402
+import gss.gen as gen  # noqa: E402
+
+POOL_N = 150_000
+
+
+@pytest.fixture(scope="module")
+def stream():
+    return gen.request_stream(nreq=48, seed=4242, pool_n=POOL_N, sizes=(1_000, POOL_N))
+
+
+@pytest.fixture(scope="module")
+def pools_dev(stream):
+    import torch
+
+    dev = device()
+    text = torch.from_numpy(stream.pools.text).to(dev)
+    off = torch.from_numpy(stream.pools.offsets.view(np.int32)).to(dev)
+    return text, off
+
+
+def _budget_for(slots, cap_strings):
+    """A budget whose per-slot batch capacity is roughly cap_strings."""
+    return slots * cap_strings * 110
+
+
+def _alone(text, off, a, n, text_bytes):
+    import torch
+
+    out = torch.empty(n, dtype=torch.int32, device=text.device)
+    lc = torch.empty(max(1, n - 1), dtype=torch.int64, device=text.device)
+    gss.sort_device(text, off[a:a + n], out, lc, text_bytes=text_bytes)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().view(np.uint32), lc.cpu().numpy().view(np.uint64)[: max(0, n - 1)]
+
+
+def test_served_stream_matches_alone_and_oracle(stream, pools_dev, oracle_mod):
+    import torch
+
+    text, off = pools_dev
+    tb = stream.pools.text_bytes
+    rng = np.random.default_rng(7)
+    reqs = [(stream.first(i), int(stream.size[i])) for i in range(stream.nreq)]
+    # Edge sizes: empty, single string, a pair, one past the local threshold.
+    reqs += [(0, 0), (5, 1), (9, 2), (stream.pool_begin[1] + 11, 8193)]
+    with gss.Server(text, tb, hbm_budget=_budget_for(3, 400_000), slots=3,
+                    max_batch_requests=16) as srv:
+        cap = srv.stats().batch_capacity
+        log(f"served stream: {len(reqs)} requests, batch capacity {cap}")
+        assert cap < sum(n for _, n in reqs), "capacity must force several batches"
+        outs, lcps, ids = [], [], []
+        for k, (a, n) in enumerate(reqs):
+            o = torch.full((max(1, n),), -1, dtype=torch.int32, device=text.device)
+            l = None
+            if k % 2 == 1:
+                l = torch.full((max(1, n - 1),), -7, dtype=torch.int64, device=text.device)
+            outs.append(o)
+            lcps.append(l)
+            ids.append(srv.submit(off[a:a + n], o[:n], l))
+            if k >= 20:  # the first 20 arrive at once, the rest spread out
+                time.sleep(float(rng.uniform(0, 0.002)))
+        lat = [srv.wait(i) for i in ids]
+        st = srv.stats()
+        log(f"batches {st.batches}, max requests per batch {st.max_batch_requests_seen}, "
+            f"p50 {st.p50_ms:.3f} ms, p99 {st.p99_ms:.3f} ms")
+        assert st.completed == len(reqs) and st.submitted == len(reqs)
+        assert st.max_batch_requests_seen > 1, "queued requests must be folded into one batch"
+        assert st.batches < len(reqs)
+        assert all(x >= 0 for x in lat)
+    for k, (a, n) in enumerate(reqs):
+        if n == 0:
+            continue
+        got = outs[k].cpu().numpy().view(np.uint32)[:n]
+        alone, alone_lcp = _alone(text, off, a, n, tb)
+        assert np.array_equal(got, alone), f"request {k} (n={n}) differs from its solo sort"
+        want, want_lcp = oracle_mod.sort_offsets(stream.pools.text, stream.pools.offsets[a:a + n],
+                                                 "msd_ce2", lcp=True)
+        assert np.array_equal(got, want), f"request {k} (n={n}) differs from msd_CE2"
+        if lcps[k] is not None and n > 1:
+            gl = lcps[k].cpu().numpy().view(np.uint64)[: n - 1]
+            assert np.array_equal(gl, want_lcp), f"request {k} LCP"
+            assert np.array_equal(gl, alone_lcp)
+
+
+def test_served_admission_by_budget(stream, pools_dev):
+    import torch
+
+    text, off = pools_dev
+    with gss.Server(text, stream.pools.text_bytes, hbm_budget=_budget_for(1, 20_000), slots=1,
+                    max_batch_requests=4) as srv:
+        cap = srv.stats().batch_capacity
+        assert 1_000 < cap < 60_000
+        out = torch.empty(cap + 1, dtype=torch.int32, device=text.device)
+        with pytest.raises(gss.GssError, match="exceeds the batch capacity"):
+            srv.submit(off[: cap + 1], out)
+        rid = srv.submit(off[:cap], out[:cap])
+        srv.wait(rid)
+        alone, _ = _alone(text, off, 0, cap, stream.pools.text_bytes)
+        assert np.array_equal(out[:cap].cpu().numpy().view(np.uint32), alone)
+        with pytest.raises(gss.GssError):
+            srv.wait(rid)  # already released
+
+
+def test_served_close_finishes_queued_requests(stream, pools_dev):
+    import torch
+
+    text, off = pools_dev
+    tb = stream.pools.text_bytes
+    srv = gss.Server(text, tb, hbm_budget=_budget_for(2, 200_000), slots=2, max_batch_requests=8)
+    reqs = [(stream.first(i), int(stream.size[i])) for i in range(12)]
+    outs = []
+    for a, n in reqs:
+        o = torch.empty(n, dtype=torch.int32, device=text.device)
+        outs.append(o)
+        srv.submit(off[a:a + n], o)
+    srv.close()  # no wait: closing drains the queue
+    for (a, n), o in zip(reqs, outs):
+        alone, _ = _alone(text, off, a, n, tb)
+        assert np.array_equal(o.cpu().numpy().view(np.uint32), alone)
