@@ -13,7 +13,12 @@ Besides the contract fields the JSON line carries:
   cpu_baseline  the reference's msd_CE2 (compiled from its sources into
                 oracle/_ref) timed on this host's cores on the same workload;
   hbm_gbps      algorithmic bytes of the whole sort / sort time;
-  e2e           the routine.h pointer path (gss_msd), PCIe transfers included.
+  e2e           the routine.h pointer path (gss_msd), PCIe transfers included;
+  batched       the request stream (20,000 sorts of 1e3..4e6 strings from the
+                four generators' seeded pools): the served loop (continuous
+                batching, gss_server_*) against one plain gss_sort_device32
+                call per request -- requests/s, p50/p99 latency at swept
+                Poisson rates, and whether both outputs are identical.
 """
 
 from __future__ import annotations
@@ -80,14 +85,14 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device twice: the "
                          "reference dispatch that scales FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--stream-requests", type=int, default=2000,
+    ap.add_argument("--stream-requests", type=int, default=20000,
                     help="requests of the served-stream comparison (0 = skip it)")
     ap.add_argument("--stream-only", action="store_true",
                     help="run only the served-stream comparison")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
     ap.add_argument("--serve-max-batch", type=int, default=1024)
-    ap.add_argument("--rates", type=str, default="0.5,1,2,4",
+    ap.add_argument("--rates", type=str, default="1,2,4",
                     help="swept Poisson rates, as multiples of the plain path's capacity")
     return ap.parse_args()
 
