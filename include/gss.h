@@ -154,6 +154,49 @@ int gss_server_get_stats(gss_server *s, gss_server_stats *out);
 void gss_server_destroy(gss_server *s);
 
 /* ------------------------------------------------------------------------
+ * The steps either side of the sort, on the device.  All take device
+ * pointers, return 0 or a negative GSS_E* code and return once the result is
+ * complete on `stream` (NULL = the null stream).  A text is 8-byte aligned;
+ * its last, partial 8-byte word may be read (never written) in full.
+ *
+ * Ingest replaces create_strings_delim (src/sortstring.c:310-335): every
+ * byte equal to `delim` ends a string, which starts after the previous
+ * delimiter (or at 0); unless delim is 0 each delimiter is overwritten with
+ * a NUL.  Bytes after the last delimiter belong to no string.  Writes the
+ * string count to *n_strings and the offsets to d_off (max_strings entries;
+ * d_off == NULL only counts).  No delimiter at all is an error, as in the
+ * reference ("unable to read any lines").  sortstring's plain input uses
+ * delim '\n', its --raw input delim 0 (src/sortstring.c:337-347). */
+int gss_ingest_device32(unsigned char *d_text, uint64_t text_bytes, int delim,
+			uint32_t *d_off, uint64_t max_strings,
+			uint64_t *n_strings, void *stream);
+
+/* Suffix mode, create_suffixes (src/sortstring.c:349-358): d_off[i] = i for
+ * every text position; the suffixes are then sorted like any strings (each
+ * runs to the text's NUL or end). */
+int gss_suffixes_device32(uint64_t text_bytes, uint32_t *d_off, void *stream);
+
+/* --write formatting, write_result (src/sortstring.c:360-391): the strings
+ * d_off[0..n) in array order, each followed by '\n', into d_out (16-byte
+ * aligned, out_cap bytes).  *out_bytes receives the output size; d_out ==
+ * NULL only computes it. */
+int gss_write_device32(const unsigned char *d_text, uint64_t text_bytes,
+		       const uint32_t *d_off, uint64_t n, unsigned char *d_out,
+		       uint64_t out_cap, uint64_t *out_bytes, void *stream);
+
+/* LCP-aware merge of two sorted runs of strings of one text, with their LCP
+ * arrays (na-1 and nb-1 entries, as written by the sort), replacing
+ * merge_lcp_2way (src/mergesort_lcp.cpp:133-211).  Ties take A first, so
+ * merging the stable sorts of two inputs gives the stable sort of A followed
+ * by B.  d_out receives na+nb offsets, d_out_lcp (optional) their LCP array.
+ * The LCPs of the runs decide most comparisons; only strings whose LCP with
+ * the last output is equal are compared, from that depth on. */
+int gss_merge_lcp_device32(const unsigned char *d_text, uint64_t text_bytes,
+			   const uint32_t *d_a, const uint64_t *d_a_lcp, uint64_t na,
+			   const uint32_t *d_b, const uint64_t *d_b_lcp, uint64_t nb,
+			   uint32_t *d_out, uint64_t *d_out_lcp, void *stream);
+
+/* ------------------------------------------------------------------------
  * Tuning knobs (no reference counterpart: the reference fixes its
  * thresholds at compile time, e.g. src/msd_ce.cpp:79 n < 32 and :216
  * n >= 0x10000).  Values outside the supported range are rejected with

@@ -314,3 +314,148 @@ or_sort_offsets32(int alg, const unsigned char *text, const uint32_t *off,
 	free(s);
 	return 0;
 }
+
+/* src/sortstring.c:311-335: count the delimiters, then record the start of
+ * the line each one ends and turn it into a NUL (unless it already is). */
+size_t
+or_ingest32(unsigned char *text, size_t len, int delim, uint32_t *off,
+	    size_t max)
+{
+	size_t cnt = 0;
+	for (size_t i = 0; i < len; ++i)
+		if (text[i] == (unsigned char)delim)
+			++cnt;
+	size_t line_start = 0, j = 0;
+	for (size_t i = 0; i < len; ++i)
+		if (text[i] == (unsigned char)delim) {
+			if (j < max)
+				off[j] = (uint32_t)line_start;
+			++j;
+			line_start = i + 1;
+			if (delim != '\0')
+				text[i] = '\0';
+		}
+	return cnt;
+}
+
+/* src/sortstring.c:361-391: fputs(strings[i]); fputc('\n'). */
+size_t
+or_write32(const unsigned char *text, size_t len, const uint32_t *off,
+	   size_t n, unsigned char *out, size_t cap)
+{
+	size_t total = 0;
+	for (size_t i = 0; i < n; ++i) {
+		size_t e = off[i];
+		while (e < len && text[e] != 0)
+			++e;
+		total += e - off[i] + 1;
+	}
+	if (!out || total > cap)
+		return total;
+	size_t p = 0;
+	for (size_t i = 0; i < n; ++i) {
+		for (size_t e = off[i]; e < len && text[e] != 0; ++e)
+			out[p++] = text[e];
+		out[p++] = '\n';
+	}
+	return total;
+}
+
+/* src/mergesort_lcp.cpp:102-114. */
+int
+or_compare(const unsigned char *a, const unsigned char *b, size_t depth,
+	   size_t *lcp)
+{
+	for (size_t i = depth;; ++i) {
+		const unsigned char A = a[i], B = b[i];
+		if (A == 0 || A != B) {
+			*lcp = i;
+			return (int)A - (int)B;
+		}
+	}
+}
+
+/* src/mergesort_lcp.cpp:133-211.  lcp0/lcp1 hold the LCP of the last output
+ * with the heads of run 0 / run 1; the larger one wins without comparing,
+ * equal values compare from that depth. */
+void
+or_merge_lcp32(const unsigned char *text, const uint32_t *a,
+	       const uint64_t *alcp, size_t na, const uint32_t *b,
+	       const uint64_t *blcp, size_t nb, uint32_t *out, uint64_t *olcp)
+{
+	size_t i = 0, j = 0, k = 0;
+	size_t lcp0 = 0, lcp1 = 0;
+	if (na == 0 || nb == 0) {
+		for (; i < na; ++i) {
+			out[k] = a[i];
+			if (olcp && i + 1 < na)
+				olcp[k] = alcp[i];
+			++k;
+		}
+		for (; j < nb; ++j) {
+			out[k] = b[j];
+			if (olcp && j + 1 < nb)
+				olcp[k] = blcp[j];
+			++k;
+		}
+		return;
+	}
+	{
+		size_t l01;
+		const int c = or_compare(text + a[0], text + b[0], 0, &l01);
+		if (c <= 0) {
+			out[k++] = a[i];
+			lcp0 = na > 1 ? alcp[0] : 0;
+			lcp1 = l01;
+			++i;
+		} else {
+			out[k++] = b[j];
+			lcp1 = nb > 1 ? blcp[0] : 0;
+			lcp0 = l01;
+			++j;
+		}
+	}
+	while (i < na && j < nb) {
+		if (lcp0 > lcp1) {
+			if (olcp)
+				olcp[k - 1] = lcp0;
+			out[k++] = a[i];
+			lcp0 = i + 1 < na ? alcp[i] : 0;
+			++i;
+		} else if (lcp0 < lcp1) {
+			if (olcp)
+				olcp[k - 1] = lcp1;
+			out[k++] = b[j];
+			lcp1 = j + 1 < nb ? blcp[j] : 0;
+			++j;
+		} else {
+			size_t l01;
+			const int c = or_compare(text + a[i], text + b[j], lcp0, &l01);
+			if (olcp)
+				olcp[k - 1] = lcp0;
+			if (c <= 0) {
+				out[k++] = a[i];
+				lcp1 = l01;
+				lcp0 = i + 1 < na ? alcp[i] : 0;
+				++i;
+			} else {
+				out[k++] = b[j];
+				lcp0 = l01;
+				lcp1 = j + 1 < nb ? blcp[j] : 0;
+				++j;
+			}
+		}
+	}
+	/* One run is exhausted: the other's head follows the last output with
+	 * the tracked LCP, then its own LCPs. */
+	for (int first = 1; i < na; ++i, first = 0) {
+		if (olcp)
+			olcp[k - 1] = first ? lcp0 : alcp[i - 1];
+		out[k++] = a[i];
+	}
+	for (int first = 1; j < nb; ++j, first = 0) {
+		if (olcp)
+			olcp[k - 1] = first ? lcp1 : blcp[j - 1];
+		out[k++] = b[j];
+	}
+}

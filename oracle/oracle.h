@@ -58,6 +58,36 @@ int or_check_result(unsigned char *const *strings, size_t n,
 int or_sort_offsets32(int alg, const unsigned char *text, const uint32_t *off,
 		      size_t n, uint32_t *off_out, uint64_t *lcp_out);
 
+/* Splits text[0..len) at every byte equal to `delim`: string j starts after
+ * delimiter j-1 (or at 0) and ends at delimiter j, which becomes a NUL
+ * unless delim is 0; bytes after the last delimiter belong to no string.
+ * Returns the string count and writes the first `max` offsets.
+ * Restates create_strings_delim, src/sortstring.c:311-335. */
+size_t or_ingest32(unsigned char *text, size_t len, int delim, uint32_t *off,
+		   size_t max);
+
+/* The strings text+off[i], in array order, each followed by '\n' (a string
+ * ends at its NUL or at byte len).  Returns the output size; writes it to
+ * `out` when it fits `cap`.  Restates write_result, src/sortstring.c:361-391
+ * (fputs then fputc('\n') per string). */
+size_t or_write32(const unsigned char *text, size_t len, const uint32_t *off,
+		  size_t n, unsigned char *out, size_t cap);
+
+/* strcmp order and LCP of a and b, known equal on their first `depth` bytes:
+ * returns A-B of the first differing or ending byte and its index in *lcp.
+ * Restates compare(), src/mergesort_lcp.cpp:102-114. */
+int or_compare(const unsigned char *a, const unsigned char *b, size_t depth,
+	       size_t *lcp);
+
+/* Merges the sorted runs a (na offsets, alcp: na-1 LCPs) and b into out
+ * (na+nb offsets) and olcp (na+nb-1 LCPs, or NULL); ties take a first.
+ * Restates merge_lcp_2way, src/mergesort_lcp.cpp:133-211, with LCP arrays
+ * of n-1 entries (lcp[i] = lcp(s[i], s[i+1])). */
+void or_merge_lcp32(const unsigned char *text, const uint32_t *a,
+		    const uint64_t *alcp, size_t na, const uint32_t *b,
+		    const uint64_t *blcp, size_t nb, uint32_t *out,
+		    uint64_t *olcp);
+
 #ifdef __cplusplus
 }
 #endif

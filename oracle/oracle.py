@@ -54,6 +54,16 @@ def lib() -> C.CDLL:
         l.or_lcp_array.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p]
         l.or_insertion_sort.restype = None
         l.or_insertion_sort.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        l.or_ingest32.restype = C.c_size_t
+        l.or_ingest32.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_size_t]
+        l.or_write32.restype = C.c_size_t
+        l.or_write32.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p,
+                                 C.c_size_t]
+        l.or_compare.restype = C.c_int
+        l.or_compare.argtypes = [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+        l.or_merge_lcp32.restype = None
+        l.or_merge_lcp32.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                     C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]
         for name in ("or_msd_ce2", "or_multikey_cache8"):
             fn = getattr(l, name)
             fn.restype = None
@@ -101,6 +111,41 @@ def lcp_array(text: np.ndarray, off_sorted: np.ndarray) -> np.ndarray:
 
 def key8(s: bytes, depth: int = 0) -> int:
     return lib().or_key8(bytes(s) + b"\0" * 8, depth)
+
+
+def ingest(text: np.ndarray, delim: int = ord("\n")):
+    """create_strings_delim on a copy of `text` (uint8): returns the uint32
+    offsets and the rewritten text."""
+    t = np.ascontiguousarray(text, dtype=np.uint8).copy()
+    n = lib().or_ingest32(_ptr(t), t.shape[0], int(delim), None, 0)
+    t2 = np.ascontiguousarray(text, dtype=np.uint8).copy()
+    off = np.empty(max(1, n), dtype=np.uint32)
+    lib().or_ingest32(_ptr(t2), t2.shape[0], int(delim), _ptr(off), n)
+    return off[:n], t2
+
+
+def write(text: np.ndarray, off: np.ndarray, length: int | None = None) -> bytes:
+    """write_result's bytes for the strings at `off` (in that order)."""
+    off = np.ascontiguousarray(off, dtype=np.uint32)
+    ln = text.shape[0] if length is None else int(length)
+    total = lib().or_write32(_ptr(text), ln, _ptr(off), off.shape[0], None, 0)
+    out = np.empty(max(1, total), dtype=np.uint8)
+    lib().or_write32(_ptr(text), ln, _ptr(off), off.shape[0], _ptr(out), total)
+    return out[:total].tobytes()
+
+
+def merge_lcp(text: np.ndarray, a, alcp, b, blcp):
+    """merge_lcp_2way of two sorted offset runs with their n-1 LCPs."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    alcp = np.ascontiguousarray(alcp if alcp is not None else np.zeros(1), dtype=np.uint64)
+    blcp = np.ascontiguousarray(blcp if blcp is not None else np.zeros(1), dtype=np.uint64)
+    n = a.shape[0] + b.shape[0]
+    out = np.empty(max(1, n), dtype=np.uint32)
+    ol = np.empty(max(1, n - 1), dtype=np.uint64)
+    lib().or_merge_lcp32(_ptr(text), _ptr(a), _ptr(alcp), a.shape[0], _ptr(b), _ptr(blcp),
+                         b.shape[0], _ptr(out), _ptr(ol))
+    return out[:n], ol[: max(0, n - 1)]
 
 
 # ---------------------------------------------------------------------------

@@ -1,0 +1,656 @@
+// The steps either side of the sort, on the device: ingest (a delimited text
+// becomes NUL-terminated strings and their offsets), suffix offsets, and the
+// --write formatting of a sorted offset array (gfx950, wave64).
+//
+//   ingest   create_strings_delim (src/sortstring.c:310-335): one string per
+//            delimiter byte, starting after the previous delimiter (or at 0);
+//            each delimiter becomes the string's NUL (unless it is NUL); bytes
+//            after the last delimiter belong to no string.  Two launches over
+//            the text: per-chunk delimiter counts, then (after a scan of the
+//            counts) every delimiter writes the offset of the string after it.
+//   suffixes create_suffixes (src/sortstring.c:349-358): offset i for every
+//            text position i.
+//   write    write_result (src/sortstring.c:360-391): the strings in array
+//            order, each followed by '\n'.  Lengths, a 64-bit scan of
+//            length+1, then every 16-byte block of the output is assembled
+//            in registers from the strings overlapping it and stored once.
+
+#include <mutex>
+#include <string>
+
+#include "gss_internal.h"
+#include "host.h"
+
+namespace gss {
+
+namespace {
+
+constexpr int kIngThreads = 256;
+constexpr int kIngWords = 8;  // 8-byte words per thread
+constexpr uint64_t kIngChunk = (uint64_t)kIngThreads * kIngWords * 8;  // 16 KiB per block
+
+constexpr int kScanThreads64 = 256;
+constexpr int kScanItems64 = 16;
+constexpr uint64_t kScanBlock64 = (uint64_t)kScanThreads64 * kScanItems64;
+
+// Exact per-byte equality flags: bit 8b+7 set when byte b of x equals byte
+// b of `pattern` (no false positives: the additions never carry).
+__device__ __forceinline__ uint64_t eq_bytes(uint64_t x, uint64_t pattern) {
+  const uint64_t v = x ^ pattern;
+  const uint64_t t = ((v & 0x7F7F7F7F7F7F7F7Full) + 0x7F7F7F7F7F7F7F7Full) | v;
+  return ~t & 0x8080808080808080ull;
+}
+
+__device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t v) {
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint64_t t = __shfl_up(v, o, kWave);
+    if ((int)lane_id() >= o) v += t;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan of 64-bit values; scratch holds BLOCK/64 + 1.
+template <int BLOCK>
+__device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* scratch,
+                                                      uint64_t* total) {
+  constexpr int W = BLOCK / kWave;
+  const uint64_t inc = wave_incl_scan64(v);
+  const int wid = threadIdx.x / kWave;
+  if ((int)lane_id() == kWave - 1) scratch[wid] = inc;
+  __syncthreads();
+  if (wid == 0) {
+    const uint64_t t = (int)lane_id() < W ? scratch[lane_id()] : 0ull;
+    const uint64_t ti = wave_incl_scan64(t);
+    if ((int)lane_id() < W) scratch[lane_id()] = ti - t;
+    if ((int)lane_id() == W - 1) scratch[W] = ti;
+  }
+  __syncthreads();
+  const uint64_t r = scratch[wid] + inc - v;
+  *total = scratch[W];
+  __syncthreads();
+  return r;
+}
+
+// Text word w of a text of `len` bytes, bytes at or beyond len cleared.
+__device__ __forceinline__ uint64_t masked_word(const uint64_t* __restrict__ t64, uint64_t len,
+                                                uint64_t w) {
+  const uint64_t b = w * 8;
+  if (b >= len) return 0;
+  const uint64_t x = t64[w];
+  const uint64_t have = len - b;
+  return have >= 8 ? x : (x & ((1ull << (8 * have)) - 1ull));
+}
+
+// ---------------------------------------------------------------------------
+// Ingest
+
+// Delimiter flags of this thread's words; bytes beyond len never match.
+__device__ __forceinline__ void ingest_flags(const uint64_t* __restrict__ t64, uint64_t len,
+                                             uint64_t pat, uint64_t w0, uint64_t (&f)[kIngWords]) {
+#pragma unroll
+  for (int q = 0; q < kIngWords; ++q) {
+    const uint64_t w = w0 + (uint64_t)q * kIngThreads;  // coalesced: word-interleaved
+    const uint64_t b = w * 8;
+    uint64_t m = 0;
+    if (b < len) {
+      m = eq_bytes(t64[w], pat);
+      const uint64_t have = len - b;
+      if (have < 8) m &= (1ull << (8 * have)) - 1ull;
+    }
+    f[q] = m;
+  }
+}
+
+// The word index of slot q of thread t in block b: blocks cover 16 KiB
+// chunks; inside a chunk thread t owns words t, t+256, ... so that loads are
+// coalesced, and the order of delimiters is word order (q-major within a
+// thread is NOT text order, so the emit kernel ranks by word index).
+__global__ __launch_bounds__(kIngThreads) void k_ingest_count(const uint64_t* __restrict__ t64,
+                                                             uint64_t len, uint64_t pat,
+                                                             uint64_t* __restrict__ counts) {
+  __shared__ uint64_t scratch[kIngThreads / kWave + 1];
+  const uint64_t w0 = (uint64_t)blockIdx.x * (kIngChunk / 8) + threadIdx.x;
+  uint64_t f[kIngWords];
+  ingest_flags(t64, len, pat, w0, f);
+  uint64_t c = 0;
+#pragma unroll
+  for (int q = 0; q < kIngWords; ++q) c += __popcll(f[q]);
+  uint64_t total;
+  block_excl_scan64<kIngThreads>(c, scratch, &total);
+  if (threadIdx.x == 0) counts[blockIdx.x] = total;
+}
+
+// Every delimiter j at byte i writes off[j+1] = i+1 (j+1 < n) and, unless
+// the delimiter is NUL, becomes a NUL.  Block bases come from the scanned
+// counts; inside a block the delimiters are ranked in text order: pass q
+// covers words [q*256, (q+1)*256) of the chunk, one per thread.
+__global__ __launch_bounds__(kIngThreads) void k_ingest_emit(uint64_t* __restrict__ t64,
+                                                            uint64_t len, uint64_t pat,
+                                                            bool clear, const uint64_t* __restrict__ base,
+                                                            uint64_t n, uint32_t* __restrict__ off) {
+  __shared__ uint64_t scratch[kIngThreads / kWave + 1];
+  const uint64_t w0 = (uint64_t)blockIdx.x * (kIngChunk / 8) + threadIdx.x;
+  uint64_t f[kIngWords];
+  ingest_flags(t64, len, pat, w0, f);
+  uint64_t run = base[blockIdx.x];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) off[0] = 0;
+#pragma unroll
+  for (int q = 0; q < kIngWords; ++q) {
+    uint64_t tot;
+    const uint64_t c = __popcll(f[q]);
+    uint64_t j = run + block_excl_scan64<kIngThreads>(c, scratch, &tot);
+    const uint64_t w = w0 + (uint64_t)q * kIngThreads;
+    uint64_t m = f[q];
+    while (m) {
+      const uint32_t bit = __builtin_ctzll(m);
+      m &= m - 1;
+      const uint64_t i = w * 8 + (bit >> 3);
+      if (j + 1 < n) off[j + 1] = (uint32_t)(i + 1);
+      ++j;
+    }
+    if (clear && f[q]) {  // delimiters -> NUL
+      if (w * 8 + 8 <= len) {
+        t64[w] &= ~((f[q] >> 7) * 0xFFull);
+      } else {  // the text's last, partial word: touch only its own bytes
+        unsigned char* t8 = reinterpret_cast<unsigned char*>(t64);
+        for (uint64_t mm = f[q]; mm; mm &= mm - 1) t8[w * 8 + (__builtin_ctzll(mm) >> 3)] = 0;
+      }
+    }
+    run += tot;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 64-bit scan of per-item values (reduce, scan of block sums, down-sweep).
+// Values: counts[i] (ingest) or len[i] + 1 (write).
+
+template <bool PLUS_ONE, typename T>
+__global__ __launch_bounds__(kScanThreads64) void k_scan64_reduce(const T* __restrict__ in,
+                                                                uint64_t n,
+                                                                uint64_t* __restrict__ bsum) {
+  __shared__ uint64_t scratch[kScanThreads64 / kWave + 1];
+  const uint64_t b0 = (uint64_t)blockIdx.x * kScanBlock64;
+  uint64_t s = 0;
+#pragma unroll
+  for (int q = 0; q < kScanItems64; ++q) {
+    const uint64_t i = b0 + (uint64_t)q * kScanThreads64 + threadIdx.x;
+    if (i < n) s += (uint64_t)in[i] + (PLUS_ONE ? 1u : 0u);
+  }
+  uint64_t total;
+  block_excl_scan64<kScanThreads64>(s, scratch, &total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan64_blocks(uint64_t* __restrict__ bsum, uint64_t nb,
+                                                       uint64_t* __restrict__ total_out) {
+  __shared__ uint64_t scratch[1024 / kWave + 1];
+  const uint64_t per = (nb + 1023) / 1024;
+  const uint64_t a = threadIdx.x * per;
+  uint64_t s = 0;
+  for (uint64_t i = a; i < a + per && i < nb; ++i) s += bsum[i];
+  uint64_t total;
+  uint64_t r = block_excl_scan64<1024>(s, scratch, &total);
+  for (uint64_t i = a; i < a + per && i < nb; ++i) {
+    const uint64_t v = bsum[i];
+    bsum[i] = r;
+    r += v;
+  }
+  if (threadIdx.x == 0) *total_out = total;
+}
+
+// out[i] = exclusive prefix of the values; out[n] = total (when WRITE_END).
+template <bool PLUS_ONE, typename T>
+__global__ __launch_bounds__(kScanThreads64) void k_scan64_down(const T* __restrict__ in,
+                                                              uint64_t n,
+                                                              const uint64_t* __restrict__ bsum,
+                                                              uint64_t* __restrict__ out) {
+  __shared__ uint64_t scratch[kScanThreads64 / kWave + 1];
+  const uint64_t b0 = (uint64_t)blockIdx.x * kScanBlock64 + (uint64_t)threadIdx.x * kScanItems64;
+  uint64_t v[kScanItems64];
+  uint64_t s = 0;
+#pragma unroll
+  for (int q = 0; q < kScanItems64; ++q) {
+    const uint64_t i = b0 + q;
+    v[q] = i < n ? (uint64_t)in[i] + (PLUS_ONE ? 1u : 0u) : 0ull;
+    s += v[q];
+  }
+  uint64_t total;
+  uint64_t r = bsum[blockIdx.x] + block_excl_scan64<kScanThreads64>(s, scratch, &total);
+#pragma unroll
+  for (int q = 0; q < kScanItems64; ++q) {
+    const uint64_t i = b0 + q;
+    if (i < n) out[i] = r;
+    r += v[q];
+    if (i + 1 == n) out[n] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Suffix offsets and --write
+
+__global__ void k_iota32(uint32_t* __restrict__ off, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    off[i] = (uint32_t)i;
+}
+
+// Length of every string: bytes before its NUL or the end of the text.
+__global__ __launch_bounds__(256) void k_strlen(const uint64_t* __restrict__ t64, uint64_t len,
+                                               const uint32_t* __restrict__ off, uint64_t n,
+                                               uint32_t* __restrict__ lens) {
+  const uint64_t nwords = (len + 7) / 8;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint64_t o = off[i];
+    uint64_t w = o >> 3;
+    // First word: bytes before the string count as non-zero.
+    uint64_t x = masked_word(t64, len, w) | ((1ull << (8 * (o & 7))) - 1ull);
+    uint64_t z = eq_bytes(x, 0);
+    while (!z && w + 1 < nwords) {
+      ++w;
+      x = masked_word(t64, len, w);
+      z = eq_bytes(x, 0);
+    }
+    const uint64_t end = z ? w * 8 + (__builtin_ctzll(z) >> 3) : len;  // NUL or text end
+    lens[i] = (uint32_t)(end - o);
+  }
+}
+
+// first[b] = the string whose output range holds byte 16*b.
+__global__ __launch_bounds__(256) void k_block_owner(const uint64_t* __restrict__ pos, uint64_t n,
+                                                    uint32_t* __restrict__ first) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
+    const uint64_t a = pos[s], e = pos[s + 1];
+    for (uint64_t b = (a + 15) / 16; b * 16 < e; ++b) first[b] = (uint32_t)s;
+  }
+}
+
+// One thread per 16 output bytes.
+__global__ __launch_bounds__(256) void k_write(const uint64_t* __restrict__ t64, uint64_t len,
+                                              const uint32_t* __restrict__ off,
+                                              const uint64_t* __restrict__ pos, uint64_t n,
+                                              const uint32_t* __restrict__ first,
+                                              uint64_t total, unsigned char* __restrict__ out) {
+  const uint64_t nwords = (len + 7) / 8;
+  const uint64_t nblk = (total + 15) / 16;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nblk; b += stride) {
+    const uint64_t p0 = b * 16, p1 = p0 + 16 < total ? p0 + 16 : total;
+    uint64_t acc[2] = {0, 0};  // little-endian output bytes
+    uint64_t s = first[b];
+    uint64_t p = p0;
+    while (p < p1 && s < n) {
+      const uint64_t ps = pos[s], pe = pos[s + 1];  // string s: [ps, pe-1), '\n' at pe-1
+      const uint64_t hi = pe < p1 ? pe : p1;
+      // Up to 16 string bytes starting at byte (p - ps) of string s.
+      const uint64_t src = (uint64_t)off[s] + (p - ps);
+      uint64_t raw[3];
+      text_words<3>(t64, nwords, src >> 3, raw);
+      const uint32_t sh = (uint32_t)(src & 7) * 8;
+      const uint64_t v0 = funnel(raw[0], raw[1], sh), v1 = funnel(raw[1], raw[2], sh);
+      for (uint64_t q = p; q < hi; ++q) {
+        const uint32_t k = (uint32_t)(q - p);  // byte of (v0, v1)
+        const uint64_t byte = q + 1 == pe ? (uint64_t)'\n'
+                                          : (((k < 8 ? v0 : v1) >> (8 * (k & 7))) & 0xFFull);
+        const uint32_t d = (uint32_t)(q - p0);
+        acc[d >> 3] |= byte << (8 * (d & 7));
+      }
+      p = hi;
+      ++s;
+    }
+    if (p1 - p0 == 16) {
+      *reinterpret_cast<TextPair*>(out + p0) = TextPair{acc[0], acc[1]};
+    } else {
+      for (uint64_t q = p0; q < p1; ++q) {
+        const uint32_t d = (uint32_t)(q - p0);
+        out[q] = (unsigned char)(acc[d >> 3] >> (8 * (d & 7)));
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LCP-aware merge of two sorted runs (merge_lcp_2way, src/mergesort_lcp.cpp:
+// 133-211, restated for a parallel merge path).  Ties take A first, so the
+// merge of two stable sorts is the stable sort of A followed by B.
+
+constexpr int kMergeThreads = 256;
+constexpr int kMergeItems = 8;
+constexpr uint64_t kMergeTile = (uint64_t)kMergeThreads * kMergeItems;
+
+struct StrCmp {
+  int sign;      // <0, 0, >0 as strcmp
+  uint64_t lcp;  // equal bytes before the first mismatch or NUL
+};
+
+// Compares the strings at byte offsets x and y, known equal on their first
+// h bytes, 8 bytes at a time (the key packing of multikey_cache).
+__device__ __forceinline__ StrCmp str_compare(const uint64_t* __restrict__ t64, uint64_t nwords,
+                                              uint64_t x, uint64_t y, uint64_t h) {
+  for (uint64_t d = h;; d += 8) {
+    const uint64_t kx = load_key(t64, nwords, x + d), ky = load_key(t64, nwords, y + d);
+    if (kx != ky) return StrCmp{kx < ky ? -1 : 1, d + key_lcp(kx, ky)};
+    if ((kx & 0xFFu) == 0) return StrCmp{0, d + key_len(kx)};
+  }
+}
+
+// Smallest i in [lo, hi] with A[i] > B[d-1-i] (ties A first): the number of
+// A strings among the first d outputs.
+__device__ __forceinline__ uint64_t merge_path(const uint64_t* __restrict__ t64, uint64_t nwords,
+                                               const uint32_t* __restrict__ a,
+                                               const uint32_t* __restrict__ b, uint64_t d,
+                                               uint64_t lo, uint64_t hi) {
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (str_compare(t64, nwords, a[mid], b[d - 1 - mid], 0).sign <= 0)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// split[p] = A strings among the first min(p * kMergeTile, na + nb) outputs.
+__global__ __launch_bounds__(256) void k_merge_partition(const uint64_t* __restrict__ t64,
+                                                        uint64_t nwords,
+                                                        const uint32_t* __restrict__ a, uint64_t na,
+                                                        const uint32_t* __restrict__ b, uint64_t nb,
+                                                        uint64_t nparts, uint64_t* __restrict__ split) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > nparts) return;
+  const uint64_t total = na + nb;
+  const uint64_t d = p * kMergeTile < total ? p * kMergeTile : total;
+  const uint64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
+  split[p] = merge_path(t64, nwords, a, b, d, lo, hi);
+}
+
+// Each thread merges kMergeItems outputs from its own merge-path split and
+// takes one more step to learn the LCP of its last output with the next.
+// LCP merge state: s = last output, ha = lcp(s, A[i]), hb = lcp(s, B[j]).
+// ha > hb means A[i] < B[j] (and lcp(A[i], B[j]) = hb), ha < hb the
+// reverse; only equal values need a comparison, starting at that depth.
+__global__ __launch_bounds__(kMergeThreads) void k_merge(
+    const uint64_t* __restrict__ t64, uint64_t nwords, const uint32_t* __restrict__ a,
+    const uint64_t* __restrict__ alcp, uint64_t na, const uint32_t* __restrict__ b,
+    const uint64_t* __restrict__ blcp, uint64_t nb, const uint64_t* __restrict__ split,
+    uint32_t* __restrict__ out, uint64_t* __restrict__ olcp) {
+  const uint64_t total = na + nb;
+  const uint64_t k0 = (uint64_t)blockIdx.x * kMergeTile;
+  const uint64_t k1 = k0 + kMergeTile < total ? k0 + kMergeTile : total;
+  const uint64_t d = k0 + (uint64_t)threadIdx.x * kMergeItems;
+  if (d >= k1) return;
+  const uint64_t i0 = split[blockIdx.x], i1 = split[blockIdx.x + 1];
+  const uint64_t j0 = k0 - i0, j1 = k1 - i1;
+  // Narrow to this block's ranges: i in [i0, i1], d - i in [j0, j1].
+  const uint64_t lo = (d > j1 && d - j1 > i0) ? d - j1 : i0;
+  const uint64_t hi = d - j0 < i1 ? d - j0 : i1;
+  uint64_t i = merge_path(t64, nwords, a, b, d, lo, hi);
+  uint64_t j = d - i;
+  const uint64_t end = d + kMergeItems < k1 ? d + kMergeItems : k1;
+  bool have_s = false;
+  uint64_t ha = 0, hb = 0;
+  for (uint64_t k = d; k < total && k <= end; ++k) {
+    bool take_a;
+    uint64_t lprev = 0;  // lcp(previous output, this output)
+    if (i == na) {
+      take_a = false;
+      lprev = hb;
+    } else if (j == nb) {
+      take_a = true;
+      lprev = ha;
+    } else if (!have_s) {
+      const StrCmp c = str_compare(t64, nwords, a[i], b[j], 0);
+      take_a = c.sign <= 0;
+      if (take_a) hb = c.lcp; else ha = c.lcp;
+    } else if (ha != hb) {
+      take_a = ha > hb;
+      lprev = take_a ? ha : hb;
+    } else {
+      const StrCmp c = str_compare(t64, nwords, a[i], b[j], ha);
+      take_a = c.sign <= 0;
+      lprev = ha;
+      if (take_a) hb = c.lcp; else ha = c.lcp;
+    }
+    if (have_s && olcp) olcp[k - 1] = lprev;
+    if (k == end) break;  // the extra step: only the LCP above
+    if (take_a) {
+      out[k] = a[i];
+      ha = i + 1 < na ? alcp[i] : 0;
+      ++i;
+    } else {
+      out[k] = b[j];
+      hb = j + 1 < nb ? blcp[j] : 0;
+      ++j;
+    }
+    have_s = true;
+  }
+}
+
+uint32_t grid_cap(uint64_t items, uint32_t block, uint32_t cap = 65536) {
+  uint64_t g = (items + block - 1) / block;
+  if (g < 1) g = 1;
+  return (uint32_t)(g > cap ? cap : g);
+}
+
+// Per-device grow-only scratch and a pinned word for read-backs; calls that
+// use it hold its mutex for their whole duration.
+struct Scratch {
+  std::mutex mu;
+  char* dev = nullptr;
+  size_t bytes = 0;
+  uint64_t* host = nullptr;
+};
+
+constexpr int kMaxDev = 16;
+Scratch g_scratch[kMaxDev];
+
+#define PL_HIP(call)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(GSS_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int scratch_for(Scratch** out) {
+  int dev = 0;
+  PL_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDev) return fail(GSS_E_DEVICE, "device id out of range");
+  *out = &g_scratch[dev];
+  return GSS_OK;
+}
+
+// Ensures `bytes` of device scratch (called with the mutex held).
+int scratch_reserve(Scratch& s, size_t bytes, hipStream_t st) {
+  if (!s.host) PL_HIP(hipHostMalloc((void**)&s.host, 64, hipHostMallocDefault));
+  if (bytes <= s.bytes) return GSS_OK;
+  PL_HIP(hipStreamSynchronize(st));
+  if (s.dev) (void)hipFree(s.dev);
+  s.dev = nullptr;
+  s.bytes = 0;
+  if (hipMalloc((void**)&s.dev, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(GSS_E_NOMEM, "device scratch of " + std::to_string(bytes) + " bytes");
+  }
+  s.bytes = bytes;
+  return GSS_OK;
+}
+
+size_t al256(size_t v) { return (v + 255) / 256 * 256; }
+
+// Exclusive 64-bit scan of values (+1 when PLUS_ONE) into out[0..n]; writes
+// the total to *host_total after a stream sync.  bsum: scratch of
+// ceil(n / kScanBlock64) + 1 words.
+template <bool PLUS_ONE, typename T>
+int scan64(const T* in, uint64_t n, uint64_t* bsum, uint64_t* out, uint64_t* d_total,
+           uint64_t* host_total, hipStream_t st) {
+  const uint64_t nb = (n + kScanBlock64 - 1) / kScanBlock64;
+  if (nb) k_scan64_reduce<PLUS_ONE, T><<<(uint32_t)nb, kScanThreads64, 0, st>>>(in, n, bsum);
+  k_scan64_blocks<<<1, 1024, 0, st>>>(bsum, nb, d_total);
+  if (out && nb) k_scan64_down<PLUS_ONE, T><<<(uint32_t)nb, kScanThreads64, 0, st>>>(in, n, bsum, out);
+  PL_HIP(hipGetLastError());
+  PL_HIP(hipMemcpyAsync(host_total, d_total, 8, hipMemcpyDeviceToHost, st));
+  PL_HIP(hipStreamSynchronize(st));
+  return GSS_OK;
+}
+
+}  // namespace
+
+}  // namespace gss
+
+using namespace gss;
+
+extern "C" {
+
+int gss_ingest_device32(unsigned char* d_text, uint64_t text_bytes, int delim, uint32_t* d_off,
+                        uint64_t max_strings, uint64_t* n_strings, void* stream) {
+  if (n_strings) *n_strings = 0;
+  if (!d_text || !n_strings) return fail(GSS_E_INVALID, "NULL text or count pointer");
+  if (reinterpret_cast<uintptr_t>(d_text) & 7)
+    return fail(GSS_E_INVALID, "the device text must be 8-byte aligned");
+  if (delim < 0 || delim > 255) return fail(GSS_E_INVALID, "the delimiter must be a byte value");
+  if (text_bytes >= (1ull << 32))
+    return fail(GSS_E_TOO_LARGE, "text of 4 GiB or more: offsets are 32-bit");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  Scratch* sc = nullptr;
+  int rc = scratch_for(&sc);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(sc->mu);
+  const uint64_t nblk = (text_bytes + kIngChunk - 1) / kIngChunk;
+  const uint64_t nsb = (nblk + kScanBlock64 - 1) / kScanBlock64 + 1;
+  rc = scratch_reserve(*sc, al256(8 * (nblk + 1)) * 2 + al256(8 * nsb) + 256, st);
+  if (rc) return rc;
+  uint64_t* counts = reinterpret_cast<uint64_t*>(sc->dev);
+  uint64_t* bases = reinterpret_cast<uint64_t*>(sc->dev + al256(8 * (nblk + 1)));
+  uint64_t* bsum = reinterpret_cast<uint64_t*>(sc->dev + 2 * al256(8 * (nblk + 1)));
+  uint64_t* d_total = reinterpret_cast<uint64_t*>(sc->dev + 2 * al256(8 * (nblk + 1)) +
+                                                  al256(8 * nsb));
+  const uint64_t pat = 0x0101010101010101ull * (uint64_t)(uint8_t)delim;
+  uint64_t* t64 = reinterpret_cast<uint64_t*>(d_text);
+  if (nblk) {
+    k_ingest_count<<<(uint32_t)nblk, kIngThreads, 0, st>>>(t64, text_bytes, pat, counts);
+    PL_HIP(hipGetLastError());
+  }
+  rc = scan64<false, uint64_t>(counts, nblk, bsum, bases, d_total, sc->host, st);
+  if (rc) return rc;
+  const uint64_t n = *sc->host;
+  if (n == 0) return fail(GSS_E_INVALID, "unable to read any lines from the input");
+  if (n > max_strings || !d_off) {
+    *n_strings = n;
+    return d_off ? fail(GSS_E_TOO_LARGE, std::to_string(n) + " strings exceed max_strings")
+                 : GSS_OK;  // size query
+  }
+  k_ingest_emit<<<(uint32_t)nblk, kIngThreads, 0, st>>>(t64, text_bytes, pat, delim != 0, bases,
+                                                        n, d_off);
+  PL_HIP(hipGetLastError());
+  PL_HIP(hipStreamSynchronize(st));
+  *n_strings = n;
+  return GSS_OK;
+}
+
+int gss_merge_lcp_device32(const unsigned char* d_text, uint64_t text_bytes, const uint32_t* d_a,
+                           const uint64_t* d_a_lcp, uint64_t na, const uint32_t* d_b,
+                           const uint64_t* d_b_lcp, uint64_t nb, uint32_t* d_out,
+                           uint64_t* d_out_lcp, void* stream) {
+  const uint64_t total = na + nb;
+  if (total == 0) return GSS_OK;
+  if (!d_text || !d_out || (na && !d_a) || (nb && !d_b))
+    return fail(GSS_E_INVALID, "NULL text, run or output pointer");
+  if ((na > 1 && !d_a_lcp) || (nb > 1 && !d_b_lcp))
+    return fail(GSS_E_INVALID, "each run of two or more strings needs its LCP array");
+  if (reinterpret_cast<uintptr_t>(d_text) & 7)
+    return fail(GSS_E_INVALID, "the device text must be 8-byte aligned");
+  if (total >= (1ull << 32)) return fail(GSS_E_TOO_LARGE, "at most 2^32-1 strings");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  Scratch* sc = nullptr;
+  int rc = scratch_for(&sc);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(sc->mu);
+  const uint64_t nparts = (total + kMergeTile - 1) / kMergeTile;
+  rc = scratch_reserve(*sc, al256(8 * (nparts + 1)), st);
+  if (rc) return rc;
+  uint64_t* split = reinterpret_cast<uint64_t*>(sc->dev);
+  const uint64_t* t64 = reinterpret_cast<const uint64_t*>(d_text);
+  const uint64_t nwords = (text_bytes + 7) / 8;
+  k_merge_partition<<<(uint32_t)((nparts + 1 + 255) / 256), 256, 0, st>>>(t64, nwords, d_a, na, d_b,
+                                                                         nb, nparts, split);
+  k_merge<<<(uint32_t)nparts, kMergeThreads, 0, st>>>(t64, nwords, d_a, d_a_lcp, na, d_b, d_b_lcp,
+                                                      nb, split, d_out, d_out_lcp);
+  PL_HIP(hipGetLastError());
+  PL_HIP(hipStreamSynchronize(st));
+  return GSS_OK;
+}
+
+int gss_suffixes_device32(uint64_t text_bytes, uint32_t* d_off, void* stream) {
+  if (!d_off && text_bytes) return fail(GSS_E_INVALID, "NULL offset array");
+  if (text_bytes >= (1ull << 31))
+    return fail(GSS_E_TOO_LARGE, "at most 2^31-1 suffixes on the 32-bit path");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (text_bytes) k_iota32<<<grid_cap(text_bytes, 256), 256, 0, st>>>(d_off, text_bytes);
+  PL_HIP(hipGetLastError());
+  PL_HIP(hipStreamSynchronize(st));
+  return GSS_OK;
+}
+
+int gss_write_device32(const unsigned char* d_text, uint64_t text_bytes, const uint32_t* d_off,
+                       uint64_t n, unsigned char* d_out, uint64_t out_cap, uint64_t* out_bytes,
+                       void* stream) {
+  if (out_bytes) *out_bytes = 0;
+  if (!out_bytes) return fail(GSS_E_INVALID, "NULL out_bytes pointer");
+  if (n == 0) return GSS_OK;
+  if (!d_text || !d_off) return fail(GSS_E_INVALID, "NULL text or offsets");
+  if (reinterpret_cast<uintptr_t>(d_text) & 7)
+    return fail(GSS_E_INVALID, "the device text must be 8-byte aligned");
+  if (d_out && (reinterpret_cast<uintptr_t>(d_out) & 15))
+    return fail(GSS_E_INVALID, "the output buffer must be 16-byte aligned");
+  if (n >= (1ull << 32)) return fail(GSS_E_TOO_LARGE, "at most 2^32-1 strings");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  Scratch* sc = nullptr;
+  int rc = scratch_for(&sc);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(sc->mu);
+  const uint64_t nsb = (n + kScanBlock64 - 1) / kScanBlock64 + 1;
+  const size_t head = al256(4 * n) + al256(8 * (n + 1)) + al256(8 * nsb) + 256;
+  rc = scratch_reserve(*sc, head, st);
+  if (rc) return rc;
+  uint32_t* lens = reinterpret_cast<uint32_t*>(sc->dev);
+  uint64_t* pos = reinterpret_cast<uint64_t*>(sc->dev + al256(4 * n));
+  uint64_t* bsum = reinterpret_cast<uint64_t*>(sc->dev + al256(4 * n) + al256(8 * (n + 1)));
+  uint64_t* d_total = reinterpret_cast<uint64_t*>(sc->dev + head - 256);
+  const uint64_t* t64 = reinterpret_cast<const uint64_t*>(d_text);
+  k_strlen<<<grid_cap(n, 256), 256, 0, st>>>(t64, text_bytes, d_off, n, lens);
+  PL_HIP(hipGetLastError());
+  rc = scan64<true, uint32_t>(lens, n, bsum, pos, d_total, sc->host, st);
+  if (rc) return rc;
+  const uint64_t total = *sc->host;
+  *out_bytes = total;
+  if (!d_out) return GSS_OK;  // size query
+  if (total > out_cap)
+    return fail(GSS_E_TOO_LARGE, "output of " + std::to_string(total) + " bytes exceeds out_cap");
+  const uint64_t nblk = (total + 15) / 16;
+  // The block-owner table follows the scan arrays (the scratch may grow: the
+  // lengths and positions are rebuilt after a regrow).
+  const size_t need = head + al256(4 * nblk);
+  if (need > sc->bytes) {
+    rc = scratch_reserve(*sc, need, st);
+    if (rc) return rc;
+    lens = reinterpret_cast<uint32_t*>(sc->dev);
+    pos = reinterpret_cast<uint64_t*>(sc->dev + al256(4 * n));
+    bsum = reinterpret_cast<uint64_t*>(sc->dev + al256(4 * n) + al256(8 * (n + 1)));
+    d_total = reinterpret_cast<uint64_t*>(sc->dev + head - 256);
+    k_strlen<<<grid_cap(n, 256), 256, 0, st>>>(t64, text_bytes, d_off, n, lens);
+    PL_HIP(hipGetLastError());
+    rc = scan64<true, uint32_t>(lens, n, bsum, pos, d_total, sc->host, st);
+    if (rc) return rc;
+  }
+  uint32_t* first = reinterpret_cast<uint32_t*>(sc->dev + head);
+  k_block_owner<<<grid_cap(n, 256), 256, 0, st>>>(pos, n, first);
+  k_write<<<grid_cap(nblk, 256), 256, 0, st>>>(t64, text_bytes, d_off, pos, n, first, total,
+                                                d_out);
+  PL_HIP(hipGetLastError());
+  PL_HIP(hipStreamSynchronize(st));
+  return GSS_OK;
+}
+
+}  // extern "C"

@@ -90,6 +90,14 @@ _SIGNATURES = {
     "gss_server_done": (C.c_int, [C.c_void_p, C.c_int64]),
     "gss_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "gss_server_destroy": (None, [C.c_void_p]),
+    "gss_ingest_device32": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_uint64,
+                                      C.POINTER(C.c_uint64), C.c_void_p]),
+    "gss_suffixes_device32": (C.c_int, [C.c_uint64, C.c_void_p, C.c_void_p]),
+    "gss_write_device32": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p,
+                                     C.c_uint64, C.POINTER(C.c_uint64), C.c_void_p]),
+    "gss_merge_lcp_device32": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                         C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint64,
+                                         C.c_void_p, C.c_void_p, C.c_void_p]),
     "gss_get_tuning": (C.c_int, [C.POINTER(Tuning)]),
     "gss_set_tuning": (C.c_int, [C.POINTER(Tuning)]),
     "gss_get_stats": (C.c_int, [C.POINTER(Stats)]),
@@ -265,6 +273,79 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+def _stream_ptr(t, stream):
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream(t.device)
+    return s.cuda_stream
+
+
+def _ptr(t):
+    return t.data_ptr() if t is not None else None
+
+
+def ingest(text, delim=b"\n", text_bytes=None, stream=None):
+    """Device-side create_strings_delim: splits the uint8 device tensor
+    `text` in place at every `delim` byte (each becomes a NUL unless it is
+    NUL) and returns the int32 offsets of the strings (one per delimiter)."""
+    import torch
+
+    d = delim[0] if isinstance(delim, (bytes, bytearray)) else int(delim)
+    tb = text.numel() if text_bytes is None else int(text_bytes)
+    n = C.c_uint64()
+    st = _stream_ptr(text, stream)
+    _check(lib().gss_ingest_device32(text.data_ptr(), tb, d, None, 0, C.byref(n), st),
+           "gss_ingest_device32")
+    off = torch.empty(max(1, n.value), dtype=torch.int32, device=text.device)
+    _check(lib().gss_ingest_device32(text.data_ptr(), tb, d, off.data_ptr(), n.value, C.byref(n),
+                                     st), "gss_ingest_device32")
+    return off[: n.value]
+
+
+def suffixes(text_bytes, device, stream=None):
+    """Device-side create_suffixes: int32 offsets 0..text_bytes-1."""
+    import torch
+
+    off = torch.empty(max(1, int(text_bytes)), dtype=torch.int32, device=device)
+    _check(lib().gss_suffixes_device32(int(text_bytes), off.data_ptr(), _stream_ptr(off, stream)),
+           "gss_suffixes_device32")
+    return off[: int(text_bytes)]
+
+
+def write(text, offsets, text_bytes=None, stream=None):
+    """Device-side write_result: the strings at `offsets`, in order, each
+    followed by '\\n', as a uint8 device tensor."""
+    import torch
+
+    tb = text.numel() if text_bytes is None else int(text_bytes)
+    n = offsets.numel()
+    size = C.c_uint64()
+    st = _stream_ptr(text, stream)
+    _check(lib().gss_write_device32(text.data_ptr(), tb, offsets.data_ptr(), n, None, 0,
+                                    C.byref(size), st), "gss_write_device32")
+    out = torch.empty(max(16, size.value), dtype=torch.uint8, device=text.device)
+    _check(lib().gss_write_device32(text.data_ptr(), tb, offsets.data_ptr(), n, out.data_ptr(),
+                                    out.numel(), C.byref(size), st), "gss_write_device32")
+    return out[: size.value]
+
+
+def merge_lcp(text, a, a_lcp, b, b_lcp, text_bytes=None, want_lcp=True, stream=None):
+    """LCP-aware merge of two sorted runs (int32 offset tensors with their
+    int64 LCP tensors of n-1 entries); returns (offsets, lcp or None)."""
+    import torch
+
+    tb = text.numel() if text_bytes is None else int(text_bytes)
+    na, nb = a.numel(), b.numel()
+    out = torch.empty(max(1, na + nb), dtype=torch.int32, device=text.device)
+    ol = torch.empty(max(1, na + nb - 1), dtype=torch.int64, device=text.device) if want_lcp else None
+    _check(lib().gss_merge_lcp_device32(text.data_ptr(), tb, _ptr(a) if na else None,
+                                        _ptr(a_lcp) if na > 1 else None, na,
+                                        _ptr(b) if nb else None, _ptr(b_lcp) if nb > 1 else None,
+                                        nb, out.data_ptr(), _ptr(ol), _stream_ptr(text, stream)),
+           "gss_merge_lcp_device32")
+    return out[: na + nb], (ol[: max(0, na + nb - 1)] if want_lcp else None)
 
 
 class Server:

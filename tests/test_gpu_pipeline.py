@@ -1,0 +1,194 @@
+"""GPU: the steps either side of the sort -- device ingest of a delimited
+text, suffix offsets, --write formatting and the LCP-aware merge of sorted
+runs -- each bit-exact against the oracle (whose ingest and write are pinned
+against the reference's own sortstring, and whose merge against brute
+force), and the whole file -> ingest -> sort -> write pipeline on the device
+against the oracle's."""
+
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from gpu_util import device, log
+
+pytestmark = pytest.mark.gpu
+
+import gss  # noqa: E402
+import gss.gen as gen  # noqa: E402
+
+
+def _dev(a):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device())
+
+
+def _text_tensor(data: bytes, pad=16):
+    return _dev(np.frombuffer(data + b"\0" * pad, np.uint8).copy())
+
+
+def _edge_text(delim):
+    d = bytes([delim])
+    lines = [b"bb", b"", b"ab", b"", b"", b"a" * 40, b"\x01\xff", b"zz", b"ab"]
+    return d.join(lines) + d + b"tail-without-delimiter"
+
+
+@pytest.mark.parametrize("delim", [ord("\n"), 0])
+@pytest.mark.parametrize("kind", ["edge", "url_like", "dna", "long_lines"])
+def test_ingest_matches_oracle(delim, kind, oracle_mod):
+    if kind == "edge":
+        data = _edge_text(delim)
+    elif kind == "long_lines":  # lines spanning many 16 KiB ingest chunks
+        rng = np.random.default_rng(3)
+        parts = [bytes(rng.integers(97, 123, size=int(rng.integers(0, 70000)), dtype=np.uint8))
+                 for _ in range(12)]
+        data = bytes([delim]).join(parts) + bytes([delim])
+    else:
+        ss = gen.make({"url_like": 1, "dna": 2}[kind], n=200_000, seed=7)
+        t = ss.text[: ss.text_bytes].copy()
+        t[t == 0] = delim
+        data = t.tobytes()
+    log(f"ingest {kind} delim={delim}: {len(data)} bytes")
+    text = _text_tensor(data)
+    off = gss.ingest(text, delim, text_bytes=len(data))
+    want_off, want_text = oracle_mod.ingest(np.frombuffer(data, np.uint8), delim)
+    assert off.numel() == want_off.shape[0]
+    assert np.array_equal(off.cpu().numpy().view(np.uint32), want_off)
+    assert np.array_equal(text.cpu().numpy()[: len(data)], want_text)
+    assert not text.cpu().numpy()[len(data):].any(), "bytes past the text were written"
+
+
+def test_ingest_without_delimiter_is_an_error():
+    text = _text_tensor(b"no newline here")
+    with pytest.raises(gss.GssError, match="unable to read any lines"):
+        gss.ingest(text, b"\n", text_bytes=15)
+
+
+@pytest.mark.parametrize("config", [0, 1, 2, 3])
+def test_write_matches_oracle(config, oracle_mod):
+    import torch
+
+    ss = gen.make(config, n=50_000, seed=21)
+    text = _dev(ss.text)
+    off = _dev(ss.offsets.view(np.int32))
+    out = torch.empty_like(off)
+    gss.sort_device(text, off, out, text_bytes=ss.text_bytes)
+    got = gss.write(text, out, text_bytes=ss.text_bytes).cpu().numpy().tobytes()
+    srt = oracle_mod.sort_offsets(ss.text, ss.offsets)
+    assert got == oracle_mod.write(ss.text, srt, length=ss.text_bytes)
+
+
+def test_write_edge_cases(oracle_mod):
+    # strings of every length 0..40 around the 16-byte output blocks, and a
+    # string cut by the end of the text
+    strs = [b"x" * k for k in range(41)] + [b"", b"", b"q"]
+    ss = gen.from_strings(strs)
+    text = _dev(ss.text)
+    rng = np.random.default_rng(1)
+    order = ss.offsets[rng.permutation(ss.n)]
+    got = gss.write(text, _dev(order.view(np.int32)), text_bytes=ss.text_bytes)
+    assert got.cpu().numpy().tobytes() == oracle_mod.write(ss.text, order, length=ss.text_bytes)
+    raw = np.frombuffer(b"abcdefghij", np.uint8)
+    got = gss.write(_text_tensor(raw.tobytes()), _dev(np.array([7, 2], np.int32)), text_bytes=10)
+    assert got.cpu().numpy().tobytes() == b"hij\ncdefghij\n"
+
+
+@pytest.mark.parametrize("kind", ["plain", "raw"])
+def test_file_pipeline_on_device(kind, oracle_mod):
+    """sortstring's path on the device: the file moves once, strings are cut
+    there, sorted and written; equals the oracle's ingest + msd_CE2 + write."""
+    import torch
+
+    delim = 0 if kind == "raw" else ord("\n")
+    ss = gen.make(1, n=300_000, seed=31)
+    t = ss.text[: ss.text_bytes].copy()
+    t[t == 0] = delim
+    data = t.tobytes() + b"trailing"
+    text = _text_tensor(data)
+    off = gss.ingest(text, delim, text_bytes=len(data))
+    out = torch.empty_like(off)
+    gss.sort_device(text, off, out, text_bytes=len(data))
+    got = gss.write(text, out, text_bytes=len(data)).cpu().numpy().tobytes()
+    want_off, want_text = oracle_mod.ingest(np.frombuffer(data, np.uint8), delim)
+    want_text = np.concatenate([want_text, np.zeros(16, np.uint8)])
+    want = oracle_mod.write(want_text, oracle_mod.sort_offsets(want_text, want_off),
+                            length=len(data))
+    assert hashlib.md5(got).hexdigest() == hashlib.md5(want).hexdigest()
+
+
+@pytest.mark.parametrize("text", [b"banana\n", b"mississippi" * 3, b"ab" * 700 + b"b",
+                                  b"a" * 2000], ids=["banana", "mississippi", "abab", "a2000"])
+def test_suffix_sort_equals_naive(text, oracle_mod):
+    """Suffix mode: every suffix of the text (each runs to the text's end),
+    sorted on the device, equals a naive suffix sort; LCPs equal lcp()."""
+    import torch
+
+    t = _text_tensor(text)
+    off = gss.suffixes(len(text), device())
+    out = torch.empty_like(off)
+    lc = torch.empty(max(1, len(text) - 1), dtype=torch.int64, device=device())
+    gss.sort_device(t, off, out, lc, text_bytes=len(text))
+    got = out.cpu().numpy().tolist()
+    assert got == sorted(range(len(text)), key=lambda i: text[i:])
+    suf = [text[i:] for i in got]
+    want_lcp = [len(os.path.commonprefix([a, b])) for a, b in zip(suf, suf[1:])]
+    assert lc.cpu().numpy()[: len(text) - 1].tolist() == want_lcp
+
+
+def test_suffix_sort_dna_text(oracle_mod):
+    """Suffixes of a 200 kB random ACGT text, against the oracle's msd_CE2."""
+    import torch
+
+    rng = np.random.default_rng(9)
+    data = bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), size=200_000).astype(np.uint8))
+    text = np.frombuffer(data + b"\0" * 16, np.uint8)
+    t = _dev(text)
+    off = gss.suffixes(len(data), device())
+    out = torch.empty_like(off)
+    lc = torch.empty(len(data) - 1, dtype=torch.int64, device=device())
+    gss.sort_device(t, off, out, lc, text_bytes=len(data))
+    want, want_lcp = oracle_mod.sort_offsets(text, np.arange(len(data), dtype=np.uint32),
+                                             lcp=True)
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), want)
+    assert np.array_equal(lc.cpu().numpy().view(np.uint64), want_lcp)
+
+
+def _sorted_run(text, off, text_bytes):
+    import torch
+
+    o = _dev(off.view(np.int32))
+    out = torch.empty_like(o)
+    lc = torch.empty(max(1, o.numel() - 1), dtype=torch.int64, device=device())
+    gss.sort_device(text, o, out, lc, text_bytes=text_bytes)
+    return out, lc
+
+
+@pytest.mark.parametrize("sizes", [(60_000, 45_000), (0, 5_000), (5_000, 0), (1, 1), (1, 9_000),
+                                   (100_003, 2_048)])
+def test_merge_lcp_matches_oracle(sizes, oracle_mod):
+    """Two sorted runs of one text (drawn from all four generators, with
+    duplicates across the runs) merge into the stable sort of A then B with
+    exact LCPs."""
+    na, nb = sizes
+    parts = [gen.make(c, n=30_000, seed=40 + c) for c in range(4)]
+    joined, _ = gen.concat(parts)
+    rng = np.random.default_rng(na * 7 + nb)
+    pick = rng.integers(0, joined.n, size=na + nb)
+    pick[na: na + min(na, nb) // 3] = pick[: min(na, nb) // 3]  # cross-run duplicates
+    a_in = joined.offsets[pick[:na]]
+    b_in = joined.offsets[pick[na:]]
+    text = _dev(joined.text)
+    a, al = _sorted_run(text, a_in, joined.text_bytes)
+    b, bl = _sorted_run(text, b_in, joined.text_bytes)
+    out, ol = gss.merge_lcp(text, a[:na], al[: max(0, na - 1)], b[:nb], bl[: max(0, nb - 1)],
+                            text_bytes=joined.text_bytes)
+    got = out.cpu().numpy().view(np.uint32)
+    want, want_lcp = oracle_mod.sort_offsets(joined.text, np.concatenate([a_in, b_in]), lcp=True)
+    assert np.array_equal(got, want)
+    assert np.array_equal(ol.cpu().numpy().view(np.uint64), want_lcp)
+    ao, alo = oracle_mod.sort_offsets(joined.text, a_in, lcp=True)
+    bo, blo = oracle_mod.sort_offsets(joined.text, b_in, lcp=True)
+    mo, mlo = oracle_mod.merge_lcp(joined.text, ao, alo, bo, blo)
+    assert np.array_equal(got, mo) and np.array_equal(ol.cpu().numpy().view(np.uint64), mlo)

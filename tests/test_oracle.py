@@ -168,3 +168,101 @@ def test_restatement_equals_reference_build(config, oracle_mod):
     for name in ("msd_CE6", "msd_ci_adaptive", "msd_A2_adaptive", "multikey_cache8"):
         other = oracle_mod.ref_sort_offsets(ss.text, ss.offsets, name)
         assert strings_at(ss, other) == strings_at(ss, want), name
+
+
+# ---------------------------------------------------------------------------
+# The steps either side of the sort: ingest, --write, LCP merge.
+
+def _delim_text(rng, n, delim, trailing=b"tail"):
+    lines = _random_strings(rng, n, np.frombuffer(b"abcxyz\x01\xff", np.uint8), 12)
+    lines = [l.replace(bytes([delim]), b"") for l in lines] + [b"", b""]
+    return np.frombuffer(bytes([delim]).join(lines) + bytes([delim]) + trailing, np.uint8)
+
+
+@pytest.mark.parametrize("delim", [ord("\n"), 0])
+def test_ingest_closed_form(delim, oracle_mod):
+    """Offsets are 0 and every delimiter position + 1 except the last; the
+    count is the number of delimiters; delimiters become NUL, other bytes
+    are untouched; bytes after the last delimiter form no string."""
+    rng = np.random.default_rng(5)
+    text = _delim_text(rng, 300, delim)
+    off, t2 = oracle_mod.ingest(text, delim)
+    pos = np.nonzero(text == delim)[0]
+    assert off.shape[0] == pos.shape[0]
+    assert np.array_equal(off, np.concatenate([[0], pos[:-1] + 1]).astype(np.uint32))
+    want = text.copy()
+    want[pos] = 0
+    assert np.array_equal(t2, want)
+    assert oracle_mod.ingest(np.frombuffer(b"no delimiter", np.uint8), delim)[0].shape[0] == 0
+
+
+def test_write_format(oracle_mod):
+    """Each string then '\\n', in array order; a string also ends at the end
+    of the text."""
+    ss = gen.from_strings([b"bb", b"", b"a\xffz", b"bb"])
+    order = ss.offsets[[3, 1, 2, 0]]
+    assert oracle_mod.write(ss.text, order) == b"bb\n\na\xffz\nbb\n"
+    t = np.frombuffer(b"abcdef", np.uint8)
+    assert oracle_mod.write(t, np.array([4, 0], np.uint32)) == b"ef\nabcdef\n"
+    assert oracle_mod.write(t, np.array([4, 0], np.uint32), length=5) == b"e\nabcde\n"
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_merge_lcp_equals_brute_force(seed, oracle_mod):
+    """Merging two sorted runs equals the stable sort of A then B, and the
+    LCPs equal lcp() of every adjacent output pair; the LCP shortcuts are
+    exercised by long shared prefixes and cross-run duplicates."""
+    rng = np.random.default_rng(seed)
+    base = _random_strings(rng, 200, np.frombuffer(b"ab", np.uint8), 10)
+    strs = [b"prefix/" * int(rng.integers(0, 3)) + s for s in base]
+    ss = gen.from_strings(strs + strs[:50])  # duplicates across the runs
+    cut = int(rng.integers(0, ss.n))
+    a_in, b_in = ss.offsets[:cut], ss.offsets[cut:]
+    a, al = oracle_mod.sort_offsets(ss.text, a_in, lcp=True)
+    b, bl = oracle_mod.sort_offsets(ss.text, b_in, lcp=True)
+    out, ol = oracle_mod.merge_lcp(ss.text, a, al, b, bl)
+    want = oracle_mod.sort_offsets(ss.text, np.concatenate([a, b]))
+    assert np.array_equal(out, want)
+    got = strings_at(ss, out)
+    assert got == sorted(got)
+    assert ol.tolist() == brute_lcp(got)
+    for x, y in ((a[:0], b), (a, b[:0]), (a[:1], b[:1])):
+        o, l = oracle_mod.merge_lcp(ss.text, x, al[: max(0, len(x) - 1)], y,
+                                    bl[: max(0, len(y) - 1)])
+        assert np.array_equal(o, oracle_mod.sort_offsets(ss.text, np.concatenate([x, y])))
+        assert l.tolist() == brute_lcp(strings_at(ss, o))
+
+
+SORTSTRING = Path(oracle.__file__).resolve().parent / "_ref" / "sortstring"
+
+
+@pytest.mark.skipif(not SORTSTRING.exists(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("mode", ["plain", "raw", "suffix"])
+def test_reference_sortstring_equals_oracle_pipeline(mode, tmp_path, oracle_mod):
+    """The reference's own sortstring (compiled from its sources) reads the
+    file, sorts with msd_CE2 and writes it (--write); the oracle's ingest,
+    msd_CE2 and write give the same bytes.  Pins ingest and write against
+    the reference, for newline input, --raw input and --suffix-sorting."""
+    import subprocess
+
+    rng = np.random.default_rng(11)
+    if mode == "suffix":
+        data = bytes(rng.choice(np.frombuffer(b"abn\n", np.uint8), size=300).astype(np.uint8))
+    else:
+        delim = 0 if mode == "raw" else ord("\n")
+        data = _delim_text(rng, 400, delim).tobytes()
+    f = tmp_path / "in.txt"
+    f.write_bytes(data)
+    out = tmp_path / "out.txt"
+    args = [str(SORTSTRING), f"--write={out}"]
+    args += {"plain": [], "raw": ["--raw"], "suffix": ["--suffix-sorting"]}[mode]
+    subprocess.run(args + ["msd_CE2", str(f)], check=True, capture_output=True, timeout=120)
+    text = np.frombuffer(data + b"\0" * 16, np.uint8).copy()
+    if mode == "suffix":
+        off = np.arange(len(data), dtype=np.uint32)
+        t2 = text
+    else:
+        off, t2 = oracle_mod.ingest(text[: len(data)], 0 if mode == "raw" else ord("\n"))
+        t2 = np.concatenate([t2, np.zeros(16, np.uint8)])
+    srt = oracle_mod.sort_offsets(t2, off)
+    assert oracle_mod.write(t2, srt, length=len(data)) == out.read_bytes()
