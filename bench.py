@@ -14,6 +14,10 @@ Besides the contract fields the JSON line carries:
                 oracle/_ref) timed on this host's cores on the same workload;
   hbm_gbps      algorithmic bytes of the whole sort / sort time;
   e2e           the routine.h pointer path (gss_msd), PCIe transfers included;
+  pipeline      the steps either side of the sort on the same set: device
+                ingest of the newline file, --write formatting, LCP-aware
+                merge of two sorted halves, suffix sorting of a 10 MB ACGT
+                text, and file-in / file-out with PCIe;
   batched       the request stream (20,000 sorts of 1e3..4e6 strings from the
                 four generators' seeded pools): the served loop (continuous
                 batching, gss_server_*) against one plain gss_sort_device32
@@ -87,6 +91,8 @@ def parse():
                          "reference dispatch that scales FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--stream-requests", type=int, default=20000,
                     help="requests of the served-stream comparison (0 = skip it)")
+    ap.add_argument("--no-pipeline", action="store_true",
+                    help="skip the ingest / write / merge / suffix rows")
     ap.add_argument("--stream-only", action="store_true",
                     help="run only the served-stream comparison")
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -100,6 +106,130 @@ def parse():
 def _pctl(x, q):
     """Nearest-rank percentile (the smallest sample with >= q% at or below)."""
     return float(np.percentile(np.asarray(x), q, method="inverted_cdf")) if len(x) else 0.0
+
+
+def _best_time(fn, runs, prepare=None):
+    """Best wall time of `fn` (which returns once its work is complete on the
+    device) over `runs` calls; `prepare` runs untimed before each call."""
+    import torch
+
+    best = float("inf")
+    for _ in range(runs):
+        if prepare is not None:
+            prepare()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def pipeline_rows(ss, dev, runs=5):
+    """The steps either side of the sort on the same 10M URL-like set:
+    device ingest of the newline file, --write formatting of the sorted
+    output, LCP-aware merge of two sorted halves, suffix sorting of a 10 MB
+    ACGT text, and the whole file-in / file-out pipeline with PCIe."""
+    import torch
+
+    import gss
+
+    res = {}
+    n, tb = ss.n, ss.text_bytes
+    nl = ss.text.copy()
+    nl[:tb][nl[:tb] == 0] = ord("\n")
+    pristine = torch.from_numpy(nl).to(dev)
+    work = torch.empty_like(pristine)
+    box = {}
+    off_buf = torch.empty(n, dtype=torch.int32, device=dev)
+
+    def ingest():
+        box["off"] = gss.ingest(work, b"\n", text_bytes=tb, out=off_buf)
+
+    t = _best_time(ingest, runs, prepare=lambda: work.copy_(pristine))
+    off = box["off"]
+    ok = bool(torch.equal(off.cpu(), torch.from_numpy(ss.offsets.view(np.int32))))
+    alg = tb + 4 * n
+    res["ingest"] = {"seconds": t, "strings": n, "gb_per_s": alg / t / 1e9,
+                     "frac_hbm": alg / t / 1e9 / HBM_PEAK_GBPS, "matches_generator": ok,
+                     "alg_bytes": alg, "note": "text read once + 32-bit offsets written"}
+
+    out = torch.empty_like(off)
+    lcp = torch.empty(n - 1, dtype=torch.int64, device=dev)
+    gss.sort_device(work, off, out, lcp, text_bytes=tb)
+    box.clear()
+
+    def write():
+        box["w"] = gss.write(work, out, text_bytes=tb)
+
+    t = _best_time(write, runs)
+    wbytes = int(box["w"].numel())
+    alg = 4 * n + 2 * wbytes - n
+    res["write"] = {"seconds": t, "out_bytes": wbytes, "gb_per_s": alg / t / 1e9,
+                    "frac_hbm": alg / t / 1e9 / HBM_PEAK_GBPS, "alg_bytes": alg,
+                    "note": "offsets + string bytes read, formatted output written"}
+    del box["w"]
+
+    # Merge: sort each half of the input order, then merge the two runs.
+    h = n // 2
+    a, b = torch.empty(h, dtype=torch.int32, device=dev), torch.empty(n - h, dtype=torch.int32,
+                                                                      device=dev)
+    al = torch.empty(h - 1, dtype=torch.int64, device=dev)
+    bl = torch.empty(n - h - 1, dtype=torch.int64, device=dev)
+    gss.sort_device(work, off[:h].contiguous(), a, al, text_bytes=tb)
+    gss.sort_device(work, off[h:].contiguous(), b, bl, text_bytes=tb)
+
+    def merge():
+        box["m"] = gss.merge_lcp(work, a, al, b, bl, text_bytes=tb)
+
+    t = _best_time(merge, runs)
+    m, ml = box.pop("m")
+    same = bool(torch.equal(m, out)) and bool(torch.equal(ml, lcp))
+    alg = 24 * n
+    res["merge_lcp"] = {"seconds": t, "strings": n, "mstrings_per_s": n / t / 1e6,
+                        "gb_per_s": alg / t / 1e9, "frac_hbm": alg / t / 1e9 / HBM_PEAK_GBPS,
+                        "equals_full_sort": same,
+                        "note": "two sorted 5M halves; 24 B/string = offset+LCP in, offset+LCP out"}
+    del m, ml, a, b, al, bl
+
+    # Suffix sorting of a 10 MB i.i.d. ACGT text.
+    rng = np.random.default_rng(20242)
+    dna = np.concatenate([np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, size=10_000_000)],
+                          np.zeros(16, np.uint8)])
+    dt_ = torch.from_numpy(dna).to(dev)
+    soff = gss.suffixes(10_000_000, dev)
+    sout = torch.empty_like(soff)
+
+    def suffix_sort():
+        gss.sort_device(dt_, soff, sout, text_bytes=10_000_000)
+
+    t = _best_time(suffix_sort, runs)
+    s_sorted = sout.cpu().numpy().view(np.uint32)
+    import oracle
+
+    inv = oracle.check_result(dna, s_sorted)[2]
+    res["suffix_sort"] = {"seconds": t, "suffixes": 10_000_000, "mstrings_per_s": 1e7 / t / 1e6,
+                          "inversions": int(inv),
+                          "permutation": bool(np.array_equal(np.sort(s_sorted),
+                                                             np.arange(10_000_000, dtype=np.uint32)))}
+    del dt_, soff, sout
+
+    # File in, file out: H2D of the newline file, ingest, sort, write, D2H.
+    host_out = torch.empty(wbytes, dtype=torch.uint8, pin_memory=True)
+    host_in = torch.from_numpy(nl).pin_memory()
+
+    def file_pipeline():
+        w = pristine.copy_(host_in, non_blocking=True)
+        o = gss.ingest(w, b"\n", text_bytes=tb, out=off_buf)
+        srt = torch.empty_like(o)
+        gss.sort_device(w, o, srt, text_bytes=tb)
+        host_out.copy_(gss.write(w, srt, text_bytes=tb))
+
+    t = _best_time(file_pipeline, 3)
+    res["file_pipeline"] = {"seconds": t, "mstrings_per_s": n / t / 1e6,
+                            "note": "pinned H2D of the 635 MB newline file, ingest, sort, write, "
+                                    "D2H of the sorted file"}
+    return res
 
 
 def served_stream(args, dev):
@@ -376,6 +506,11 @@ def main():
         e2e_ok = bool(np.array_equal((ptrs - np.uint64(base)).astype(np.uint32), got))
         res["e2e"] = {"entry": "gss_msd (host pointers, H2D text+pointers, D2H pointers)",
                       "seconds": te, "mstrings_per_s": ss.n / te / 1e6, "matches_device": e2e_ok}
+
+    if not args.no_pipeline:
+        log("ingest / write / merge / suffix rows ...")
+        res["pipeline"] = pipeline_rows(ss, dev)
+        torch.cuda.empty_cache()
 
     if args.stream_requests > 0:
         del text, off, out, lcp

@@ -8,6 +8,9 @@
 //            after the last delimiter belong to no string.  Two launches over
 //            the text: per-chunk delimiter counts, then (after a scan of the
 //            counts) every delimiter writes the offset of the string after it.
+//   merge    merge_lcp_2way (src/mergesort_lcp.cpp:133-211) as a parallel
+//            merge path: tile splits by binary search, then every thread
+//            merges its outputs with the LCP rule of the reference.
 //   suffixes create_suffixes (src/sortstring.c:349-358): offset i for every
 //            text position i.
 //   write    write_result (src/sortstring.c:360-391): the strings in array
@@ -72,45 +75,38 @@ __device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* scra
   return r;
 }
 
-// Text word w of a text of `len` bytes, bytes at or beyond len cleared.
-__device__ __forceinline__ uint64_t masked_word(const uint64_t* __restrict__ t64, uint64_t len,
-                                                uint64_t w) {
-  const uint64_t b = w * 8;
-  if (b >= len) return 0;
-  const uint64_t x = t64[w];
-  const uint64_t have = len - b;
-  return have >= 8 ? x : (x & ((1ull << (8 * have)) - 1ull));
-}
-
 // ---------------------------------------------------------------------------
 // Ingest
 
-// Delimiter flags of this thread's words; bytes beyond len never match.
+// Delimiter flags of the 8 words [w0, w0+8) (one thread's 64 contiguous
+// bytes, loaded 16 bytes at a time); bytes at or beyond len never match.
 __device__ __forceinline__ void ingest_flags(const uint64_t* __restrict__ t64, uint64_t len,
                                              uint64_t pat, uint64_t w0, uint64_t (&f)[kIngWords]) {
+  const uint64_t nwords = (len + 7) / 8;
+  uint64_t raw[kIngWords];
+  if (w0 * 8 >= len) {
+#pragma unroll
+    for (int q = 0; q < kIngWords; ++q) f[q] = 0;
+    return;
+  }
+  text_words<kIngWords>(t64, nwords, w0, raw);
 #pragma unroll
   for (int q = 0; q < kIngWords; ++q) {
-    const uint64_t w = w0 + (uint64_t)q * kIngThreads;  // coalesced: word-interleaved
-    const uint64_t b = w * 8;
-    uint64_t m = 0;
-    if (b < len) {
-      m = eq_bytes(t64[w], pat);
-      const uint64_t have = len - b;
-      if (have < 8) m &= (1ull << (8 * have)) - 1ull;
-    }
+    const uint64_t b = (w0 + q) * 8;
+    uint64_t m = b < len ? eq_bytes(raw[q], pat) : 0ull;
+    if (b < len && len - b < 8) m &= (1ull << (8 * (len - b))) - 1ull;
     f[q] = m;
   }
 }
 
-// The word index of slot q of thread t in block b: blocks cover 16 KiB
-// chunks; inside a chunk thread t owns words t, t+256, ... so that loads are
-// coalesced, and the order of delimiters is word order (q-major within a
-// thread is NOT text order, so the emit kernel ranks by word index).
+// Block b covers the 16 KiB chunk of words [b*2048, (b+1)*2048); thread t
+// owns its words [8t, 8t+8), so a thread's delimiters are in text order and
+// one block scan ranks them.
 __global__ __launch_bounds__(kIngThreads) void k_ingest_count(const uint64_t* __restrict__ t64,
                                                              uint64_t len, uint64_t pat,
                                                              uint64_t* __restrict__ counts) {
   __shared__ uint64_t scratch[kIngThreads / kWave + 1];
-  const uint64_t w0 = (uint64_t)blockIdx.x * (kIngChunk / 8) + threadIdx.x;
+  const uint64_t w0 = (uint64_t)blockIdx.x * (kIngChunk / 8) + (uint64_t)threadIdx.x * kIngWords;
   uint64_t f[kIngWords];
   ingest_flags(t64, len, pat, w0, f);
   uint64_t c = 0;
@@ -123,29 +119,26 @@ __global__ __launch_bounds__(kIngThreads) void k_ingest_count(const uint64_t* __
 
 // Every delimiter j at byte i writes off[j+1] = i+1 (j+1 < n) and, unless
 // the delimiter is NUL, becomes a NUL.  Block bases come from the scanned
-// counts; inside a block the delimiters are ranked in text order: pass q
-// covers words [q*256, (q+1)*256) of the chunk, one per thread.
+// counts, ranks inside the block from one scan of per-thread counts.
 __global__ __launch_bounds__(kIngThreads) void k_ingest_emit(uint64_t* __restrict__ t64,
                                                             uint64_t len, uint64_t pat,
                                                             bool clear, const uint64_t* __restrict__ base,
                                                             uint64_t n, uint32_t* __restrict__ off) {
   __shared__ uint64_t scratch[kIngThreads / kWave + 1];
-  const uint64_t w0 = (uint64_t)blockIdx.x * (kIngChunk / 8) + threadIdx.x;
+  const uint64_t w0 = (uint64_t)blockIdx.x * (kIngChunk / 8) + (uint64_t)threadIdx.x * kIngWords;
   uint64_t f[kIngWords];
   ingest_flags(t64, len, pat, w0, f);
-  uint64_t run = base[blockIdx.x];
+  uint64_t c = 0;
+#pragma unroll
+  for (int q = 0; q < kIngWords; ++q) c += __popcll(f[q]);
+  uint64_t tot;
+  uint64_t j = base[blockIdx.x] + block_excl_scan64<kIngThreads>(c, scratch, &tot);
   if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) off[0] = 0;
 #pragma unroll
   for (int q = 0; q < kIngWords; ++q) {
-    uint64_t tot;
-    const uint64_t c = __popcll(f[q]);
-    uint64_t j = run + block_excl_scan64<kIngThreads>(c, scratch, &tot);
-    const uint64_t w = w0 + (uint64_t)q * kIngThreads;
-    uint64_t m = f[q];
-    while (m) {
-      const uint32_t bit = __builtin_ctzll(m);
-      m &= m - 1;
-      const uint64_t i = w * 8 + (bit >> 3);
+    const uint64_t w = w0 + q;
+    for (uint64_t m = f[q]; m; m &= m - 1) {
+      const uint64_t i = w * 8 + (__builtin_ctzll(m) >> 3);
       if (j + 1 < n) off[j + 1] = (uint32_t)(i + 1);
       ++j;
     }
@@ -157,7 +150,6 @@ __global__ __launch_bounds__(kIngThreads) void k_ingest_emit(uint64_t* __restric
         for (uint64_t mm = f[q]; mm; mm &= mm - 1) t8[w * 8 + (__builtin_ctzll(mm) >> 3)] = 0;
       }
     }
-    run += tot;
   }
 }
 
@@ -236,6 +228,8 @@ __global__ void k_iota32(uint32_t* __restrict__ off, uint64_t n) {
 }
 
 // Length of every string: bytes before its NUL or the end of the text.
+// Reads 64 bytes at a time (four 16-byte loads in flight), which covers most
+// strings in one round trip.
 __global__ __launch_bounds__(256) void k_strlen(const uint64_t* __restrict__ t64, uint64_t len,
                                                const uint32_t* __restrict__ off, uint64_t n,
                                                uint32_t* __restrict__ lens) {
@@ -244,15 +238,28 @@ __global__ __launch_bounds__(256) void k_strlen(const uint64_t* __restrict__ t64
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const uint64_t o = off[i];
     uint64_t w = o >> 3;
-    // First word: bytes before the string count as non-zero.
-    uint64_t x = masked_word(t64, len, w) | ((1ull << (8 * (o & 7))) - 1ull);
-    uint64_t z = eq_bytes(x, 0);
-    while (!z && w + 1 < nwords) {
-      ++w;
-      x = masked_word(t64, len, w);
-      z = eq_bytes(x, 0);
+    uint64_t before = (1ull << (8 * (o & 7))) - 1ull;  // bytes before the string: not NUL
+    uint64_t end = len;
+    while (w < nwords) {
+      uint64_t raw[8];
+      text_words<8>(t64, nwords, w, raw);
+      bool found = false;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const uint64_t b = (w + q) * 8;
+        uint64_t x = raw[q];
+        if (b + 8 > len) x = b >= len ? 0ull : (x & ((1ull << (8 * (len - b))) - 1ull));
+        if (q == 0) x |= before;
+        const uint64_t z = eq_bytes(x, 0);
+        if (z && !found) {
+          end = b + (__builtin_ctzll(z) >> 3);
+          found = true;
+        }
+      }
+      if (found) break;
+      before = 0;
+      w += 8;
     }
-    const uint64_t end = z ? w * 8 + (__builtin_ctzll(z) >> 3) : len;  // NUL or text end
     lens[i] = (uint32_t)(end - o);
   }
 }
@@ -317,7 +324,7 @@ __global__ __launch_bounds__(256) void k_write(const uint64_t* __restrict__ t64,
 // merge of two stable sorts is the stable sort of A followed by B.
 
 constexpr int kMergeThreads = 256;
-constexpr int kMergeItems = 8;
+constexpr int kMergeItems = 16;
 constexpr uint64_t kMergeTile = (uint64_t)kMergeThreads * kMergeItems;
 
 struct StrCmp {
@@ -337,14 +344,15 @@ __device__ __forceinline__ StrCmp str_compare(const uint64_t* __restrict__ t64, 
 }
 
 // Smallest i in [lo, hi] with A[i] > B[d-1-i] (ties A first): the number of
-// A strings among the first d outputs.
+// A strings among the first d outputs.  Every string compared is known to
+// share its first `depth` bytes with the others.
 __device__ __forceinline__ uint64_t merge_path(const uint64_t* __restrict__ t64, uint64_t nwords,
                                                const uint32_t* __restrict__ a,
                                                const uint32_t* __restrict__ b, uint64_t d,
-                                               uint64_t lo, uint64_t hi) {
+                                               uint64_t lo, uint64_t hi, uint64_t depth = 0) {
   while (lo < hi) {
     const uint64_t mid = (lo + hi) >> 1;
-    if (str_compare(t64, nwords, a[mid], b[d - 1 - mid], 0).sign <= 0)
+    if (str_compare(t64, nwords, a[mid], b[d - 1 - mid], depth).sign <= 0)
       lo = mid + 1;
     else
       hi = mid;
@@ -376,17 +384,33 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
     const uint64_t* __restrict__ alcp, uint64_t na, const uint32_t* __restrict__ b,
     const uint64_t* __restrict__ blcp, uint64_t nb, const uint64_t* __restrict__ split,
     uint32_t* __restrict__ out, uint64_t* __restrict__ olcp) {
+  __shared__ uint64_t block_depth;
   const uint64_t total = na + nb;
   const uint64_t k0 = (uint64_t)blockIdx.x * kMergeTile;
   const uint64_t k1 = k0 + kMergeTile < total ? k0 + kMergeTile : total;
-  const uint64_t d = k0 + (uint64_t)threadIdx.x * kMergeItems;
-  if (d >= k1) return;
   const uint64_t i0 = split[blockIdx.x], i1 = split[blockIdx.x + 1];
   const uint64_t j0 = k0 - i0, j1 = k1 - i1;
+  // The block's outputs are A[i0, i1) and B[j0, j1); in sorted order any two
+  // of them share at least lcp(first, last) bytes, so the comparisons below
+  // start there.
+  if (threadIdx.x == 0) {
+    uint64_t first, last;
+    if (i0 == i1) first = b[j0];
+    else if (j0 == j1) first = a[i0];
+    else first = str_compare(t64, nwords, a[i0], b[j0], 0).sign <= 0 ? a[i0] : b[j0];
+    if (i0 == i1) last = b[j1 - 1];
+    else if (j0 == j1) last = a[i1 - 1];
+    else last = str_compare(t64, nwords, a[i1 - 1], b[j1 - 1], 0).sign <= 0 ? b[j1 - 1] : a[i1 - 1];
+    block_depth = str_compare(t64, nwords, first, last, 0).lcp;
+  }
+  __syncthreads();
+  const uint64_t depth = block_depth;
+  const uint64_t d = k0 + (uint64_t)threadIdx.x * kMergeItems;
+  if (d >= k1) return;
   // Narrow to this block's ranges: i in [i0, i1], d - i in [j0, j1].
   const uint64_t lo = (d > j1 && d - j1 > i0) ? d - j1 : i0;
   const uint64_t hi = d - j0 < i1 ? d - j0 : i1;
-  uint64_t i = merge_path(t64, nwords, a, b, d, lo, hi);
+  uint64_t i = merge_path(t64, nwords, a, b, d, lo, hi, depth);
   uint64_t j = d - i;
   const uint64_t end = d + kMergeItems < k1 ? d + kMergeItems : k1;
   bool have_s = false;
@@ -401,7 +425,8 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
       take_a = true;
       lprev = ha;
     } else if (!have_s) {
-      const StrCmp c = str_compare(t64, nwords, a[i], b[j], 0);
+      // Both heads lie in the block's outputs unless one is past its range.
+      const StrCmp c = str_compare(t64, nwords, a[i], b[j], (i < i1 && j < j1) ? depth : 0);
       take_a = c.sign <= 0;
       if (take_a) hb = c.lcp; else ha = c.lcp;
     } else if (ha != hb) {

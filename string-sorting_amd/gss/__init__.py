@@ -286,22 +286,25 @@ def _ptr(t):
     return t.data_ptr() if t is not None else None
 
 
-def ingest(text, delim=b"\n", text_bytes=None, stream=None):
+def ingest(text, delim=b"\n", text_bytes=None, out=None, stream=None):
     """Device-side create_strings_delim: splits the uint8 device tensor
     `text` in place at every `delim` byte (each becomes a NUL unless it is
-    NUL) and returns the int32 offsets of the strings (one per delimiter)."""
+    NUL) and returns the int32 offsets of the strings (one per delimiter).
+    With `out` (an int32 tensor as large as the string count) the split is
+    one call; without it a counting call sizes the result first."""
     import torch
 
     d = delim[0] if isinstance(delim, (bytes, bytearray)) else int(delim)
     tb = text.numel() if text_bytes is None else int(text_bytes)
     n = C.c_uint64()
     st = _stream_ptr(text, stream)
-    _check(lib().gss_ingest_device32(text.data_ptr(), tb, d, None, 0, C.byref(n), st),
-           "gss_ingest_device32")
-    off = torch.empty(max(1, n.value), dtype=torch.int32, device=text.device)
-    _check(lib().gss_ingest_device32(text.data_ptr(), tb, d, off.data_ptr(), n.value, C.byref(n),
-                                     st), "gss_ingest_device32")
-    return off[: n.value]
+    if out is None:
+        _check(lib().gss_ingest_device32(text.data_ptr(), tb, d, None, 0, C.byref(n), st),
+               "gss_ingest_device32")
+        out = torch.empty(max(1, n.value), dtype=torch.int32, device=text.device)
+    _check(lib().gss_ingest_device32(text.data_ptr(), tb, d, out.data_ptr(), out.numel(),
+                                     C.byref(n), st), "gss_ingest_device32")
+    return out[: n.value]
 
 
 def suffixes(text_bytes, device, stream=None):
@@ -314,20 +317,26 @@ def suffixes(text_bytes, device, stream=None):
     return off[: int(text_bytes)]
 
 
-def write(text, offsets, text_bytes=None, stream=None):
+def write(text, offsets, text_bytes=None, out=None, stream=None):
     """Device-side write_result: the strings at `offsets`, in order, each
-    followed by '\\n', as a uint8 device tensor."""
+    followed by '\\n', as a uint8 device tensor.  Without `out` the first
+    attempt assumes the strings do not overlap (output <= text size) and
+    retries with the exact size when they do."""
     import torch
 
     tb = text.numel() if text_bytes is None else int(text_bytes)
     n = offsets.numel()
     size = C.c_uint64()
     st = _stream_ptr(text, stream)
-    _check(lib().gss_write_device32(text.data_ptr(), tb, offsets.data_ptr(), n, None, 0,
-                                    C.byref(size), st), "gss_write_device32")
-    out = torch.empty(max(16, size.value), dtype=torch.uint8, device=text.device)
-    _check(lib().gss_write_device32(text.data_ptr(), tb, offsets.data_ptr(), n, out.data_ptr(),
-                                    out.numel(), C.byref(size), st), "gss_write_device32")
+    if out is None:
+        out = torch.empty(max(16, tb + 16), dtype=torch.uint8, device=text.device)
+    rc = lib().gss_write_device32(text.data_ptr(), tb, offsets.data_ptr(), n, out.data_ptr(),
+                                  out.numel(), C.byref(size), st)
+    if rc == -4 and size.value > out.numel():  # GSS_E_TOO_LARGE: grow once
+        out = torch.empty(size.value, dtype=torch.uint8, device=text.device)
+        rc = lib().gss_write_device32(text.data_ptr(), tb, offsets.data_ptr(), n, out.data_ptr(),
+                                      out.numel(), C.byref(size), st)
+    _check(rc, "gss_write_device32")
     return out[: size.value]
 
 
