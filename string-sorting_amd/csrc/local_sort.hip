@@ -45,7 +45,7 @@ struct LocalSmem {
   uint16_t ss_start[SS];  // start in the active list (entry nss = nact)
   uint16_t ss_base[SS];   // start in the segment's final order
   uint8_t ss_stop[(SS + 3) & ~3];  // bit q: key q varies or holds a NUL in the group
-  uint16_t whist[W][256];
+  alignas(8) uint16_t whist[256][W];  // per-wave digit counters, digit-major
   uint64_t edge[W][kLocalWords];  // keys of each wave's last item
   uint32_t wvm[W];   // per wave: varying key bytes | 0x100 if a group is > 16
   uint32_t wtot[W];  // per wave: survivors | their group heads << 16
@@ -85,7 +85,10 @@ __device__ __forceinline__ void local_radix_pass(LocalSmem<BLOCK, CAP>& s, uint3
   const int lane = lane_id();
   const uint64_t lt = lanemask_lt();
 
-  for (int i = threadIdx.x; i < W * 256; i += BLOCK) (&s.whist[0][0])[i] = 0;
+  // 256 * W counters, 4 per thread (W = BLOCK / 64).
+  static_assert(256 * W == 4 * BLOCK, "four digit counters per thread");
+  uint64_t* const hist4 = reinterpret_cast<uint64_t*>(&s.whist[0][0]);
+  hist4[threadIdx.x] = 0;
   __syncthreads();
 
   uint64_t k[IPT];
@@ -109,47 +112,31 @@ __device__ __forceinline__ void local_radix_pass(LocalSmem<BLOCK, CAP>& s, uint3
       }
       const uint64_t m = match_digit<8>(d, __ballot(v));
       if (v) {
-        const uint32_t b = s.whist[wid][d];
+        const uint32_t b = s.whist[d][wid];
         rk[j] = (d << 16) | (b + (uint32_t)__popcll(m & lt));
-        if (lane == __builtin_ctzll(m)) s.whist[wid][d] = (uint16_t)(b + __popcll(m));
+        if (lane == __builtin_ctzll(m)) s.whist[d][wid] = (uint16_t)(b + __popcll(m));
       }
     }
   }
   __syncthreads();
 
-  // Digit-major, then wave order: exclusive starts of every (digit, wave).
-  constexpr int DPT = 256 >= BLOCK ? 256 / BLOCK : 1;
-  uint32_t tot[DPT];
-  uint32_t sum = 0;
-#pragma unroll
-  for (int q = 0; q < DPT; ++q) {
-    const int d = threadIdx.x * DPT + q;
-    uint32_t r = 0;
-    if (d < 256) {
-      for (int w = 0; w < W; ++w) {
-        const uint32_t c = s.whist[w][d];
-        s.whist[w][d] = (uint16_t)r;
-        r += c;
-      }
-    }
-    tot[q] = r;
-    sum += r;
-  }
-  uint32_t total;
-  uint32_t run = block_excl_scan<BLOCK>(sum, s.scan, &total);
-#pragma unroll
-  for (int q = 0; q < DPT; ++q) {
-    const int d = threadIdx.x * DPT + q;
-    if (d < 256) {
-      for (int w = 0; w < W; ++w) s.whist[w][d] = (uint16_t)(s.whist[w][d] + run);
-    }
-    run += tot[q];
+  // Exclusive starts of every (digit, wave) in digit-major order: each
+  // thread scans its four contiguous counters, one block scan joins them.
+  {
+    const uint64_t c4 = hist4[threadIdx.x];
+    const uint32_t c0 = (uint32_t)(c4 & 0xFFFFu), c1 = (uint32_t)((c4 >> 16) & 0xFFFFu);
+    const uint32_t c2 = (uint32_t)((c4 >> 32) & 0xFFFFu), c3 = (uint32_t)(c4 >> 48);
+    uint32_t total;
+    const uint32_t r0 = block_excl_scan<BLOCK>(c0 + c1 + c2 + c3, s.scan, &total);
+    const uint32_t r1 = r0 + c0, r2 = r1 + c1, r3 = r2 + c2;
+    hist4[threadIdx.x] = (uint64_t)r0 | ((uint64_t)r1 << 16) | ((uint64_t)r2 << 32) |
+                         ((uint64_t)r3 << 48);
   }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     if ((uint32_t)j < jn && base + j * kWave < nact) {
-      const uint32_t dst = s.whist[wid][rk[j] >> 16] + (rk[j] & 0xFFFFu);
+      const uint32_t dst = s.whist[rk[j] >> 16][wid] + (rk[j] & 0xFFFFu);
       s.key[dst] = k[j];
       s.off[dst] = o[j];
       s.sid[dst] = (uint16_t)sd[j];
