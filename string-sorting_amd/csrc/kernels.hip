@@ -613,7 +613,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
   constexpr int W = kTileThreads / kWave;
   constexpr int IPW = kTile / W;  // items per wave
   constexpr uint32_t DPT = NB >= kTileThreads ? NB / kTileThreads : 1;
-  __shared__ uint16_t s_whist[W][NB];
+  __shared__ __attribute__((aligned(16))) uint16_t s_whist[NB][W];  // digit-major
   __shared__ uint32_t s_tstart[NB];
   __shared__ uint32_t s_gbase[NB];
   __shared__ uint32_t s_off[kTile];
@@ -630,7 +630,9 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
   const uint32_t* __restrict__ src = p.idx[sg.buf];
   uint32_t* __restrict__ dst = p.idx[sg.buf == 1 ? 2 : 1];
 
-  for (uint32_t i = threadIdx.x; i < W * NB; i += kTileThreads) (&s_whist[0][0])[i] = 0;
+  static_assert(W == 8, "one 16-byte row of wave counters per digit");
+  uint4* const rows = reinterpret_cast<uint4*>(&s_whist[0][0]);
+  for (uint32_t d = threadIdx.x; d < NB; d += kTileThreads) rows[d] = make_uint4(0, 0, 0, 0);
   __syncthreads();
 
   const int wid = threadIdx.x / kWave;
@@ -651,26 +653,32 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
     const uint64_t m = match_digit_n(d, __ballot(v), nbits);
     rnk[j] = 0;
     if (v) {
-      const uint32_t base = s_whist[wid][d];
+      const uint32_t base = s_whist[d][wid];
       rnk[j] = base + (uint32_t)__popcll(m & lt);
-      if (lane == __builtin_ctzll(m)) s_whist[wid][d] = (uint16_t)(base + __popcll(m));
+      if (lane == __builtin_ctzll(m)) s_whist[d][wid] = (uint16_t)(base + __popcll(m));
     }
   }
   __syncthreads();
 
-  // Per digit: exclusive prefix over waves (in place) and tile total.
+  // Per digit: exclusive prefix over waves (in place) and tile total; a
+  // digit's eight wave counters are one 16-byte row of LDS.
   uint32_t tot[DPT];
   uint32_t sum = 0;
 #pragma unroll
   for (uint32_t q = 0; q < DPT; ++q) {
     const uint32_t d = threadIdx.x * DPT + q;
     uint32_t r = 0;
-    if (d < ds.nbu) {
-      for (int w = 0; w < W; ++w) {
-        const uint32_t c = s_whist[w][d];
-        s_whist[w][d] = (uint16_t)r;
-        r += c;
+    if (d < NB) {
+      const uint4 c = rows[d];
+      const uint32_t w[4] = {c.x, c.y, c.z, c.w};  // two 16-bit counters each
+      uint32_t out[4];
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const uint32_t lo = w[h] & 0xFFFFu, hi = w[h] >> 16;
+        out[h] = r | ((r + lo) << 16);
+        r += lo + hi;
       }
+      rows[d] = make_uint4(out[0], out[1], out[2], out[3]);
     }
     tot[q] = r;
     sum += r;
@@ -695,7 +703,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
     const uint32_t i = wid * IPW + j * kWave + lane;
     if (i < t.len) {
       const uint32_t d = dig[j];
-      const uint32_t lp = s_tstart[d] + s_whist[wid][d] + rnk[j];
+      const uint32_t lp = s_tstart[d] + s_whist[d][wid] + rnk[j];
       s_off[lp] = off[j];
       s_dig[lp] = (uint16_t)d;
     }
