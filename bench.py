@@ -93,6 +93,8 @@ def parse():
                     help="requests of the served-stream comparison (0 = skip it)")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="skip the ingest / write / merge / suffix rows")
+    ap.add_argument("--all-configs", action="store_true",
+                    help="also time every workload at its full size (adds about a minute)")
     ap.add_argument("--stream-only", action="store_true",
                     help="run only the served-stream comparison")
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -230,6 +232,46 @@ def pipeline_rows(ss, dev, runs=5):
                             "note": "pinned H2D of the 635 MB newline file, ingest, sort, write, "
                                     "D2H of the sorted file"}
     return res
+
+
+def all_configs(dev, runs=5):
+    """Each workload at its full size (1M uniform a-z, 10M URL-like, 32M DNA,
+    100M Zipf words): median device-resident sort time over `runs` sorts,
+    without and with the LCP array, and the pass statistics."""
+    import torch
+
+    import gss
+    import gss.gen as gen
+
+    out = {}
+    for cfg in range(4):
+        t0 = time.time()
+        ss = gen.make(cfg)
+        log(f"config {cfg}: {ss.n} strings generated in {time.time() - t0:.1f}s")
+        text = torch.from_numpy(ss.text).to(dev)
+        off = torch.from_numpy(ss.offsets.view(np.int32)).to(dev)
+        o = torch.empty_like(off)
+        lcp = torch.empty(ss.n - 1, dtype=torch.int64, device=dev)
+        row = {"workload": gen.NAMES[cfg], "strings": ss.n, "text_bytes": ss.text_bytes}
+        for name, lc in (("sort", None), ("sort_lcp", lcp)):
+            for _ in range(2):
+                gss.sort_device(text, off, o, lc, text_bytes=ss.text_bytes)
+            ts = []
+            for _ in range(runs):
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                gss.sort_device(text, off, o, lc, text_bytes=ss.text_bytes)
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t1)
+            t = sorted(ts)[len(ts) // 2]
+            row[name] = {"ms": t * 1e3, "mstrings_per_s": ss.n / t / 1e6}
+        st = gss.stats()
+        row["global_passes"] = int(st.global_passes)
+        row["local_round_strings"] = int(st.local_round_strings)
+        out[gen.NAMES[cfg]] = row
+        del text, off, o, lcp, ss
+        torch.cuda.empty_cache()
+    return out
 
 
 def served_stream(args, dev):
@@ -516,6 +558,10 @@ def main():
         del text, off, out, lcp
         torch.cuda.empty_cache()
         res["batched"] = served_stream(args, dev)
+
+    if args.all_configs:
+        torch.cuda.empty_cache()
+        res["configs"] = all_configs(dev)
 
     if not args.no_cpu:
         log("timing the CPU baseline ...")
