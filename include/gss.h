@@ -212,6 +212,32 @@ typedef struct gss_tuning {
 int gss_get_tuning(gss_tuning *out);
 int gss_set_tuning(const gss_tuning *in);
 
+/* Autotuner (no reference counterpart).  An input's signature is its mean
+ * string length, alphabet size and string count, each as a power-of-two
+ * bucket, read from 512 evenly spaced strings.  gss_autotune_device32 sorts
+ * a strided sample of the input (at most 2M strings) with every candidate
+ * tuning -- local_threshold 1024/2048/4096/8192 x radix_bits 11/8 -- and
+ * returns the fastest (best_ms; default_ms is the current tuning's time, 0
+ * when it is not a candidate), caching it for the signature.  It does not
+ * change the current tuning.  With gss_autotune_enable(1), gss_sort_device32
+ * (and the pointer entry points) sort inputs of 65536 strings or more with
+ * the cached tuning of their signature, tuning a new signature first. */
+typedef struct gss_autotune_result {
+	gss_tuning best;
+	double best_ms;
+	double default_ms;
+	uint64_t sample;       /* strings sorted per trial */
+	uint32_t signature;
+	uint32_t candidates;
+} gss_autotune_result;
+
+int gss_autotune_device32(const unsigned char *d_text, uint64_t text_bytes,
+			  const uint32_t *d_off, uint64_t n,
+			  gss_autotune_result *out, void *stream);
+int gss_autotune_enable(int on);
+int gss_autotune_reset(void);   /* forgets every cached signature */
+int gss_autotune_cached(void);  /* number of cached signatures */
+
 /* ------------------------------------------------------------------------
  * Per-call statistics of the last sort on this thread's library instance
  * (no reference counterpart).  active[r] is the number of strings still

@@ -21,6 +21,20 @@ int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t
 // The default context's stream on the current device (created on first use).
 int library_stream(hipStream_t* out);
 
+// As sort_device32 on the default context, with an explicit tuning (the
+// autotuner's trial sorts).
+int sort_device32_tuned(const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
+                        uint32_t* out, uint64_t n, hipStream_t stream, const gss_tuning& tuning);
+
+// GSS_OK when the tuning is supported (else the GSS_E_INVALID of gss_set_tuning).
+int check_tuning(const gss_tuning& t);
+gss_tuning current_tuning();
+
+// When the autotuner is on and n is large enough: *use = true and *out = the
+// tuning for the input's signature (tuned on first sight).  Else *use = false.
+int autotune_choose(const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
+                    uint64_t n, hipStream_t stream, gss_tuning* out, bool* use);
+
 // Device bytes of a context's workspace sized for n strings under the
 // current tuning.
 size_t workspace_bytes(uint64_t n);

@@ -317,7 +317,7 @@ int ensure_init_lists(Workspace& w, uint32_t nreq) {
 // (req == nullptr: one request of n strings).  Called with c.mu held.
 int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
              uint32_t* out, uint64_t* lcp, uint64_t n, const uint64_t* req, uint32_t nreq,
-             hipStream_t stream) {
+             hipStream_t stream, const gss_tuning* tuning) {
   gss_stats& stats = c.stats;
   memset(&stats, 0, sizeof(stats));
   stats.n = n;
@@ -342,7 +342,9 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   hipStream_t st = stream ? stream : c.stream;
 
   gss_tuning tun;
-  {
+  if (tuning) {
+    tun = *tuning;
+  } else {
     std::lock_guard<std::mutex> lk(g_mu);
     tun = g_tuning;
   }
@@ -494,9 +496,9 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
 
 int run_locked(Context& c, const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
                uint32_t* out, uint64_t* lcp, uint64_t n, const uint64_t* req, uint32_t nreq,
-               hipStream_t stream) {
+               hipStream_t stream, const gss_tuning* tuning = nullptr) {
   std::lock_guard<std::mutex> lk(c.mu);
-  const int rc = run_sort(c, text, text_bytes, in, out, lcp, n, req, nreq, stream);
+  const int rc = run_sort(c, text, text_bytes, in, out, lcp, n, req, nreq, stream, tuning);
   g_last_stats = c.stats;
   return rc;
 }
@@ -519,9 +521,36 @@ int library_stream(hipStream_t* out) {
 int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
                   uint32_t* out, uint64_t* lcp, uint64_t n, hipStream_t stream) {
   Context* c = nullptr;
+  int rc = default_context(&c);
+  if (rc) return rc;
+  gss_tuning tuned;
+  bool use = false;
+  rc = autotune_choose(text, text_bytes, in, n, stream, &tuned, &use);
+  if (rc) return rc;
+  return run_locked(*c, text, text_bytes, in, out, lcp, n, nullptr, 0, stream,
+                    use ? &tuned : nullptr);
+}
+
+int sort_device32_tuned(const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
+                        uint32_t* out, uint64_t n, hipStream_t stream, const gss_tuning& tuning) {
+  Context* c = nullptr;
   const int rc = default_context(&c);
   if (rc) return rc;
-  return run_locked(*c, text, text_bytes, in, out, lcp, n, nullptr, 0, stream);
+  return run_locked(*c, text, text_bytes, in, out, nullptr, n, nullptr, 0, stream, &tuning);
+}
+
+int check_tuning(const gss_tuning& t) {
+  if (t.local_threshold < 64 || t.local_threshold > kLocalCapMax)
+    return fail(GSS_E_INVALID, "local_threshold must lie in [64, 8192]");
+  if (t.radix_bits != 8 && t.radix_bits != 11)
+    return fail(GSS_E_INVALID, "radix_bits must be 8 or 11");
+  if (t.tile != kTile) return fail(GSS_E_INVALID, "tile must be 4096");
+  return GSS_OK;
+}
+
+gss_tuning current_tuning() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return g_tuning;
 }
 
 size_t workspace_bytes(uint64_t n) {
@@ -628,11 +657,8 @@ int gss_get_tuning(gss_tuning* out) {
 
 int gss_set_tuning(const gss_tuning* in) {
   if (!in) return fail(GSS_E_INVALID, "NULL tuning pointer");
-  if (in->local_threshold < 64 || in->local_threshold > kLocalCapMax)
-    return fail(GSS_E_INVALID, "local_threshold must lie in [64, 8192]");
-  if (in->radix_bits != 8 && in->radix_bits != 11)
-    return fail(GSS_E_INVALID, "radix_bits must be 8 or 11");
-  if (in->tile != kTile) return fail(GSS_E_INVALID, "tile must be 4096");
+  const int rc = check_tuning(*in);
+  if (rc) return rc;
   std::lock_guard<std::mutex> lk(g_mu);
   g_tuning = *in;
   return GSS_OK;

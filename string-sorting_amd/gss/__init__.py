@@ -46,6 +46,11 @@ class Stats(C.Structure):
     ]
 
 
+class AutotuneResult(C.Structure):
+    _fields_ = [("best", Tuning), ("best_ms", C.c_double), ("default_ms", C.c_double),
+                ("sample", C.c_uint64), ("signature", C.c_uint32), ("candidates", C.c_uint32)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("slots", C.c_uint32),
                 ("max_batch_requests", C.c_uint32)]
@@ -98,6 +103,11 @@ _SIGNATURES = {
     "gss_merge_lcp_device32": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
                                          C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint64,
                                          C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gss_autotune_device32": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+                                        C.POINTER(AutotuneResult), C.c_void_p]),
+    "gss_autotune_enable": (C.c_int, [C.c_int]),
+    "gss_autotune_reset": (C.c_int, []),
+    "gss_autotune_cached": (C.c_int, []),
     "gss_get_tuning": (C.c_int, [C.POINTER(Tuning)]),
     "gss_set_tuning": (C.c_int, [C.POINTER(Tuning)]),
     "gss_get_stats": (C.c_int, [C.POINTER(Stats)]),
@@ -428,6 +438,29 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def autotune(text, offsets, text_bytes=None, stream=None) -> AutotuneResult:
+    """Times every candidate tuning on a sample of this input and returns
+    the fastest (gss_autotune_device32); the current tuning is unchanged."""
+    r = AutotuneResult()
+    tb = text.numel() if text_bytes is None else int(text_bytes)
+    _check(lib().gss_autotune_device32(text.data_ptr(), tb, offsets.data_ptr(), offsets.numel(),
+                                       C.byref(r), _stream_ptr(text, stream)),
+           "gss_autotune_device32")
+    return r
+
+
+def autotune_enable(on: bool = True) -> None:
+    _check(lib().gss_autotune_enable(1 if on else 0), "gss_autotune_enable")
+
+
+def autotune_reset() -> None:
+    _check(lib().gss_autotune_reset(), "gss_autotune_reset")
+
+
+def autotune_cached() -> int:
+    return int(lib().gss_autotune_cached())
 
 
 def get_tuning() -> Tuning:

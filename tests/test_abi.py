@@ -116,3 +116,20 @@ def test_server_create_validates_before_touching_a_device():
     assert lib.gss_server_wait(None, 0, None) < 0
     assert lib.gss_server_done(None, 0) < 0
     lib.gss_server_destroy(None)
+
+
+def test_autotune_switches_without_device():
+    """Enabling, resetting and counting the autotuner's cache touch no
+    device; tuning rejects missing inputs before any device call."""
+    import ctypes as C
+
+    gss.autotune_reset()
+    assert gss.autotune_cached() == 0
+    gss.autotune_enable(True)
+    gss.autotune_enable(False)
+    r = gss.AutotuneResult()
+    lib = gss.lib()
+    assert lib.gss_autotune_device32(None, 0, None, 10, C.byref(r), None) < 0
+    assert lib.gss_autotune_device32(C.c_void_p(1 << 20), 64, C.c_void_p(1 << 21), 1,
+                                     C.byref(r), None) < 0
+    assert "fewer than two" in gss.last_error()
