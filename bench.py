@@ -498,6 +498,21 @@ def main():
     alg_bytes = BYTES_PER_ROUND_STRING * round_strings + 4 * ss.n + (8 * (ss.n - 1) if args.lcp else 0)
     hbm_gbps = alg_bytes / (ms_per_step * 1e-3) / 1e9
 
+    # Per pass: every global MSD pass and the local phase, bracketed as a
+    # whole (HIP events on the launch stream), against 24 B per active
+    # string (per string-round in the local phase).
+    passes = []
+    for name in [f"pass{r}" for r in range(9)] + ["pass9+", "local_phase"]:
+        launches, ms, items = gss.profile_get(name)
+        if not launches:
+            continue
+        units = items / launches
+        avg = ms / launches
+        gbps = BYTES_PER_ROUND_STRING * units / (avg * 1e-3) / 1e9
+        passes.append({"pass": name, "strings": units, "ms": avg,
+                       "alg_bytes": BYTES_PER_ROUND_STRING * units, "gb_per_s": gbps,
+                       "frac": gbps / HBM_PEAK_GBPS})
+
     dominant = max(prof, key=lambda k: prof[k]["ms_total"])
     d = prof[dominant]
     per_launch_bytes = BYTES_PER_ROUND_STRING * d["items"] / d["launches"]
@@ -525,6 +540,7 @@ def main():
                    "local_round_strings": int(st.local_round_strings),
                    "local_rounds_max": int(st.local_rounds_max)},
         "kernels": prof,
+        "passes": passes,
         "roofline": roofline,
     }
 

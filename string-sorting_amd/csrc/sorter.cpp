@@ -431,6 +431,9 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
       GSS_HIP(hipMemcpyAsync(p.aux, w.h_aux, sizeof(SegAux) * nbig, hipMemcpyHostToDevice, st));
       const BigSeg* segs = p.big[cur];
       BigSeg* next = p.big[cur ^ 1];
+      static const char* kPassNames[10] = {"pass0", "pass1", "pass2", "pass3", "pass4",
+                                           "pass5", "pass6", "pass7", "pass8", "pass9+"};
+      ProfToken pass_tok = prof_begin(c, kPassNames[pass < 9 ? pass : 9], active, st);
       GSS_LAUNCH("gather", active, launch_gather_minmax(p, segs, ntiles, st));
       GSS_LAUNCH("segreduce", nbig, launch_segreduce(p, nbig, st));
       GSS_LAUNCH("layout", nbig, launch_layout(p, nbig, rb, st));
@@ -441,6 +444,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
       GSS_LAUNCH("segscan", nbig, launch_segscan(p, segs, next, nbig, rb, st));
       GSS_LAUNCH("scatter", active, launch_scatter(p, segs, ntiles, rb, st));
       if (lcp) GSS_LAUNCH("boundary", 0, launch_boundary_lcp(p, bound_first, st));
+      prof_end(c, pass_tok, st);
       GSS_HIP(hipMemcpyAsync(w.h_ctr, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
       GSS_HIP(hipMemcpyAsync(w.h_big, next, sizeof(BigSeg) * p.cap_big, hipMemcpyDeviceToHost,
                              st));
@@ -460,6 +464,9 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     }
     nretire = w.h_ctr->retire;
   }
+  // The local phase as one bracket ("local_phase"; its items, the strings
+  // summed over the local key rounds, are filled in after the read-back).
+  ProfToken phase_tok = prof_begin(c, "local_phase", 0, st);
   if (nretire) GSS_LAUNCH("retire", nretire, launch_retire(p, nretire, st));
   static const char* kLocalNames[3] = {"local64", "local1024", "local8192"};
   long local_tok[3] = {-1, -1, -1};  // pending profile entries of the local launches
@@ -473,6 +480,9 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     stats.local_segments += nlocal[cls];
     stats.local_strings += local_strings[cls];
   }
+  const size_t before_end = c.pending.size();
+  prof_end(c, phase_tok, st);
+  const long phase_idx = c.pending.size() > before_end ? (long)before_end : -1;
   GSS_HIP(hipMemcpyAsync(w.h_ctr, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
   GSS_HIP(hipMemcpyAsync(w.h_lstats, p.lstats, sizeof(LocalStats), hipMemcpyDeviceToHost, st));
   GSS_HIP(hipStreamSynchronize(st));
@@ -487,6 +497,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     if (local_tok[cls] >= 0) c.pending[local_tok[cls]].items = round_strings[cls];
   }
   stats.local_rounds_max = (uint32_t)rounds_max;
+  if (phase_idx >= 0) c.pending[phase_idx].items = stats.local_round_strings;
   prof_collect(c);
   if (w.h_ctr->error)
     return fail(GSS_E_INTERNAL,
