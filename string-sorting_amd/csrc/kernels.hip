@@ -305,7 +305,10 @@ __global__ __launch_bounds__(1024) void k_layout(Plan p, uint32_t nsegs) {
     }
     carry += total;
   }
-  if (threadIdx.x == 0) p.ctr->flat_len = carry;
+  if (threadIdx.x == 0) {
+    p.ctr->flat_len = carry;
+    p.ctr->big_next = 0;  // the coming segscan counts this pass's big children
+  }
 }
 
 // ---------------------------------------------------------------------------

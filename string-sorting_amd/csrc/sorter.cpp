@@ -38,6 +38,13 @@ struct Workspace {
   LocalStats* h_lstats = nullptr;
   LocalSeg* h_local = nullptr;  // initial local segments, 3 classes of cap_init
   uint32_t cap_init = 0;
+  // One pinned block per transfer direction of a pass: the segment aux table
+  // and the tile table go up in one copy (they are adjacent on the device
+  // too), the counters and both segment lists come back in one copy.
+  char* h_up = nullptr;
+  size_t up_tiles_off = 0;  // byte offset of the tiles behind the aux table
+  char* h_rb = nullptr;
+  size_t rb_bytes = 0;
 };
 
 struct ProfTotal {
@@ -179,11 +186,13 @@ void prof_collect(Context& c) {
 void free_workspace(Workspace& w) {
   if (w.base) (void)hipFree(w.base);
   if (w.h_big) (void)hipHostFree(w.h_big);
-  if (w.h_tiles) (void)hipHostFree(w.h_tiles);
-  if (w.h_aux) (void)hipHostFree(w.h_aux);
+  if (w.h_up) (void)hipHostFree(w.h_up);
+  if (w.h_rb) (void)hipHostFree(w.h_rb);
   if (w.h_lstats) (void)hipHostFree(w.h_lstats);
   w.base = nullptr;
   w.h_big = nullptr;
+  w.h_up = nullptr;
+  w.h_rb = nullptr;
   w.h_tiles = nullptr;
   w.h_aux = nullptr;
   w.h_lstats = nullptr;
@@ -223,13 +232,14 @@ std::vector<Item> workspace_items(Plan& p, uint32_t** idx_copy, const Caps& k, u
       {(void**)&p.idx[2], 4 * k.ncap},
       {(void**)idx_copy, 4 * k.ncap},
       {(void**)&p.keys, 8 * k.ncap},
+      {(void**)&p.ctr, sizeof(Counters)},  // ctr, big[0], big[1]: read back together
       {(void**)&p.big[0], sizeof(BigSeg) * k.big},
       {(void**)&p.big[1], sizeof(BigSeg) * k.big},
       {(void**)&p.segmm, 16 * k.big},
       {(void**)&p.bstart, 4 * k.big * nb},
       {(void**)&p.alpha, 32},
+      {(void**)&p.aux, sizeof(SegAux) * k.big},  // aux, tiles: uploaded together
       {(void**)&p.tiles, sizeof(Tile) * k.tiles},
-      {(void**)&p.aux, sizeof(SegAux) * k.big},
       {(void**)&p.flat, 4 * (k.tiles * nb + 1)},
       {(void**)&p.flat_blk, 4 * ((k.tiles * nb) / kScanBlock + 2)},
       {(void**)&p.local[0], sizeof(LocalSeg) * k.local[0]},
@@ -237,7 +247,6 @@ std::vector<Item> workspace_items(Plan& p, uint32_t** idx_copy, const Caps& k, u
       {(void**)&p.local[2], sizeof(LocalSeg) * k.local[2]},
       {(void**)&p.retire, sizeof(Retire) * k.retire},
       {(void**)&p.bounds, sizeof(Boundary) * k.bounds},
-      {(void**)&p.ctr, sizeof(Counters)},
       {(void**)&p.lstats, sizeof(LocalStats)},
       {(void**)&p.tile_mm, 16 * k.tiles},
       {(void**)&p.tile_alpha, 32 * k.tiles},
@@ -288,8 +297,14 @@ int ensure_workspace(Context& c, uint64_t n, uint32_t rb, uint32_t t) {
   p.cap_bounds = (uint32_t)cap_bounds;
   for (int i = 0; i < 3; ++i) p.cap_local[i] = (uint32_t)cap_local[i];
   GSS_HIP(hipHostMalloc((void**)&w.h_big, sizeof(BigSeg) * cap_big, hipHostMallocDefault));
-  GSS_HIP(hipHostMalloc((void**)&w.h_tiles, sizeof(Tile) * cap_tiles, hipHostMallocDefault));
-  GSS_HIP(hipHostMalloc((void**)&w.h_aux, sizeof(SegAux) * cap_big, hipHostMallocDefault));
+  w.up_tiles_off = (size_t)(reinterpret_cast<char*>(p.tiles) - reinterpret_cast<char*>(p.aux));
+  GSS_HIP(hipHostMalloc((void**)&w.h_up, w.up_tiles_off + sizeof(Tile) * cap_tiles,
+                        hipHostMallocDefault));
+  w.h_aux = reinterpret_cast<SegAux*>(w.h_up);
+  w.h_tiles = reinterpret_cast<Tile*>(w.h_up + w.up_tiles_off);
+  w.rb_bytes = (size_t)(reinterpret_cast<char*>(p.big[1]) - reinterpret_cast<char*>(p.ctr)) +
+               sizeof(BigSeg) * cap_big;
+  GSS_HIP(hipHostMalloc((void**)&w.h_rb, w.rb_bytes, hipHostMallocDefault));
   GSS_HIP(hipHostMalloc((void**)&w.h_lstats, sizeof(LocalStats), hipHostMallocDefault));
   return GSS_OK;
 }
@@ -426,9 +441,8 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
         w.h_aux[i] = SegAux{0, ntiles - first, 0, first};
       }
       if (pass < GSS_MAX_PASSES) stats.active[pass] = active;
-      GSS_HIP(hipMemcpyAsync(p.tiles, w.h_tiles, sizeof(Tile) * ntiles, hipMemcpyHostToDevice,
-                             st));
-      GSS_HIP(hipMemcpyAsync(p.aux, w.h_aux, sizeof(SegAux) * nbig, hipMemcpyHostToDevice, st));
+      GSS_HIP(hipMemcpyAsync(p.aux, w.h_up, w.up_tiles_off + sizeof(Tile) * ntiles,
+                             hipMemcpyHostToDevice, st));
       const BigSeg* segs = p.big[cur];
       BigSeg* next = p.big[cur ^ 1];
       static const char* kPassNames[10] = {"pass0", "pass1", "pass2", "pass3", "pass4",
@@ -440,15 +454,19 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
       GSS_LAUNCH("histogram", active, launch_histogram(p, ntiles, rb, st));
       GSS_LAUNCH("flat_scan", (uint64_t)ntiles * nb,
                  launch_flat_scan(p, (uint64_t)ntiles * nb, st));
-      GSS_HIP(hipMemsetAsync(&p.ctr->big_next, 0, sizeof(uint32_t), st));
       GSS_LAUNCH("segscan", nbig, launch_segscan(p, segs, next, nbig, rb, st));
       GSS_LAUNCH("scatter", active, launch_scatter(p, segs, ntiles, rb, st));
       if (lcp) GSS_LAUNCH("boundary", 0, launch_boundary_lcp(p, bound_first, st));
       prof_end(c, pass_tok, st);
-      GSS_HIP(hipMemcpyAsync(w.h_ctr, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
-      GSS_HIP(hipMemcpyAsync(w.h_big, next, sizeof(BigSeg) * p.cap_big, hipMemcpyDeviceToHost,
-                             st));
+      // Counters and the next segment list in one copy (they are adjacent).
+      const size_t next_off =
+          (size_t)(reinterpret_cast<char*>(next) - reinterpret_cast<char*>(p.ctr));
+      GSS_HIP(hipMemcpyAsync(w.h_rb, p.ctr, next_off + sizeof(BigSeg) * p.cap_big,
+                             hipMemcpyDeviceToHost, st));
       GSS_HIP(hipStreamSynchronize(st));
+      memcpy(w.h_ctr, w.h_rb, sizeof(Counters));
+      if (w.h_ctr->big_next <= p.cap_big)
+        memcpy(w.h_big, w.h_rb + next_off, sizeof(BigSeg) * w.h_ctr->big_next);
       if (w.h_ctr->error)
         return fail(GSS_E_INTERNAL, "device pass failed (code " +
                                         std::to_string(w.h_ctr->error) + ")");
