@@ -241,6 +241,22 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
     const bool wave_big = __ballot(big) != 0;
     __syncthreads();
 
+    // A round in which no group found a differing key or a NUL only moves
+    // every group KW keys deeper: nothing to sort, split or compact.
+    bool stopped = false;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j)
+      if ((uint32_t)j < jn && gv[j] != 0xFFFFu && s.ss_stop[gv[j]] != 0) stopped = true;
+    if (!__syncthreads_or(stopped)) {
+      for (uint32_t g = threadIdx.x; g < nss; g += BLOCK) s.ss_depth[g] += 8 * KW;
+      __syncthreads();
+      if ((uint64_t)rounds > p.nwords + 2) {
+        if (threadIdx.x == 0) atomicOr(&p.ctr->error, 32u);
+        break;
+      }
+      continue;
+    }
+
     // 3. Each string's key: the first key its group stops at, still in
     //    registers (reloading it would miss L2: a round's lines do not stay
     //    there); a group that does not stop gets kSkipKey.
