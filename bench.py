@@ -437,8 +437,8 @@ def cpu_baseline(ss, runs):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is a "
-              "design in DESIGN.md, not built.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); sharding the sort "
+              "across GPUs is designed but not built.", file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -3,7 +3,7 @@
  *
  * Plain-C restatement of the reference's string-sort semantics.  Only the
  * tests, the smoke check and the CPU-baseline leg of the benchmark use it, and
- * only as the checker; the product library never links or calls it.
+ * only to compare results with; the product library never links or calls it.
  * Reference paths are relative to the rantala/string-sorting source tree.
  */
 #include "oracle.h"

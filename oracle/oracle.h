@@ -2,8 +2,8 @@
  * CPU oracle for the MSD radix string sort -- TEST INFRASTRUCTURE ONLY.
  *
  * A plain-C restatement of the reference's string-sort semantics, used by
- * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
- * checker.  Nothing in the product library links, loads or calls this code.
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg to compare
+ * results with.  Nothing in the product library links, loads or calls this code.
  *
  * Every function cites the reference file:line it restates (paths relative to
  * the rantala/string-sorting source tree).

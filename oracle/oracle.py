@@ -1,7 +1,7 @@
 """Python access to the CPU oracle -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
-module, and only as the checker or the timed CPU baseline.  The product
+module, and only to compare results with or as the timed CPU baseline.  The product
 library never imports it.
 
 * ``liboracle.so`` -- the plain-C restatement in oracle.c (built by

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker:
+"""Pins the CPU oracle (oracle/oracle.c) before results are compared with it:
 against the golden vectors produced by the reference's own compiled code,
 against a textbook comparison sort, against the reference unit test's known
 answers, and against closed forms.  CPU only."""
