@@ -45,8 +45,9 @@ HBM_PEAK_GBPS = 8000.0
 # Algorithmic bytes per active string per MSD round: read its 32-bit offset
 # and 8 text bytes, write the partitioned (8-byte key, offset) pair.
 BYTES_PER_ROUND_STRING = 2 * (4 + 8)
-KERNELS = ["gather", "layout", "histogram", "flat_scan", "segscan", "scatter", "boundary",
-           "retire", "local64", "local1024", "local8192", "init", "init_segs"]
+KERNELS = ["gather", "segreduce", "layout", "histogram", "flat_scan", "segscan", "scatter",
+           "boundary", "retire", "local64", "local1024", "local8192", "local64_early",
+           "local1024_early", "local8192_early"]
 
 
 def log(msg):
@@ -513,8 +514,19 @@ def main():
                        "alg_bytes": BYTES_PER_ROUND_STRING * units, "gb_per_s": gbps,
                        "frac": gbps / HBM_PEAK_GBPS})
 
-    dominant = max(prof, key=lambda k: prof[k]["ms_total"])
-    d = prof[dominant]
+    # A local class launched early (overlapping the next global pass, on a
+    # second stream) and finally is one kernel: combine both for the roofline.
+    combined = {}
+    for k, v in prof.items():
+        c = combined.setdefault(k.replace("_early", ""),
+                                {"launches": 0, "ms_total": 0.0, "items": 0})
+        c["launches"] += v["launches"]
+        c["ms_total"] += v["ms_total"]
+        c["items"] += v["items"]
+    for v in combined.values():
+        v["ms_avg"] = v["ms_total"] / v["launches"]
+    dominant = max(combined, key=lambda k: combined[k]["ms_total"])
+    d = combined[dominant]
     per_launch_bytes = BYTES_PER_ROUND_STRING * d["items"] / d["launches"]
     achieved = per_launch_bytes / (d["ms_avg"] * 1e-3) / 1e9
     traffic = measured_traffic(dominant)

@@ -356,7 +356,8 @@ hipError_t launch_scatter(const Plan& p, const BigSeg* segs, uint32_t ntiles,
                           uint32_t radix_bits, hipStream_t s);
 hipError_t launch_boundary_lcp(const Plan& p, uint32_t first, hipStream_t s);
 hipError_t launch_retire(const Plan& p, uint32_t nretire, hipStream_t s);
-hipError_t launch_local(const Plan& p, int cls, uint32_t nsegs, hipStream_t s);
+// Finishes local segments [first, first + nsegs) of class cls.
+hipError_t launch_local(const Plan& p, int cls, uint32_t first, uint32_t nsegs, hipStream_t s);
 hipError_t launch_zero_counters(const Plan& p, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* dst, uint64_t n, uint64_t v, hipStream_t s);
 hipError_t launch_ptr_to_off(const uint64_t* ptrs, uint64_t base, uint32_t* off,

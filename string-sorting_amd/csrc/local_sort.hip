@@ -418,17 +418,17 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
   }
 }
 
-hipError_t launch_local(const Plan& p, int cls, uint32_t nsegs, hipStream_t s) {
+hipError_t launch_local(const Plan& p, int cls, uint32_t first, uint32_t nsegs, hipStream_t s) {
   if (!nsegs) return hipSuccess;
   switch (cls) {
     case 0:
-      k_local<64, 64><<<nsegs, 64, 0, s>>>(p, p.local[0]);
+      k_local<64, 64><<<nsegs, 64, 0, s>>>(p, p.local[0] + first);
       break;
     case 1:
-      k_local<256, 1024><<<nsegs, 256, 0, s>>>(p, p.local[1]);
+      k_local<256, 1024><<<nsegs, 256, 0, s>>>(p, p.local[1] + first);
       break;
     case 2:
-      k_local<1024, 8192><<<nsegs, 1024, 0, s>>>(p, p.local[2]);
+      k_local<1024, 8192><<<nsegs, 1024, 0, s>>>(p, p.local[2] + first);
       break;
     default:
       return hipErrorInvalidValue;

@@ -79,6 +79,10 @@ int local_class(uint64_t size) {
 struct Context {
   int device = -1;
   hipStream_t stream = nullptr;
+  // Local segments that a global pass finishes early are sorted on this
+  // stream while the next passes run on `stream`.
+  hipStream_t side = nullptr;
+  hipEvent_t side_done = nullptr;
   Workspace ws;
   std::mutex mu;
   gss_stats stats{};
@@ -108,6 +112,8 @@ gss_ctx* g_default[kMaxDevices] = {};
 int init_context(Context& c) {
   GSS_HIP(hipGetDevice(&c.device));
   GSS_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  GSS_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+  GSS_HIP(hipEventCreateWithFlags(&c.side_done, hipEventDisableTiming));
   GSS_HIP(hipHostMalloc((void**)&c.ws.h_ctr, sizeof(Counters), hipHostMallocDefault));
   return GSS_OK;
 }
@@ -247,7 +253,7 @@ std::vector<Item> workspace_items(Plan& p, uint32_t** idx_copy, const Caps& k, u
       {(void**)&p.local[2], sizeof(LocalSeg) * k.local[2]},
       {(void**)&p.retire, sizeof(Retire) * k.retire},
       {(void**)&p.bounds, sizeof(Boundary) * k.bounds},
-      {(void**)&p.lstats, sizeof(LocalStats)},
+      {(void**)&p.lstats, 2 * sizeof(LocalStats)},  // [0]: main stream, [1]: side stream
       {(void**)&p.tile_mm, 16 * k.tiles},
       {(void**)&p.tile_alpha, 32 * k.tiles},
       {(void**)&p.seg_alpha, 32 * k.big},
@@ -305,7 +311,7 @@ int ensure_workspace(Context& c, uint64_t n, uint32_t rb, uint32_t t) {
   w.rb_bytes = (size_t)(reinterpret_cast<char*>(p.big[1]) - reinterpret_cast<char*>(p.ctr)) +
                sizeof(BigSeg) * cap_big;
   GSS_HIP(hipHostMalloc((void**)&w.h_rb, w.rb_bytes, hipHostMallocDefault));
-  GSS_HIP(hipHostMalloc((void**)&w.h_lstats, sizeof(LocalStats), hipHostMallocDefault));
+  GSS_HIP(hipHostMalloc((void**)&w.h_lstats, 2 * sizeof(LocalStats), hipHostMallocDefault));
   return GSS_OK;
 }
 
@@ -414,9 +420,44 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   if (nbig)
     GSS_HIP(hipMemcpyAsync(p.big[0], w.h_big, sizeof(BigSeg) * nbig, hipMemcpyHostToDevice, st));
   GSS_HIP(hipMemcpyAsync(p.ctr, &h0, sizeof(Counters), hipMemcpyHostToDevice, st));
-  GSS_HIP(launch_fill_u64(reinterpret_cast<uint64_t*>(p.lstats), sizeof(LocalStats) / 8, 0, st));
+  GSS_HIP(launch_fill_u64(reinterpret_cast<uint64_t*>(p.lstats), 2 * sizeof(LocalStats) / 8, 0,
+                          st));
+  // Local segments launched early run on the side stream with their own
+  // statistics block; the side stream first waits for the uploads above.
+  Plan p_side = p;
+  p_side.lstats = p.lstats + 1;
+  uint32_t launched[3] = {0, 0, 0};
+  uint64_t early_strings = 0;  // strings of the local segments launched early
+  bool side_used = false;
+  long side_tok[3] = {-1, -1, -1};
+  auto launch_early = [&](const uint32_t* upto) -> int {
+    bool any = false;
+    for (int cls = 0; cls < 3; ++cls) any = any || upto[cls] > launched[cls];
+    if (!any) return GSS_OK;
+    // The side stream waits for the work queued on the main stream so far.
+    GSS_HIP(hipEventRecord(c.side_done, st));
+    GSS_HIP(hipStreamWaitEvent(c.side, c.side_done, 0));
+    side_used = true;
+    static const char* kEarlyNames[3] = {"local64_early", "local1024_early", "local8192_early"};
+    for (int cls = 2; cls >= 0; --cls) {
+      if (upto[cls] <= launched[cls]) continue;
+      if (upto[cls] > p.cap_local[cls]) return fail(GSS_E_INTERNAL, "local segment list overflow");
+      const size_t mark = c.pending.size();
+      ProfToken tok = prof_begin(c, kEarlyNames[cls], 0, c.side);
+      GSS_HIP(launch_local(p_side, cls, launched[cls], upto[cls] - launched[cls], c.side));
+      prof_end(c, tok, c.side);
+      if (c.pending.size() > mark) side_tok[cls] = (long)mark;
+      launched[cls] = upto[cls];
+    }
+    return GSS_OK;
+  };
 
   uint32_t nlocal[3] = {h0.local[0], h0.local[1], h0.local[2]};
+  if (nbig) {  // small requests of a batch sort while the big ones take global passes
+    const int erc = launch_early(nlocal);
+    if (erc) return erc;
+    for (int cls = 0; cls < 3; ++cls) early_strings += h0.local_strings[cls];
+  }
   uint32_t local_strings[3] = {h0.local_strings[0], h0.local_strings[1], h0.local_strings[2]};
   uint32_t nretire = h0.retire;
   if (nbig) {
@@ -474,6 +515,22 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
       bound_first = w.h_ctr->boundary;
       cur ^= 1;
       ++pass;
+      if (nbig) {
+        // More passes follow.  The local segments found so far are sorted on
+        // the side stream meanwhile when they hold at least as many strings
+        // as the next pass: a heavier pass would only contend with them for
+        // memory bandwidth.
+        uint64_t next_active = 0, waiting = 0;
+        for (uint32_t i = 0; i < nbig; ++i) next_active += w.h_big[i].size;
+        for (int cls = 0; cls < 3; ++cls) waiting += w.h_ctr->local_strings[cls];
+        waiting -= early_strings;
+        if (waiting >= next_active) {
+          const uint32_t upto[3] = {w.h_ctr->local[0], w.h_ctr->local[1], w.h_ctr->local[2]};
+          const int erc = launch_early(upto);
+          if (erc) return erc;
+          early_strings += waiting;
+        }
+      }
     }
     stats.global_passes = pass;
     for (int i = 0; i < 3; ++i) {
@@ -490,9 +547,10 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   long local_tok[3] = {-1, -1, -1};  // pending profile entries of the local launches
   for (int cls = 2; cls >= 0; --cls) {
     if (nlocal[cls] > p.cap_local[cls]) return fail(GSS_E_INTERNAL, "local segment list overflow");
-    if (nlocal[cls]) {
+    if (nlocal[cls] > launched[cls]) {
       const size_t mark = c.pending.size();
-      GSS_LAUNCH(kLocalNames[cls], 0, launch_local(p, cls, nlocal[cls], st));
+      GSS_LAUNCH(kLocalNames[cls], 0,
+                 launch_local(p, cls, launched[cls], nlocal[cls] - launched[cls], st));
       if (c.pending.size() > mark) local_tok[cls] = (long)mark;
     }
     stats.local_segments += nlocal[cls];
@@ -501,21 +559,31 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   const size_t before_end = c.pending.size();
   prof_end(c, phase_tok, st);
   const long phase_idx = c.pending.size() > before_end ? (long)before_end : -1;
-  GSS_HIP(hipMemcpyAsync(w.h_ctr, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
-  GSS_HIP(hipMemcpyAsync(w.h_lstats, p.lstats, sizeof(LocalStats), hipMemcpyDeviceToHost, st));
-  GSS_HIP(hipStreamSynchronize(st));
-  // A local launch's unit of work is one active string in one key round.
-  uint64_t round_strings[3] = {0, 0, 0}, rounds_max = 0;
-  for (int slot = 0; slot < kStatSlots; ++slot) {
-    for (int cls = 0; cls < 3; ++cls) round_strings[cls] += w.h_lstats->v[slot][cls];
-    rounds_max = std::max<uint64_t>(rounds_max, w.h_lstats->v[slot][3]);
+  if (side_used) {  // the early local sorts must be done before the results are
+    GSS_HIP(hipEventRecord(c.side_done, c.side));
+    GSS_HIP(hipStreamWaitEvent(st, c.side_done, 0));
   }
+  GSS_HIP(hipMemcpyAsync(w.h_ctr, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
+  GSS_HIP(hipMemcpyAsync(w.h_lstats, p.lstats, 2 * sizeof(LocalStats), hipMemcpyDeviceToHost,
+                         st));
+  GSS_HIP(hipStreamSynchronize(st));
+  // A local launch's unit of work is one active string in one key round;
+  // block 0 of the statistics holds the final launches, block 1 the early ones.
+  uint64_t round_strings[2][3] = {{0, 0, 0}, {0, 0, 0}}, rounds_max = 0;
+  for (int b = 0; b < 2; ++b)
+    for (int slot = 0; slot < kStatSlots; ++slot) {
+      for (int cls = 0; cls < 3; ++cls) round_strings[b][cls] += w.h_lstats[b].v[slot][cls];
+      rounds_max = std::max<uint64_t>(rounds_max, w.h_lstats[b].v[slot][3]);
+    }
+  uint64_t final_round_strings = 0;
   for (int cls = 0; cls < 3; ++cls) {
-    stats.local_round_strings += round_strings[cls];
-    if (local_tok[cls] >= 0) c.pending[local_tok[cls]].items = round_strings[cls];
+    stats.local_round_strings += round_strings[0][cls] + round_strings[1][cls];
+    final_round_strings += round_strings[0][cls];
+    if (local_tok[cls] >= 0) c.pending[local_tok[cls]].items = round_strings[0][cls];
+    if (side_tok[cls] >= 0) c.pending[side_tok[cls]].items = round_strings[1][cls];
   }
   stats.local_rounds_max = (uint32_t)rounds_max;
-  if (phase_idx >= 0) c.pending[phase_idx].items = stats.local_round_strings;
+  if (phase_idx >= 0) c.pending[phase_idx].items = final_round_strings;
   prof_collect(c);
   if (w.h_ctr->error)
     return fail(GSS_E_INTERNAL,
@@ -652,6 +720,9 @@ void gss_ctx_destroy(gss_ctx* h) {
     if (h->c.ws.h_ctr) (void)hipHostFree(h->c.ws.h_ctr);
     if (h->c.ws.h_local) (void)hipHostFree(h->c.ws.h_local);
     for (auto e : h->c.events) (void)hipEventDestroy(e);
+    (void)hipStreamSynchronize(h->c.side);
+    (void)hipEventDestroy(h->c.side_done);
+    (void)hipStreamDestroy(h->c.side);
     (void)hipStreamDestroy(h->c.stream);
   }
   delete h;
