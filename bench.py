@@ -44,7 +44,7 @@ METRIC = "Mstrings/s + HBM GB/s (10M URL-like, 1 GPU); batched p50/p99 vs plain"
 HBM_PEAK_GBPS = 8000.0
 # Algorithmic bytes per active string per MSD round: read its 32-bit offset
 # and 8 text bytes, write the partitioned (8-byte key, offset) pair.
-BYTES_PER_ROUND_STRING = 2 * (4 + 8)
+BYTES_PER_ROUND_STRING = 2 * (4 + 8)  # per string and 8-byte key level
 KERNELS = ["gather", "segreduce", "layout", "histogram", "flat_scan", "segscan", "scatter",
            "boundary", "retire", "local64", "local1024", "local8192", "local64_early",
            "local1024_early", "local8192_early"]
@@ -269,6 +269,7 @@ def all_configs(dev, runs=5):
         st = gss.stats()
         row["global_passes"] = int(st.global_passes)
         row["local_round_strings"] = int(st.local_round_strings)
+        row["local_level_strings"] = int(st.local_level_strings)
         out[gen.NAMES[cfg]] = row
         del text, off, o, lcp, ss
         torch.cuda.empty_cache()
@@ -466,14 +467,20 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    gss.profile_reset()
-    gss.profile_enable(True)
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    # Kernel and pass breakdown: the same sort again, each launch and pass
+    # bracketed by HIP events (and the local kernels counting key levels),
+    # outside the timed steps above.
+    prof_steps = max(1, min(args.steps, 10))
+    gss.profile_reset()
+    gss.profile_enable(True)
+    for _ in range(prof_steps):
+        step()
+    torch.cuda.synchronize()
     gss.profile_enable(False)
     st = gss.stats()
     if args.calibrate_copy:
@@ -493,15 +500,17 @@ def main():
         if launches:
             prof[k] = {"launches": launches, "ms_total": ms, "ms_avg": ms / launches,
                        "items": items}
-    # Work units of one sort: active strings summed over all rounds.
+    # Work units of one sort: active strings summed over all 8-byte MSD key
+    # levels -- one level per global pass, and in the local kernel the levels
+    # each string's 32-byte rounds took it through.
     global_rounds = [int(st.active[i]) for i in range(min(st.global_passes, 64))]
-    round_strings = sum(global_rounds) + int(st.local_round_strings)
+    round_strings = sum(global_rounds) + int(st.local_level_strings)
     alg_bytes = BYTES_PER_ROUND_STRING * round_strings + 4 * ss.n + (8 * (ss.n - 1) if args.lcp else 0)
     hbm_gbps = alg_bytes / (ms_per_step * 1e-3) / 1e9
 
     # Per pass: every global MSD pass and the local phase, bracketed as a
     # whole (HIP events on the launch stream), against 24 B per active
-    # string (per string-round in the local phase).
+    # string per 8-byte key level (a local round covers up to four levels).
     passes = []
     for name in [f"pass{r}" for r in range(9)] + ["pass9+", "local_phase"]:
         launches, ms, items = gss.profile_get(name)
@@ -539,6 +548,7 @@ def main():
     res = {
         "metric": METRIC, "value": value, "unit": "Mstrings/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "profiled_steps": prof_steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
         "data": "synthetic (seeded URL-like generator, string-sorting_amd/gss/gen.py)",
         "config": {"workload": "url_like_10M", "strings": ss.n, "text_bytes": ss.text_bytes,
@@ -550,6 +560,7 @@ def main():
                    "local_segments": int(st.local_segments),
                    "local_strings": int(st.local_strings),
                    "local_round_strings": int(st.local_round_strings),
+                   "local_level_strings": int(st.local_level_strings),
                    "local_rounds_max": int(st.local_rounds_max)},
         "kernels": prof,
         "passes": passes,

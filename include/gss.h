@@ -253,6 +253,11 @@ typedef struct gss_stats {
 	uint64_t local_strings;  /* strings finished inside one workgroup */
 	uint32_t local_rounds_max;  /* most 8-byte key rounds one such segment took */
 	uint64_t local_round_strings; /* sum over those rounds of active strings */
+	uint64_t local_level_strings; /* sum over strings of the 8-byte key levels
+					 the local rounds took them through (a
+					 round reads 32 bytes: up to 4 levels);
+					 counted only while gss_profile_enable
+					 is on, else 0 */
 } gss_stats;
 
 int gss_get_stats(gss_stats *out);            /* last sort on this thread */

@@ -94,7 +94,8 @@ struct Counters {
 // updates slot b % kStatSlots, each slot in its own 256-byte line) so that
 // thousands of blocks do not serialise on one address; the host combines
 // the slots.  v[slot][c] for c < 3: sum over class-c rounds of the active
-// strings; v[slot][3]: most rounds one segment needed.
+// strings; v[slot][3]: most rounds one segment needed; v[slot][4 + c]: the
+// 8-byte key levels class-c strings advanced through (up to 4 per round).
 constexpr int kStatSlots = 64;
 struct LocalStats {
   unsigned long long v[kStatSlots][32];
@@ -356,8 +357,10 @@ hipError_t launch_scatter(const Plan& p, const BigSeg* segs, uint32_t ntiles,
                           uint32_t radix_bits, hipStream_t s);
 hipError_t launch_boundary_lcp(const Plan& p, uint32_t first, hipStream_t s);
 hipError_t launch_retire(const Plan& p, uint32_t nretire, hipStream_t s);
-// Finishes local segments [first, first + nsegs) of class cls.
-hipError_t launch_local(const Plan& p, int cls, uint32_t first, uint32_t nsegs, hipStream_t s);
+// Finishes local segments [first, first + nsegs) of class cls; count_levels
+// also counts the 8-byte key levels into the statistics (slower: profiling).
+hipError_t launch_local(const Plan& p, int cls, uint32_t first, uint32_t nsegs, hipStream_t s,
+                        bool count_levels);
 hipError_t launch_zero_counters(const Plan& p, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* dst, uint64_t n, uint64_t v, hipStream_t s);
 hipError_t launch_ptr_to_off(const uint64_t* ptrs, uint64_t base, uint32_t* off,

@@ -145,7 +145,9 @@ __device__ __forceinline__ void local_radix_pass(LocalSmem<BLOCK, CAP>& s, uint3
   __syncthreads();
 }
 
-template <int BLOCK, int CAP>
+// COUNT: also count the 8-byte key levels the strings advance through (for
+// profiling only: the count costs registers).
+template <int BLOCK, int CAP, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
   constexpr int W = BLOCK / kWave;
   constexpr int IPT = CAP / BLOCK;
@@ -177,6 +179,10 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
   }
   uint32_t nact = n, nss = 1, rounds = 0;
   unsigned long long round_strings = 0;
+  // 8-byte key levels this wave's strings advanced through: a round's group
+  // that stops at key e consumed e + 1 of them, a skipping group KW.  Built
+  // from ballots, so the count is wave-uniform (scalar registers).
+  unsigned long long wave_levels = 0;
   __syncthreads();
 
   while (true) {
@@ -248,6 +254,11 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
     for (int j = 0; j < IPT; ++j)
       if ((uint32_t)j < jn && gv[j] != 0xFFFFu && s.ss_stop[gv[j]] != 0) stopped = true;
     if (!__syncthreads_or(stopped)) {
+      if (COUNT) {
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+          if ((uint32_t)j < jn) wave_levels += KW * (uint64_t)__popcll(__ballot(gv[j] != 0xFFFFu));
+      }
       for (uint32_t g = threadIdx.x; g < nss; g += BLOCK) s.ss_depth[g] += 8 * KW;
       __syncthreads();
       if ((uint64_t)rounds > p.nwords + 2) {
@@ -262,14 +273,23 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
     //    there); a group that does not stop gets kSkipKey.
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
-      if ((uint32_t)j < jn && gv[j] != 0xFFFFu) {
-        const uint32_t i = base + j * kWave;
-        const uint32_t e = __builtin_ctz(s.ss_stop[gv[j]] | (1u << KW));
-        uint64_t k = kSkipKey;
+      if ((uint32_t)j < jn) {
+        uint32_t lv = 0;
+        if (gv[j] != 0xFFFFu) {
+          const uint32_t i = base + j * kWave;
+          const uint32_t e = __builtin_ctz(s.ss_stop[gv[j]] | (1u << KW));
+          uint64_t k = kSkipKey;
 #pragma unroll
-        for (int q = 0; q < KW; ++q)
-          if ((uint32_t)q == e) k = kw[j][q];
-        s.key[i] = k;
+          for (int q = 0; q < KW; ++q)
+            if ((uint32_t)q == e) k = kw[j][q];
+          s.key[i] = k;
+          lv = e < (uint32_t)KW ? e + 1 : (uint32_t)KW;
+        }
+        if (COUNT) {
+#pragma unroll
+          for (int q = 1; q <= KW; ++q)
+            wave_levels += (uint64_t)__popcll(__ballot(lv >= (uint32_t)q));
+        }
       }
     }
     __syncthreads();
@@ -411,29 +431,37 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
       break;
     }
   }
+  unsigned long long* slot = p.lstats->v[blockIdx.x % kStatSlots];
+  if (COUNT && lane == 0 && wave_levels) atomicAdd(&slot[4 + kClass], wave_levels);
   if (threadIdx.x == 0) {
-    unsigned long long* slot = p.lstats->v[blockIdx.x % kStatSlots];
     atomicAdd(&slot[kClass], round_strings);
     atomicMax(&slot[3], (unsigned long long)rounds);
   }
 }
 
-hipError_t launch_local(const Plan& p, int cls, uint32_t first, uint32_t nsegs, hipStream_t s) {
-  if (!nsegs) return hipSuccess;
+template <bool COUNT>
+hipError_t launch_local_t(const Plan& p, int cls, uint32_t first, uint32_t nsegs, hipStream_t s) {
   switch (cls) {
     case 0:
-      k_local<64, 64><<<nsegs, 64, 0, s>>>(p, p.local[0] + first);
+      k_local<64, 64, COUNT><<<nsegs, 64, 0, s>>>(p, p.local[0] + first);
       break;
     case 1:
-      k_local<256, 1024><<<nsegs, 256, 0, s>>>(p, p.local[1] + first);
+      k_local<256, 1024, COUNT><<<nsegs, 256, 0, s>>>(p, p.local[1] + first);
       break;
     case 2:
-      k_local<1024, 8192><<<nsegs, 1024, 0, s>>>(p, p.local[2] + first);
+      k_local<1024, 8192, COUNT><<<nsegs, 1024, 0, s>>>(p, p.local[2] + first);
       break;
     default:
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_local(const Plan& p, int cls, uint32_t first, uint32_t nsegs, hipStream_t s,
+                        bool count_levels) {
+  if (!nsegs) return hipSuccess;
+  return count_levels ? launch_local_t<true>(p, cls, first, nsegs, s)
+                      : launch_local_t<false>(p, cls, first, nsegs, s);
 }
 
 }  // namespace gss

@@ -426,6 +426,8 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   // statistics block; the side stream first waits for the uploads above.
   Plan p_side = p;
   p_side.lstats = p.lstats + 1;
+  // Key levels are counted only while kernels are profiled.
+  const bool count_levels = g_prof.load(std::memory_order_relaxed);
   uint32_t launched[3] = {0, 0, 0};
   uint64_t early_strings = 0;  // strings of the local segments launched early
   bool side_used = false;
@@ -444,7 +446,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
       if (upto[cls] > p.cap_local[cls]) return fail(GSS_E_INTERNAL, "local segment list overflow");
       const size_t mark = c.pending.size();
       ProfToken tok = prof_begin(c, kEarlyNames[cls], 0, c.side);
-      GSS_HIP(launch_local(p_side, cls, launched[cls], upto[cls] - launched[cls], c.side));
+      GSS_HIP(launch_local(p_side, cls, launched[cls], upto[cls] - launched[cls], c.side, count_levels));
       prof_end(c, tok, c.side);
       if (c.pending.size() > mark) side_tok[cls] = (long)mark;
       launched[cls] = upto[cls];
@@ -550,7 +552,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     if (nlocal[cls] > launched[cls]) {
       const size_t mark = c.pending.size();
       GSS_LAUNCH(kLocalNames[cls], 0,
-                 launch_local(p, cls, launched[cls], nlocal[cls] - launched[cls], st));
+                 launch_local(p, cls, launched[cls], nlocal[cls] - launched[cls], st, count_levels));
       if (c.pending.size() > mark) local_tok[cls] = (long)mark;
     }
     stats.local_segments += nlocal[cls];
@@ -569,21 +571,28 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   GSS_HIP(hipStreamSynchronize(st));
   // A local launch's unit of work is one active string in one key round;
   // block 0 of the statistics holds the final launches, block 1 the early ones.
+  // The profiled unit of a local launch is one string through one 8-byte key
+  // level (the algorithmic unit of an MSD round with 8-byte keys).
   uint64_t round_strings[2][3] = {{0, 0, 0}, {0, 0, 0}}, rounds_max = 0;
+  uint64_t level_strings[2][3] = {{0, 0, 0}, {0, 0, 0}};
   for (int b = 0; b < 2; ++b)
     for (int slot = 0; slot < kStatSlots; ++slot) {
-      for (int cls = 0; cls < 3; ++cls) round_strings[b][cls] += w.h_lstats[b].v[slot][cls];
+      for (int cls = 0; cls < 3; ++cls) {
+        round_strings[b][cls] += w.h_lstats[b].v[slot][cls];
+        level_strings[b][cls] += w.h_lstats[b].v[slot][4 + cls];
+      }
       rounds_max = std::max<uint64_t>(rounds_max, w.h_lstats[b].v[slot][3]);
     }
-  uint64_t final_round_strings = 0;
+  uint64_t final_level_strings = 0;
   for (int cls = 0; cls < 3; ++cls) {
     stats.local_round_strings += round_strings[0][cls] + round_strings[1][cls];
-    final_round_strings += round_strings[0][cls];
-    if (local_tok[cls] >= 0) c.pending[local_tok[cls]].items = round_strings[0][cls];
-    if (side_tok[cls] >= 0) c.pending[side_tok[cls]].items = round_strings[1][cls];
+    stats.local_level_strings += level_strings[0][cls] + level_strings[1][cls];
+    final_level_strings += level_strings[0][cls];
+    if (local_tok[cls] >= 0) c.pending[local_tok[cls]].items = level_strings[0][cls];
+    if (side_tok[cls] >= 0) c.pending[side_tok[cls]].items = level_strings[1][cls];
   }
   stats.local_rounds_max = (uint32_t)rounds_max;
-  if (phase_idx >= 0) c.pending[phase_idx].items = final_round_strings;
+  if (phase_idx >= 0) c.pending[phase_idx].items = final_level_strings;
   prof_collect(c);
   if (w.h_ctr->error)
     return fail(GSS_E_INTERNAL,
