@@ -48,6 +48,7 @@ BYTES_PER_ROUND_STRING = 2 * (4 + 8)  # per string and 8-byte key level
 KERNELS = ["gather", "segreduce", "layout", "histogram", "flat_scan", "segscan", "scatter",
            "boundary", "retire", "local64", "local1024", "local8192", "local64_early",
            "local1024_early", "local8192_early"]
+PASSES = [f"pass{r}" for r in range(9)] + ["pass9+", "local_phase"]
 
 
 def log(msg):
@@ -56,8 +57,8 @@ def log(msg):
 
 # Library profile names -> device symbol fragments in the rocprofv3 tables.
 SYMBOLS = {"gather": "k_gather_minmax", "histogram": "k_histogram", "scatter": "k_scatter",
-           "segscan": "k_segscan", "local64": "k_local<64, 64>",
-           "local1024": "k_local<256, 1024>", "local8192": "k_local<1024, 8192>"}
+           "segscan": "k_segscan", "local64": "k_local<64, 64, false>",
+           "local1024": "k_local<256, 1024, false>", "local8192": "k_local<1024, 8192, false>"}
 PROFILE = ROOT / "profiles" / "url_like_10M_kernels.md"
 
 
@@ -473,15 +474,23 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     # Kernel and pass breakdown: the same sort again, each launch and pass
-    # bracketed by HIP events (and the local kernels counting key levels),
-    # outside the timed steps above.
+    # bracketed by HIP events, outside the timed steps above.  The key levels
+    # the local kernels take strings through (their work units) come from one
+    # more sort with their counting variant on; durations come from the
+    # production kernels.
     prof_steps = max(1, min(args.steps, 10))
     gss.profile_reset()
     gss.profile_enable(True)
     for _ in range(prof_steps):
         step()
     torch.cuda.synchronize()
+    timing = {k: gss.profile_get(k) for k in KERNELS + PASSES}
+    gss.profile_reset()
+    gss.profile_enable(True, levels=True)
+    step()
+    torch.cuda.synchronize()
     gss.profile_enable(False)
+    counting = {k: gss.profile_get(k) for k in KERNELS + PASSES}
     st = gss.stats()
     if args.calibrate_copy:
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
@@ -494,9 +503,16 @@ def main():
     ms_per_step = dt / args.steps * 1e3
     value = ss.n * args.steps / dt / 1e6
 
+    def per_sort(k):
+        """(launches, ms, units) of kernel or pass `k` per sort: launches
+        and time from the timing steps, units from the counting sort."""
+        launches, ms, _ = timing[k]
+        c_launches, _, c_items = counting[k]
+        return launches / prof_steps, ms / prof_steps, c_items if c_launches else 0
+
     prof = {}
     for k in KERNELS:
-        launches, ms, items = gss.profile_get(k)
+        launches, ms, items = per_sort(k)
         if launches:
             prof[k] = {"launches": launches, "ms_total": ms, "ms_avg": ms / launches,
                        "items": items}
@@ -512,8 +528,8 @@ def main():
     # whole (HIP events on the launch stream), against 24 B per active
     # string per 8-byte key level (a local round covers up to four levels).
     passes = []
-    for name in [f"pass{r}" for r in range(9)] + ["pass9+", "local_phase"]:
-        launches, ms, items = gss.profile_get(name)
+    for name in PASSES:
+        launches, ms, items = per_sort(name)
         if not launches:
             continue
         units = items / launches

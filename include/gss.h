@@ -256,18 +256,23 @@ typedef struct gss_stats {
 	uint64_t local_level_strings; /* sum over strings of the 8-byte key levels
 					 the local rounds took them through (a
 					 round reads 32 bytes: up to 4 levels);
-					 counted only while gss_profile_enable
-					 is on, else 0 */
+					 counted only under GSS_PROFILE_LEVELS,
+					 else 0 */
 } gss_stats;
 
 int gss_get_stats(gss_stats *out);            /* last sort on this thread */
 int gss_ctx_get_stats(gss_ctx *ctx, gss_stats *out); /* last sort of ctx */
 
-/* Kernel timing: when enabled, every kernel launch is bracketed by HIP events
- * recorded on the stream the kernel runs on; totals per kernel accumulate
- * until reset.  `name` selects the kernel ("gather", "histogram", "scatter",
- * "local", ...). */
-int gss_profile_enable(int on);
+/* Kernel timing: with GSS_PROFILE_EVENTS set, every kernel launch is
+ * bracketed by HIP events recorded on the stream the kernel runs on; totals
+ * per kernel accumulate until reset.  `name` selects the kernel ("gather",
+ * "histogram", "scatter", "local", ...).  GSS_PROFILE_LEVELS runs the local
+ * kernels' counting variant, which fills gss_stats.local_level_strings (and
+ * the items of the local kernels' totals) at some cost in kernel time: time
+ * with EVENTS alone, count with LEVELS.  0 turns both off. */
+#define GSS_PROFILE_EVENTS 1
+#define GSS_PROFILE_LEVELS 2
+int gss_profile_enable(int mode);
 int gss_profile_reset(void);
 int gss_profile_get(const char *name, uint64_t *launches, double *total_ms,
 		    uint64_t *items);

@@ -61,7 +61,8 @@ struct Pending {
 
 std::mutex g_mu;  // tuning, profile totals, default contexts
 gss_tuning g_tuning = {8192u, 11u, kTile};
-std::atomic<bool> g_prof{false};
+std::atomic<bool> g_prof{false};    // HIP event brackets
+std::atomic<bool> g_levels{false};  // counting variant of the local kernels
 std::map<std::string, ProfTotal> g_prof_totals;
 thread_local std::string g_last_error;
 thread_local gss_stats g_last_stats;
@@ -427,7 +428,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   Plan p_side = p;
   p_side.lstats = p.lstats + 1;
   // Key levels are counted only while kernels are profiled.
-  const bool count_levels = g_prof.load(std::memory_order_relaxed);
+  const bool count_levels = g_levels.load(std::memory_order_relaxed);
   uint32_t launched[3] = {0, 0, 0};
   uint64_t early_strings = 0;  // strings of the local segments launched early
   bool side_used = false;
@@ -779,8 +780,11 @@ int gss_get_stats(gss_stats* out) {
   return GSS_OK;
 }
 
-int gss_profile_enable(int on) {
-  g_prof.store(on != 0);
+int gss_profile_enable(int mode) {
+  if (mode & ~(GSS_PROFILE_EVENTS | GSS_PROFILE_LEVELS))
+    return fail(GSS_E_INVALID, "unknown profile mode bits");
+  g_prof.store((mode & GSS_PROFILE_EVENTS) != 0);
+  g_levels.store((mode & GSS_PROFILE_LEVELS) != 0);
   return GSS_OK;
 }
 

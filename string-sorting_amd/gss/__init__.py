@@ -488,8 +488,11 @@ def stats() -> Stats:
     return s
 
 
-def profile_enable(on: bool = True) -> None:
-    _check(lib().gss_profile_enable(1 if on else 0), "gss_profile_enable")
+def profile_enable(on: bool = True, levels: bool = False) -> None:
+    """Kernel timing (HIP event brackets) on or off; `levels` runs the
+    local kernels' counting variant, which fills Stats.local_level_strings."""
+    _check(lib().gss_profile_enable((1 if on else 0) | (2 if levels else 0)),
+           "gss_profile_enable")
 
 
 def profile_reset() -> None:

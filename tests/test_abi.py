@@ -90,6 +90,11 @@ def test_stats_and_profile_without_device():
     assert s.global_passes >= 0
     gss.profile_reset()
     assert gss.profile_get("scatter") == (0, 0.0, 0)
+    lib = gss.lib()
+    assert lib.gss_profile_enable(4) == -1  # GSS_E_INVALID: unknown mode bit
+    assert b"mode" in lib.gss_last_error()
+    gss.profile_enable(True, levels=True)
+    gss.profile_enable(False)
 
 
 def test_server_create_validates_before_touching_a_device():
