@@ -60,6 +60,11 @@ __device__ __forceinline__ uint64_t wave_shr1_64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Lane i receives lane i-1's value, lane 0 receives `first`.
+__device__ __forceinline__ uint32_t wave_shr1_32(uint32_t v, uint32_t first) {
+  return __builtin_amdgcn_update_dpp(first, v, 0x138, 0xF, 0xF, false);
+}
+
 __device__ __forceinline__ uint64_t lane63_64(uint64_t v) {
   const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, kWave - 1);
   const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), kWave - 1);
@@ -238,8 +243,14 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
           const uint64_t prev = lane ? up : wrap;
           if ((same && kw[j][q] != prev) || (kw[j][q] & 0xFFu) == 0) stop |= 1u << q;
         }
+        // Neighbouring lanes mostly share a group and its stop bits: only
+        // the first lane of each run of equal (group, bits) issues the LDS
+        // atomic, which would otherwise serialise on one word.
+        const uint32_t tag = (gv[j] << 4) | stop;
+        const uint32_t prev_tag = wave_shr1_32(tag, 0xFFFFFFFFu);
         if (gv[j] != 0xFFFFu) {
-          if (stop) atomicOr((uint32_t*)&s.ss_stop[0] + (gv[j] >> 2), stop << (8 * (gv[j] & 3)));
+          if (stop && tag != prev_tag)
+            atomicOr((uint32_t*)&s.ss_stop[0] + (gv[j] >> 2), stop << (8 * (gv[j] & 3)));
           big = big || (s.ss_start[gv[j] + 1] - s.ss_start[gv[j]]) > 16;
         }
       }
