@@ -227,6 +227,17 @@ __device__ __forceinline__ uint32_t key_lcp(uint64_t a, uint64_t b) {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Tile taken by block b of an n-block launch over the pass's tiles.  Blocks
+// are dispatched to the eight XCDs in turn (block b on XCD b % 8), each XCD
+// with its own L2; giving every XCD a contiguous range of tiles lets
+// neighbouring tiles, which share cache lines of the count matrix and of
+// each bucket's output, meet in the same L2.  A bijection on [0, n).
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+  const uint32_t x = b % kXcds, i = b / kXcds, per = n / kXcds, rem = n % kXcds;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
   return (1ull << lane_id()) - 1ull;
 }

@@ -181,7 +181,8 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const Bi
   __shared__ uint64_t s_mn[kTileThreads / kWave], s_mx[kTileThreads / kWave];
   __shared__ uint8_t s_present[256];
   if (threadIdx.x < 256) s_present[threadIdx.x] = 0;
-  const Tile t = p.tiles[blockIdx.x];
+  const uint32_t ti = xcd_tile(blockIdx.x, gridDim.x);
+  const Tile t = p.tiles[ti];
   const BigSeg sg = segs[t.seg];
   const uint32_t* __restrict__ src = p.idx[sg.buf];
   uint32_t o[kTileItems];
@@ -221,7 +222,7 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const Bi
   if (threadIdx.x < 256) {
     const uint64_t bal = __ballot(s_present[threadIdx.x] != 0);
     if (lane_id() == 0) {
-      uint32_t* ta = p.tile_alpha + 8ull * blockIdx.x + threadIdx.x / 32;
+      uint32_t* ta = p.tile_alpha + 8ull * ti + threadIdx.x / 32;
       ta[0] = (uint32_t)bal;
       ta[1] = (uint32_t)(bal >> 32);
     }
@@ -231,26 +232,35 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const Bi
       mn = s_mn[w] < mn ? s_mn[w] : mn;
       mx = s_mx[w] > mx ? s_mx[w] : mx;
     }
-    p.tile_mm[2ull * blockIdx.x] = mn;
-    p.tile_mm[2ull * blockIdx.x + 1] = mx;
+    p.tile_mm[2ull * ti] = mn;
+    p.tile_mm[2ull * ti + 1] = mx;
   }
 }
 
-// One workgroup per segment: min/max key and byte set over its tiles.
-__global__ __launch_bounds__(256) void k_segreduce(Plan p) {
-  __shared__ uint64_t s_mn[256 / kWave], s_mx[256 / kWave];
+// One workgroup per segment: min/max key and byte set over its tiles.  A
+// first pass's single segment has thousands of tiles: 1024 threads with
+// unrolled loads keep its latency down.
+constexpr int kSegReduceThreads = 1024;
+
+__global__ __launch_bounds__(kSegReduceThreads) void k_segreduce(Plan p) {
+  constexpr int T = kSegReduceThreads;
+  __shared__ uint64_t s_mn[T / kWave], s_mx[T / kWave];
   __shared__ uint32_t s_al[8];
   const SegAux ax = p.aux[blockIdx.x];
   if (threadIdx.x < 8) s_al[threadIdx.x] = 0;
   uint64_t mn = ~0ull, mx = 0ull;
   uint32_t al = 0;
-  for (uint32_t t = threadIdx.x; t < ax.ntiles; t += 256) {
-    const uint64_t a = p.tile_mm[2ull * (ax.tile0 + t)], b = p.tile_mm[2ull * (ax.tile0 + t) + 1];
+  const uint64_t* __restrict__ mm = p.tile_mm + 2ull * ax.tile0;
+#pragma unroll 4
+  for (uint32_t t = threadIdx.x; t < ax.ntiles; t += T) {
+    const uint64_t a = mm[2ull * t], b = mm[2ull * t + 1];
     mn = a < mn ? a : mn;
     mx = b > mx ? b : mx;
   }
-  // Byte sets: thread (t, w) ORs word w of tiles t, t+32, ...
-  for (uint32_t i = threadIdx.x; i < 8 * ax.ntiles; i += 256) al |= p.tile_alpha[8ull * ax.tile0 + i];
+  // Byte sets: thread i ORs word i & 7 of tiles i / 8, i / 8 + T / 8, ...
+  const uint32_t* __restrict__ ta = p.tile_alpha + 8ull * ax.tile0;
+#pragma unroll 8
+  for (uint32_t i = threadIdx.x; i < 8 * ax.ntiles; i += T) al |= ta[i];
   mn = wave_min64(mn);
   mx = wave_max64(mx);
   const int wid = threadIdx.x / kWave;
@@ -259,11 +269,12 @@ __global__ __launch_bounds__(256) void k_segreduce(Plan p) {
     s_mx[wid] = mx;
   }
   __syncthreads();
-  if (al) atomicOr(&s_al[threadIdx.x & 7], al);  // 256 % 8 == 0: word index = i & 7
+  static_assert(T % 8 == 0, "word index of i is i & 7");
+  if (al) atomicOr(&s_al[threadIdx.x & 7], al);
   __syncthreads();
   if (threadIdx.x < 8) p.seg_alpha[8ull * blockIdx.x + threadIdx.x] = s_al[threadIdx.x];
   if (threadIdx.x == 0) {
-    for (int w = 1; w < 256 / kWave; ++w) {
+    for (int w = 1; w < T / kWave; ++w) {
       mn = s_mn[w] < mn ? s_mn[w] : mn;
       mx = s_mx[w] > mx ? s_mx[w] : mx;
     }
@@ -319,7 +330,7 @@ __global__ __launch_bounds__(kTileThreads) void k_histogram(Plan p) {
   constexpr uint32_t NB = 1u << RB;
   __shared__ uint32_t h[NB];
   __shared__ uint8_t rank[256];
-  const Tile t = p.tiles[blockIdx.x];
+  const Tile t = p.tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const uint64_t mn = p.segmm[2 * t.seg], mx = p.segmm[2 * t.seg + 1];
   if (mn == mx) return;
   const DigitSpec ds = digit_spec<RB>(mn, mx, build_rank(p.alpha, rank));
@@ -624,7 +635,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
   __shared__ uint32_t s_scan[W + 1];
   __shared__ uint8_t rank[256];
 
-  const Tile t = p.tiles[blockIdx.x];
+  const Tile t = p.tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const BigSeg sg = segs[t.seg];
   const uint64_t mn = p.segmm[2 * t.seg], mx = p.segmm[2 * t.seg + 1];
   if (mn == mx) return;  // whole segment stays where it is
@@ -766,7 +777,7 @@ static inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 4096)
 }
 
 hipError_t launch_segreduce(const Plan& p, uint32_t nsegs, hipStream_t s) {
-  k_segreduce<<<nsegs, 256, 0, s>>>(p);
+  k_segreduce<<<nsegs, kSegReduceThreads, 0, s>>>(p);
   return hipGetLastError();
 }
 
