@@ -388,25 +388,54 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(Plan p) {
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(Plan p) {
+  static_assert(kScanItems % 4 == 0, "whole 16-byte groups per thread");
+  constexpr int Q = kScanItems / 4;  // 16-byte groups per thread
   __shared__ uint32_t s_scan[kScanThreads / kWave + 1];
+  __shared__ __attribute__((aligned(16))) uint32_t s_v[kScanBlock];
   const uint32_t len = p.ctr->flat_len;
   const uint64_t b0 = (uint64_t)blockIdx.x * kScanBlock;
   if (b0 >= len) return;
-  // Thread t owns kScanItems consecutive entries.
-  const uint64_t first = b0 + (uint64_t)threadIdx.x * kScanItems;
-  uint32_t v[kScanItems];
+  // The block's entries go through LDS so that global loads and stores are
+  // 16-byte and coalesced while each thread scans consecutive entries.
+  const bool whole = b0 + kScanBlock <= len;
+  uint4* const v4 = reinterpret_cast<uint4*>(s_v);
+  if (whole) {
+    const uint4* __restrict__ g4 = reinterpret_cast<const uint4*>(p.flat + b0);
+#pragma unroll
+    for (int r = 0; r < Q; ++r) v4[r * kScanThreads + threadIdx.x] = g4[r * kScanThreads + threadIdx.x];
+  } else {
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kScanBlock; i += kScanThreads)
+      s_v[i] = b0 + i < len ? p.flat[b0 + i] : 0u;
+  }
+  __syncthreads();
+  // Thread t owns entries [t * kScanItems, (t + 1) * kScanItems).
+  uint4 q[Q];
   uint32_t sum = 0;
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
-    v[j] = first + j < len ? p.flat[first + j] : 0u;
-    sum += v[j];
+  for (int r = 0; r < Q; ++r) {
+    q[r] = v4[threadIdx.x * Q + r];
+    sum += q[r].x + q[r].y + q[r].z + q[r].w;
   }
   uint32_t total;
   uint32_t run = block_excl_scan<kScanThreads>(sum, s_scan, &total) + p.flat_blk[blockIdx.x];
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
-    if (first + j < len) p.flat[first + j] = run;
-    run += v[j];
+  for (int r = 0; r < Q; ++r) {
+    uint4 o;
+    o.x = run;
+    o.y = o.x + q[r].x;
+    o.z = o.y + q[r].y;
+    o.w = o.z + q[r].z;
+    run = o.w + q[r].w;
+    v4[threadIdx.x * Q + r] = o;
+  }
+  __syncthreads();
+  if (whole) {
+    uint4* __restrict__ g4 = reinterpret_cast<uint4*>(p.flat + b0);
+#pragma unroll
+    for (int r = 0; r < Q; ++r) g4[r * kScanThreads + threadIdx.x] = v4[r * kScanThreads + threadIdx.x];
+  } else {
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kScanBlock; i += kScanThreads)
+      if (b0 + i < len) p.flat[b0 + i] = s_v[i];
   }
 }
 
