@@ -90,12 +90,9 @@ __device__ __forceinline__ void local_radix_pass(LocalSmem<BLOCK, CAP>& s, uint3
   const int lane = lane_id();
   const uint64_t lt = lanemask_lt();
 
-  // 256 * W counters, 4 per thread (W = BLOCK / 64).
-  static_assert(256 * W == 4 * BLOCK, "four digit counters per thread");
-  uint64_t* const hist4 = reinterpret_cast<uint64_t*>(&s.whist[0][0]);
-  hist4[threadIdx.x] = 0;
-  __syncthreads();
-
+  // The items are read before the counters are cleared, so the LDS reads
+  // overlap the barrier; the ranking below is then one chain of counter
+  // reads and leader writes per item.
   uint64_t k[IPT];
   uint32_t o[IPT], sd[IPT], rk[IPT];
 #pragma unroll
@@ -104,17 +101,26 @@ __device__ __forceinline__ void local_radix_pass(LocalSmem<BLOCK, CAP>& s, uint3
     o[j] = 0;
     sd[j] = 0;
     rk[j] = 0;
+    const uint32_t i = base + j * kWave;
+    if ((uint32_t)j < jn && i < nact) {
+      k[j] = s.key[i];
+      o[j] = s.off[i];
+      sd[j] = s.sid[i];
+    }
+  }
+
+  // 256 * W counters, 4 per thread (W = BLOCK / 64).
+  static_assert(256 * W == 4 * BLOCK, "four digit counters per thread");
+  uint64_t* const hist4 = reinterpret_cast<uint64_t*>(&s.whist[0][0]);
+  hist4[threadIdx.x] = 0;
+  __syncthreads();
+
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
     if ((uint32_t)j < jn) {
-      const uint32_t i = base + j * kWave;
-      const bool v = i < nact;
-      uint32_t d = 0;
-      if (v) {
-        k[j] = s.key[i];
-        o[j] = s.off[i];
-        sd[j] = s.sid[i];
-        d = which < 8 ? (uint32_t)(k[j] >> (8 * which)) & 0xFFu
-                      : (which == 8 ? sd[j] & 0xFFu : sd[j] >> 8);
-      }
+      const bool v = base + j * kWave < nact;
+      const uint32_t d = which < 8 ? (uint32_t)(k[j] >> (8 * which)) & 0xFFu
+                                   : (which == 8 ? sd[j] & 0xFFu : sd[j] >> 8);
       const uint64_t m = match_digit<8>(d, __ballot(v));
       if (v) {
         const uint32_t b = s.whist[d][wid];
