@@ -49,6 +49,10 @@ struct SegAux {
   uint32_t tile0;  // index of the segment's first tile in the pass
 };
 
+// Kinds of a bucket in Plan::bstart: it goes to the next global pass, its
+// order is final, or a workgroup finishes it.
+constexpr uint32_t kBucketNext = 0, kBucketFinal = 1, kBucketLocal = 2;
+
 // Entries per block of the flat exclusive scan.
 constexpr uint32_t kScanBlock = 4096;
 
@@ -112,10 +116,18 @@ struct Plan {
   uint32_t* idx[3];      // index buffers: [0] = input, [1], [2] ping-pong
   uint32_t* out;         // sorted offsets
   uint64_t* lcp;         // NULL or n-1 LCPs
-  uint64_t* keys;        // n cached keys (global passes)
+  uint64_t* keys;        // n keys of the current global pass
+  // A pass that reads its keys from the text also keeps the 8 bytes after
+  // each key (`ext`) and its scatter writes every string bound for the next
+  // global pass its key there (`keys_next`, at the string's new position):
+  // the next pass then takes its keys as they lie (keys_carried != 0) and
+  // reads no text.  Passes alternate between the two kinds.
+  uint64_t* keys_next;
+  uint64_t* ext;
+  uint32_t keys_carried;
   BigSeg* big[2];        // BigSeg lists (current, next)
   uint64_t* segmm;       // per BigSeg: min, max key
-  uint32_t* bstart;      // per BigSeg and digit: 1 when the bucket is final
+  uint32_t* bstart;      // per BigSeg and digit: kBucket* kind of the bucket
   uint32_t* alpha;       // 256-bit set of byte values seen in the pass's keys
   Tile* tiles;
   SegAux* aux;           // per BigSeg: layout of its counts in `flat`

@@ -187,23 +187,45 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const Bi
   const uint32_t* __restrict__ src = p.idx[sg.buf];
   uint32_t o[kTileItems];
   uint64_t k[kTileItems];
-#pragma unroll
-  for (int j = 0; j < kTileItems; ++j) {
-    const uint32_t i = j * kTileThreads + threadIdx.x;
-    o[j] = i < t.len ? src[t.begin + i] : 0u;
-  }
   uint64_t mn = ~0ull, mx = 0ull;
-  // Unconditional (out-of-tile slots read offset 0) so all loads issue
-  // before the first wait.
+  if (p.keys_carried) {
+    // The previous pass left these keys at the strings' positions.
 #pragma unroll
-  for (int j = 0; j < kTileItems; ++j)
-    k[j] = load_key(p.text64, p.nwords, (uint64_t)o[j] + sg.depth);
+    for (int j = 0; j < kTileItems; ++j) {
+      const uint32_t i = j * kTileThreads + threadIdx.x;
+      k[j] = i < t.len ? p.keys[t.begin + i] : 0ull;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kTileItems; ++j) {
+      const uint32_t i = j * kTileThreads + threadIdx.x;
+      o[j] = i < t.len ? src[t.begin + i] : 0u;
+    }
+    // Unconditional (out-of-tile slots read offset 0) so all loads issue
+    // before the first wait.  16 bytes: the key and the 8 bytes after it,
+    // from which the scatter forms the next pass's keys.
+    uint64_t x[kTileItems];
+#pragma unroll
+    for (int j = 0; j < kTileItems; ++j) {
+      uint64_t kk[2];
+      load_keys<2>(p.text64, p.nwords, (uint64_t)o[j] + sg.depth, kk);
+      k[j] = kk[0];
+      x[j] = kk[1];
+    }
+#pragma unroll
+    for (int j = 0; j < kTileItems; ++j) {
+      const uint32_t i = j * kTileThreads + threadIdx.x;
+      if (i < t.len) {
+        p.keys[t.begin + i] = k[j];
+        p.ext[t.begin + i] = x[j];
+      }
+    }
+  }
   __syncthreads();  // s_present cleared
 #pragma unroll
   for (int j = 0; j < kTileItems; ++j) {
     const uint32_t i = j * kTileThreads + threadIdx.x;
     if (i < t.len) {
-      p.keys[t.begin + i] = k[j];
       mn = k[j] < mn ? k[j] : mn;
       mx = k[j] > mx ? k[j] : mx;
 #pragma unroll
@@ -559,7 +581,7 @@ __global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, Big
       }
     }
     const bool fin = ended || c == 1;
-    bs[d] = fin ? 1u : 0u;
+    bs[d] = fin ? kBucketFinal : (c <= p.local_threshold ? kBucketLocal : kBucketNext);
     if (d != dfirst) ++nb;
     if (ended) {
       if (c > 1) {
@@ -667,9 +689,18 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
   const Tile t = p.tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const BigSeg sg = segs[t.seg];
   const uint64_t mn = p.segmm[2 * t.seg], mx = p.segmm[2 * t.seg + 1];
-  if (mn == mx) return;  // whole segment stays where it is
+  const bool carry = !p.keys_carried;  // hand the next pass its keys
+  if (mn == mx) {
+    // The whole segment stays where it is and, unless its strings ended,
+    // goes 8 bytes deeper: its next keys are the 8 bytes after these.
+    if (carry && (mn & 0xFFu) != 0)
+      for (uint32_t i = threadIdx.x; i < t.len; i += kTileThreads)
+        p.keys_next[t.begin + i] = p.ext[t.begin + i];
+    return;
+  }
   const DigitSpec ds = digit_spec<RB>(mn, mx, build_rank(p.alpha, rank));
   const uint32_t nbits = digit_bits(ds.nbu);
+  const uint32_t fb = ds.fb0 + ds.k;  // bytes the children share: their keys start there
   const uint32_t* __restrict__ src = p.idx[sg.buf];
   uint32_t* __restrict__ dst = p.idx[sg.buf == 1 ? 2 : 1];
 
@@ -682,15 +713,22 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
   const int lane = lane_id();
   const uint64_t lt = lanemask_lt();
   uint32_t off[kTileItems], dig[kTileItems], rnk[kTileItems];
+  uint64_t nk[kTileItems];  // key of the string at its child's depth (carry)
 #pragma unroll
   for (int j = 0; j < kTileItems; ++j) {
     const uint32_t i = wid * IPW + j * kWave + lane;
     const bool v = i < t.len;
     uint32_t d = 0;
     off[j] = 0;
+    nk[j] = 0;
     if (v) {
       off[j] = src[t.begin + i];
-      d = digit_of(p.keys[t.begin + i], ds, rank);
+      const uint64_t key = p.keys[t.begin + i];
+      d = digit_of(key, ds, rank);
+      if (carry) {
+        const uint64_t x = p.ext[t.begin + i];
+        nk[j] = fb >= 8 ? x : (key << (8 * fb)) | (x >> (64 - 8 * fb));
+      }
     }
     dig[j] = d;
     const uint64_t m = match_digit_n(d, __ballot(v), nbits);
@@ -741,18 +779,23 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
     run += tot[q];
   }
   __syncthreads();
+  const uint32_t* bs = p.bstart + (uint64_t)t.seg * NB;
 #pragma unroll
   for (int j = 0; j < kTileItems; ++j) {
     const uint32_t i = wid * IPW + j * kWave + lane;
     if (i < t.len) {
       const uint32_t d = dig[j];
-      const uint32_t lp = s_tstart[d] + s_whist[d][wid] + rnk[j];
+      const uint32_t within = s_whist[d][wid] + rnk[j];  // place among the tile's digit d
+      const uint32_t lp = s_tstart[d] + within;
       s_off[lp] = off[j];
       s_dig[lp] = (uint16_t)d;
+      // Keys for the next pass go straight to the strings' new positions
+      // (only for buckets that take one).
+      const uint32_t pos = s_gbase[d] + within;
+      if (carry && bs[d] == kBucketNext && pos - sg.begin < sg.size) p.keys_next[pos] = nk[j];
     }
   }
   __syncthreads();
-  const uint32_t* bs = p.bstart + (uint64_t)t.seg * NB;
   for (uint32_t i = threadIdx.x; i < t.len; i += kTileThreads) {
     const uint32_t d = s_dig[i];
     const uint32_t pos = s_gbase[d] + (i - s_tstart[d]);
@@ -762,7 +805,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
       continue;
     }
     dst[pos] = o;
-    if (bs[d]) p.out[pos] = o;
+    if (bs[d] == kBucketFinal) p.out[pos] = o;
   }
 }
 

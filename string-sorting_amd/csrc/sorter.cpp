@@ -239,6 +239,8 @@ std::vector<Item> workspace_items(Plan& p, uint32_t** idx_copy, const Caps& k, u
       {(void**)&p.idx[2], 4 * k.ncap},
       {(void**)idx_copy, 4 * k.ncap},
       {(void**)&p.keys, 8 * k.ncap},
+      {(void**)&p.keys_next, 8 * k.ncap},
+      {(void**)&p.ext, 8 * k.ncap},
       {(void**)&p.ctr, sizeof(Counters)},  // ctr, big[0], big[1]: read back together
       {(void**)&p.big[0], sizeof(BigSeg) * k.big},
       {(void**)&p.big[1], sizeof(BigSeg) * k.big},
@@ -492,6 +494,11 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
       static const char* kPassNames[10] = {"pass0", "pass1", "pass2", "pass3", "pass4",
                                            "pass5", "pass6", "pass7", "pass8", "pass9+"};
       ProfToken pass_tok = prof_begin(c, kPassNames[pass < 9 ? pass : 9], active, st);
+      // Passes alternate: one reads its keys (and the next 8 bytes) from the
+      // text and hands the following pass its keys, which that pass takes
+      // as they lie.
+      p.keys_carried = pass % 2;
+      if (p.keys_carried) std::swap(p.keys, p.keys_next);
       GSS_LAUNCH("gather", active, launch_gather_minmax(p, segs, ntiles, st));
       GSS_LAUNCH("segreduce", nbig, launch_segreduce(p, nbig, st));
       GSS_LAUNCH("layout", nbig, launch_layout(p, nbig, rb, st));
