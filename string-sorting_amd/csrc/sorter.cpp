@@ -45,6 +45,10 @@ struct Workspace {
   size_t up_tiles_off = 0;  // byte offset of the tiles behind the aux table
   char* h_rb = nullptr;
   size_t rb_bytes = 0;
+  // The start of a sort: zeroed local statistics, the counters and the
+  // initial segment list go up in one copy (adjacent on the device).
+  char* h_init = nullptr;
+  size_t init_ctr_off = 0, init_big_off = 0;
 };
 
 struct ProfTotal {
@@ -196,6 +200,8 @@ void free_workspace(Workspace& w) {
   if (w.h_up) (void)hipHostFree(w.h_up);
   if (w.h_rb) (void)hipHostFree(w.h_rb);
   if (w.h_lstats) (void)hipHostFree(w.h_lstats);
+  if (w.h_init) (void)hipHostFree(w.h_init);
+  w.h_init = nullptr;
   w.base = nullptr;
   w.h_big = nullptr;
   w.h_up = nullptr;
@@ -241,7 +247,10 @@ std::vector<Item> workspace_items(Plan& p, uint32_t** idx_copy, const Caps& k, u
       {(void**)&p.keys, 8 * k.ncap},
       {(void**)&p.keys_next, 8 * k.ncap},
       {(void**)&p.ext, 8 * k.ncap},
-      {(void**)&p.ctr, sizeof(Counters)},  // ctr, big[0], big[1]: read back together
+      // lstats, ctr, big[0]: uploaded together at the start of a sort;
+      // ctr, big[0], big[1]: read back together after each pass.
+      {(void**)&p.lstats, 2 * sizeof(LocalStats)},  // [0]: main stream, [1]: side stream
+      {(void**)&p.ctr, sizeof(Counters)},
       {(void**)&p.big[0], sizeof(BigSeg) * k.big},
       {(void**)&p.big[1], sizeof(BigSeg) * k.big},
       {(void**)&p.segmm, 16 * k.big},
@@ -256,7 +265,6 @@ std::vector<Item> workspace_items(Plan& p, uint32_t** idx_copy, const Caps& k, u
       {(void**)&p.local[2], sizeof(LocalSeg) * k.local[2]},
       {(void**)&p.retire, sizeof(Retire) * k.retire},
       {(void**)&p.bounds, sizeof(Boundary) * k.bounds},
-      {(void**)&p.lstats, 2 * sizeof(LocalStats)},  // [0]: main stream, [1]: side stream
       {(void**)&p.tile_mm, 16 * k.tiles},
       {(void**)&p.tile_alpha, 32 * k.tiles},
       {(void**)&p.seg_alpha, 32 * k.big},
@@ -315,6 +323,11 @@ int ensure_workspace(Context& c, uint64_t n, uint32_t rb, uint32_t t) {
                sizeof(BigSeg) * cap_big;
   GSS_HIP(hipHostMalloc((void**)&w.h_rb, w.rb_bytes, hipHostMallocDefault));
   GSS_HIP(hipHostMalloc((void**)&w.h_lstats, 2 * sizeof(LocalStats), hipHostMallocDefault));
+  w.init_ctr_off = (size_t)(reinterpret_cast<char*>(p.ctr) - reinterpret_cast<char*>(p.lstats));
+  w.init_big_off = (size_t)(reinterpret_cast<char*>(p.big[0]) - reinterpret_cast<char*>(p.lstats));
+  const size_t init_bytes = w.init_big_off + sizeof(BigSeg) * cap_big;
+  GSS_HIP(hipHostMalloc((void**)&w.h_init, init_bytes, hipHostMallocDefault));
+  memset(w.h_init, 0, init_bytes);  // the statistics part stays zero
   return GSS_OK;
 }
 
@@ -420,11 +433,11 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   if (h0.retire)
     GSS_HIP(hipMemcpyAsync(p.retire, init_retire, sizeof(Retire) * h0.retire,
                            hipMemcpyHostToDevice, st));
-  if (nbig)
-    GSS_HIP(hipMemcpyAsync(p.big[0], w.h_big, sizeof(BigSeg) * nbig, hipMemcpyHostToDevice, st));
-  GSS_HIP(hipMemcpyAsync(p.ctr, &h0, sizeof(Counters), hipMemcpyHostToDevice, st));
-  GSS_HIP(launch_fill_u64(reinterpret_cast<uint64_t*>(p.lstats), 2 * sizeof(LocalStats) / 8, 0,
-                          st));
+  // Zeroed statistics, the counters and the big segments in one copy.
+  memcpy(w.h_init + w.init_ctr_off, &h0, sizeof(Counters));
+  memcpy(w.h_init + w.init_big_off, w.h_big, sizeof(BigSeg) * nbig);
+  GSS_HIP(hipMemcpyAsync(p.lstats, w.h_init, w.init_big_off + sizeof(BigSeg) * nbig,
+                         hipMemcpyHostToDevice, st));
   // Local segments launched early run on the side stream with their own
   // statistics block; the side stream first waits for the uploads above.
   Plan p_side = p;
