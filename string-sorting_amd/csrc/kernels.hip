@@ -356,12 +356,20 @@ __global__ __launch_bounds__(kTileThreads) void k_histogram(Plan p) {
   const uint64_t mn = p.segmm[2 * t.seg], mx = p.segmm[2 * t.seg + 1];
   if (mn == mx) return;
   const DigitSpec ds = digit_spec<RB>(mn, mx, build_rank(p.alpha, rank));
+  // All keys are loaded before the first is used (slots past the tile read
+  // its last key and are not counted), so the loads overlap.
+  uint64_t key[kTileItems];
+#pragma unroll
+  for (int j = 0; j < kTileItems; ++j) {
+    const uint32_t i = j * kTileThreads + threadIdx.x;
+    key[j] = p.keys[t.begin + (i < t.len ? i : t.len - 1)];
+  }
   for (uint32_t d = threadIdx.x; d < ds.nbu; d += kTileThreads) h[d] = 0;
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < kTileItems; ++j) {
     const uint32_t i = j * kTileThreads + threadIdx.x;
-    if (i < t.len) atomicAdd(&h[digit_of(p.keys[t.begin + i], ds, rank)], 1u);
+    if (i < t.len) atomicAdd(&h[digit_of(key[j], ds, rank)], 1u);
   }
   __syncthreads();
   const SegAux ax = p.aux[t.seg];
@@ -528,12 +536,16 @@ __global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, Big
   const uint32_t fb = ds.fb0 + ds.k;  // bytes every child shares in full
   const SegAux ax = p.aux[si];
   const uint32_t* col0 = p.flat + ax.flat_base;
-  const uint32_t base0 = col0[0];
   // Bucket of digit d: [start(d), start(d+1)), start(nbu) = segment size.
-  auto start = [&](uint32_t d) -> uint32_t {
-    return d < ds.nbu ? col0[(uint64_t)d * ax.ntiles] - base0 : sg.size;
-  };
-
+  // The DPT + 1 starts a thread needs are loaded before any is used (clamped
+  // to a valid entry, so the strided loads overlap).
+  uint32_t sv[DPT + 1];
+  const uint32_t base0 = col0[0];
+#pragma unroll
+  for (uint32_t q = 0; q <= DPT; ++q) {
+    const uint32_t d = threadIdx.x * DPT + q;
+    sv[q] = col0[(uint64_t)(d < ds.nbu ? d : ds.nbu - 1) * ax.ntiles];
+  }
   uint32_t st[DPT], cnt[DPT];
   uint32_t firstd = NB;
 #pragma unroll
@@ -542,8 +554,8 @@ __global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, Big
     st[q] = 0;
     cnt[q] = 0;
     if (d < ds.nbu) {
-      st[q] = start(d);
-      cnt[q] = start(d + 1) - st[q];
+      st[q] = sv[q] - base0;
+      cnt[q] = (d + 1 < ds.nbu ? sv[q + 1] - base0 : sg.size) - st[q];
       if (cnt[q] && firstd == NB) firstd = d;
     }
   }
@@ -681,6 +693,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
   __shared__ __attribute__((aligned(16))) uint16_t s_whist[NB][W];  // digit-major
   __shared__ uint32_t s_tstart[NB];
   __shared__ uint32_t s_gbase[NB];
+  __shared__ uint8_t s_kind[NB];  // kBucket* kind of each digit's bucket
   __shared__ uint32_t s_off[kTile];
   __shared__ uint16_t s_dig[kTile];
   __shared__ uint32_t s_scan[W + 1];
@@ -714,22 +727,27 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
   const uint64_t lt = lanemask_lt();
   uint32_t off[kTileItems], dig[kTileItems], rnk[kTileItems];
   uint64_t nk[kTileItems];  // key of the string at its child's depth (carry)
+  // Every item's loads are issued before any is used (slots past the tile
+  // read the tile's last string and are masked out), so they overlap.
+  uint64_t key[kTileItems];
+#pragma unroll
+  for (int j = 0; j < kTileItems; ++j) {
+    const uint32_t i = wid * IPW + j * kWave + lane;
+    const uint32_t at = t.begin + (i < t.len ? i : t.len - 1);
+    off[j] = src[at];
+    key[j] = p.keys[at];
+    nk[j] = carry ? p.ext[at] : 0ull;
+  }
+#pragma unroll
+  for (int j = 0; j < kTileItems; ++j) {
+    dig[j] = digit_of(key[j], ds, rank);
+    if (carry) nk[j] = fb >= 8 ? nk[j] : (key[j] << (8 * fb)) | (nk[j] >> (64 - 8 * fb));
+  }
 #pragma unroll
   for (int j = 0; j < kTileItems; ++j) {
     const uint32_t i = wid * IPW + j * kWave + lane;
     const bool v = i < t.len;
-    uint32_t d = 0;
-    off[j] = 0;
-    nk[j] = 0;
-    if (v) {
-      off[j] = src[t.begin + i];
-      const uint64_t key = p.keys[t.begin + i];
-      d = digit_of(key, ds, rank);
-      if (carry) {
-        const uint64_t x = p.ext[t.begin + i];
-        nk[j] = fb >= 8 ? x : (key << (8 * fb)) | (x >> (64 - 8 * fb));
-      }
-    }
+    const uint32_t d = v ? dig[j] : 0u;
     dig[j] = d;
     const uint64_t m = match_digit_n(d, __ballot(v), nbits);
     rnk[j] = 0;
@@ -740,6 +758,21 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
     }
   }
   __syncthreads();
+
+  // This tile's place in each digit's bucket and the bucket's kind, loaded
+  // before the scan below so that the strided reads overlap it.
+  const SegAux ax = p.aux[t.seg];
+  const uint32_t* col = p.flat + ax.flat_base;
+  const uint32_t* bs = p.bstart + (uint64_t)t.seg * NB;
+  const uint32_t base0 = col[0];
+  uint32_t gcol[DPT], gkind[DPT];
+#pragma unroll
+  for (uint32_t q = 0; q < DPT; ++q) {
+    const uint32_t d = threadIdx.x * DPT + q;
+    const uint32_t dc = d < ds.nbu ? d : ds.nbu - 1;
+    gcol[q] = col[(uint64_t)dc * ax.ntiles + t.tidx];
+    gkind[q] = bs[dc];
+  }
 
   // Per digit: exclusive prefix over waves (in place) and tile total; a
   // digit's eight wave counters are one 16-byte row of LDS.
@@ -766,20 +799,17 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
   }
   uint32_t tsum;
   uint32_t run = block_excl_scan<kTileThreads>(sum, s_scan, &tsum);
-  const SegAux ax = p.aux[t.seg];
-  const uint32_t* col = p.flat + ax.flat_base;
-  const uint32_t base0 = col[0];
 #pragma unroll
   for (uint32_t q = 0; q < DPT; ++q) {
     const uint32_t d = threadIdx.x * DPT + q;
     if (d < ds.nbu) {
       s_tstart[d] = run;
-      s_gbase[d] = sg.begin + (col[(uint64_t)d * ax.ntiles + t.tidx] - base0);
+      s_gbase[d] = sg.begin + (gcol[q] - base0);
+      s_kind[d] = (uint8_t)gkind[q];
     }
     run += tot[q];
   }
   __syncthreads();
-  const uint32_t* bs = p.bstart + (uint64_t)t.seg * NB;
 #pragma unroll
   for (int j = 0; j < kTileItems; ++j) {
     const uint32_t i = wid * IPW + j * kWave + lane;
@@ -792,7 +822,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
       // Keys for the next pass go straight to the strings' new positions
       // (only for buckets that take one).
       const uint32_t pos = s_gbase[d] + within;
-      if (carry && bs[d] == kBucketNext && pos - sg.begin < sg.size) p.keys_next[pos] = nk[j];
+      if (carry && s_kind[d] == kBucketNext && pos - sg.begin < sg.size) p.keys_next[pos] = nk[j];
     }
   }
   __syncthreads();
@@ -805,7 +835,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
       continue;
     }
     dst[pos] = o;
-    if (bs[d] == kBucketFinal) p.out[pos] = o;
+    if (s_kind[d] == kBucketFinal) p.out[pos] = o;
   }
 }
 
