@@ -274,7 +274,29 @@ __global__ __launch_bounds__(256) void k_block_owner(const uint64_t* __restrict_
   }
 }
 
-// One thread per 16 output bytes.
+// 128-bit helpers on (lo, hi) little-endian byte pairs.
+struct U128 {
+  uint64_t lo, hi;
+};
+
+// Shifted towards higher byte positions by k bytes (0 <= k < 16).
+__device__ __forceinline__ U128 shl_bytes(U128 v, uint32_t k) {
+  if (k == 0) return v;
+  if (k >= 8) return U128{0ull, v.lo << (8 * (k - 8))};
+  return U128{v.lo << (8 * k), (v.hi << (8 * k)) | (v.lo >> (64 - 8 * k))};
+}
+
+// Bytes [0, k) set (0 <= k <= 16).
+__device__ __forceinline__ U128 low_bytes(uint32_t k) {
+  if (k >= 16) return U128{~0ull, ~0ull};
+  if (k >= 8) return U128{~0ull, k == 8 ? 0ull : (~0ull >> (64 - 8 * (k - 8)))};
+  return U128{k == 0 ? 0ull : (~0ull >> (64 - 8 * k)), 0ull};
+}
+
+// One thread per 16 output bytes.  Each string that reaches into the block
+// contributes one 16-byte text window, shifted to its place and masked to
+// its bytes; the newline after it is ORed in (the text byte there is the
+// NUL, or lies past the text, and is masked out).
 __global__ __launch_bounds__(256) void k_write(const uint64_t* __restrict__ t64, uint64_t len,
                                               const uint32_t* __restrict__ off,
                                               const uint64_t* __restrict__ pos, uint64_t n,
@@ -285,34 +307,40 @@ __global__ __launch_bounds__(256) void k_write(const uint64_t* __restrict__ t64,
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nblk; b += stride) {
     const uint64_t p0 = b * 16, p1 = p0 + 16 < total ? p0 + 16 : total;
-    uint64_t acc[2] = {0, 0};  // little-endian output bytes
+    U128 acc{0ull, 0ull};  // little-endian output bytes
     uint64_t s = first[b];
     uint64_t p = p0;
     while (p < p1 && s < n) {
       const uint64_t ps = pos[s], pe = pos[s + 1];  // string s: [ps, pe-1), '\n' at pe-1
       const uint64_t hi = pe < p1 ? pe : p1;
+      const uint64_t tend = pe - 1 < hi ? pe - 1 : hi;  // end of its text bytes here
       // Up to 16 string bytes starting at byte (p - ps) of string s.
       const uint64_t src = (uint64_t)off[s] + (p - ps);
       uint64_t raw[3];
       text_words<3>(t64, nwords, src >> 3, raw);
       const uint32_t sh = (uint32_t)(src & 7) * 8;
-      const uint64_t v0 = funnel(raw[0], raw[1], sh), v1 = funnel(raw[1], raw[2], sh);
-      for (uint64_t q = p; q < hi; ++q) {
-        const uint32_t k = (uint32_t)(q - p);  // byte of (v0, v1)
-        const uint64_t byte = q + 1 == pe ? (uint64_t)'\n'
-                                          : (((k < 8 ? v0 : v1) >> (8 * (k & 7))) & 0xFFull);
-        const uint32_t d = (uint32_t)(q - p0);
-        acc[d >> 3] |= byte << (8 * (d & 7));
+      const U128 v{funnel(raw[0], raw[1], sh), funnel(raw[1], raw[2], sh)};
+      const uint32_t a = (uint32_t)(p - p0);  // first output byte of this string here
+      const U128 w = shl_bytes(v, a);
+      const U128 m = low_bytes((uint32_t)(tend - p0)), m0 = low_bytes(a);
+      acc.lo |= w.lo & m.lo & ~m0.lo;
+      acc.hi |= w.hi & m.hi & ~m0.hi;
+      if (pe - 1 < p1) {  // the newline falls in this block
+        const uint32_t d = (uint32_t)(pe - 1 - p0);
+        if (d < 8)
+          acc.lo |= (uint64_t)'\n' << (8 * d);
+        else
+          acc.hi |= (uint64_t)'\n' << (8 * (d - 8));
       }
       p = hi;
       ++s;
     }
     if (p1 - p0 == 16) {
-      *reinterpret_cast<TextPair*>(out + p0) = TextPair{acc[0], acc[1]};
+      *reinterpret_cast<TextPair*>(out + p0) = TextPair{acc.lo, acc.hi};
     } else {
       for (uint64_t q = p0; q < p1; ++q) {
         const uint32_t d = (uint32_t)(q - p0);
-        out[q] = (unsigned char)(acc[d >> 3] >> (8 * (d & 7)));
+        out[q] = (unsigned char)((d < 8 ? acc.lo : acc.hi) >> (8 * (d & 7)));
       }
     }
   }
