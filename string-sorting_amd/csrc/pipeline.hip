@@ -407,20 +407,26 @@ __global__ __launch_bounds__(256) void k_merge_partition(const uint64_t* __restr
 // LCP merge state: s = last output, ha = lcp(s, A[i]), hb = lcp(s, B[j]).
 // ha > hb means A[i] < B[j] (and lcp(A[i], B[j]) = hb), ha < hb the
 // reverse; only equal values need a comparison, starting at that depth.
+//
+// The block's outputs are A[i0, i1) and B[j0, j1); in sorted order any two
+// of them share D = lcp(first, last) bytes.  Their offsets and their 8-byte
+// keys at depth D are staged in LDS, so the per-thread merge-path search
+// and the merge's comparisons are decided by the keys; the text is read
+// only past depth D + 8, for strings equal on those 8 bytes.
 __global__ __launch_bounds__(kMergeThreads) void k_merge(
     const uint64_t* __restrict__ t64, uint64_t nwords, const uint32_t* __restrict__ a,
     const uint64_t* __restrict__ alcp, uint64_t na, const uint32_t* __restrict__ b,
     const uint64_t* __restrict__ blcp, uint64_t nb, const uint64_t* __restrict__ split,
     uint32_t* __restrict__ out, uint64_t* __restrict__ olcp) {
   __shared__ uint64_t block_depth;
+  __shared__ uint64_t s_key[kMergeTile];  // A's keys at [0, ia), then B's
+  __shared__ uint32_t s_off[kMergeTile];
   const uint64_t total = na + nb;
   const uint64_t k0 = (uint64_t)blockIdx.x * kMergeTile;
   const uint64_t k1 = k0 + kMergeTile < total ? k0 + kMergeTile : total;
   const uint64_t i0 = split[blockIdx.x], i1 = split[blockIdx.x + 1];
   const uint64_t j0 = k0 - i0, j1 = k1 - i1;
-  // The block's outputs are A[i0, i1) and B[j0, j1); in sorted order any two
-  // of them share at least lcp(first, last) bytes, so the comparisons below
-  // start there.
+  const uint32_t ia = (uint32_t)(i1 - i0), jb = (uint32_t)(j1 - j0);  // ia + jb = k1 - k0
   if (threadIdx.x == 0) {
     uint64_t first, last;
     if (i0 == i1) first = b[j0];
@@ -433,12 +439,34 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
   }
   __syncthreads();
   const uint64_t depth = block_depth;
+  for (uint32_t x = threadIdx.x; x < ia + jb; x += kMergeThreads) {
+    const uint32_t o = x < ia ? a[i0 + x] : b[j0 + (x - ia)];
+    s_off[x] = o;
+    s_key[x] = load_key(t64, nwords, (uint64_t)o + depth);
+  }
+  __syncthreads();
+  // A[i] against B[j] for strings of the block (sharing `depth` bytes), as
+  // str_compare: the staged keys decide unless they are equal without a NUL.
+  auto cmp_block = [&](uint64_t i, uint64_t j) -> StrCmp {
+    const uint64_t ka = s_key[i - i0], kb = s_key[ia + (j - j0)];
+    if (ka != kb) return StrCmp{ka < kb ? -1 : 1, depth + key_lcp(ka, kb)};
+    if ((ka & 0xFFu) == 0) return StrCmp{0, depth + key_len(ka)};
+    return str_compare(t64, nwords, s_off[i - i0], s_off[ia + (j - j0)], depth + 8);
+  };
   const uint64_t d = k0 + (uint64_t)threadIdx.x * kMergeItems;
   if (d >= k1) return;
-  // Narrow to this block's ranges: i in [i0, i1], d - i in [j0, j1].
-  const uint64_t lo = (d > j1 && d - j1 > i0) ? d - j1 : i0;
-  const uint64_t hi = d - j0 < i1 ? d - j0 : i1;
-  uint64_t i = merge_path(t64, nwords, a, b, d, lo, hi, depth);
+  // Merge-path split of this thread inside the block: i in [lo, hi], with
+  // A[mid] and B[d - 1 - mid] inside the block's ranges for every probe.
+  uint64_t lo = (d > j1 && d - j1 > i0) ? d - j1 : i0;
+  uint64_t hi = d - j0 < i1 ? d - j0 : i1;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (cmp_block(mid, d - 1 - mid).sign <= 0)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  uint64_t i = lo;
   uint64_t j = d - i;
   const uint64_t end = d + kMergeItems < k1 ? d + kMergeItems : k1;
   bool have_s = false;
@@ -446,6 +474,9 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
   for (uint64_t k = d; k < total && k <= end; ++k) {
     bool take_a;
     uint64_t lprev = 0;  // lcp(previous output, this output)
+    // Heads past the block's ranges (only on the extra step) are compared
+    // on the text.
+    const bool inside = i < i1 && j < j1;
     if (i == na) {
       take_a = false;
       lprev = hb;
@@ -453,15 +484,16 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
       take_a = true;
       lprev = ha;
     } else if (!have_s) {
-      // Both heads lie in the block's outputs unless one is past its range.
-      const StrCmp c = str_compare(t64, nwords, a[i], b[j], (i < i1 && j < j1) ? depth : 0);
+      const StrCmp c = inside ? cmp_block(i, j) : str_compare(t64, nwords, a[i], b[j], 0);
       take_a = c.sign <= 0;
       if (take_a) hb = c.lcp; else ha = c.lcp;
     } else if (ha != hb) {
       take_a = ha > hb;
       lprev = take_a ? ha : hb;
     } else {
-      const StrCmp c = str_compare(t64, nwords, a[i], b[j], ha);
+      // Both share ha >= depth bytes with the last output, hence with each
+      // other: the keys at `depth` decide as a comparison from ha would.
+      const StrCmp c = inside ? cmp_block(i, j) : str_compare(t64, nwords, a[i], b[j], ha);
       take_a = c.sign <= 0;
       lprev = ha;
       if (take_a) hb = c.lcp; else ha = c.lcp;
@@ -469,11 +501,11 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
     if (have_s && olcp) olcp[k - 1] = lprev;
     if (k == end) break;  // the extra step: only the LCP above
     if (take_a) {
-      out[k] = a[i];
+      out[k] = i < i1 ? s_off[i - i0] : a[i];
       ha = i + 1 < na ? alcp[i] : 0;
       ++i;
     } else {
-      out[k] = b[j];
+      out[k] = j < j1 ? s_off[ia + (j - j0)] : b[j];
       hb = j + 1 < nb ? blcp[j] : 0;
       ++j;
     }
