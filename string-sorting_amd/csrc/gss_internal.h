@@ -384,8 +384,6 @@ hipError_t launch_retire(const Plan& p, uint32_t nretire, hipStream_t s);
 // also counts the 8-byte key levels into the statistics (slower: profiling).
 hipError_t launch_local(const Plan& p, int cls, uint32_t first, uint32_t nsegs, hipStream_t s,
                         bool count_levels);
-hipError_t launch_zero_counters(const Plan& p, hipStream_t s);
-hipError_t launch_fill_u64(uint64_t* dst, uint64_t n, uint64_t v, hipStream_t s);
 hipError_t launch_ptr_to_off(const uint64_t* ptrs, uint64_t base, uint32_t* off,
                              uint64_t n, hipStream_t s);
 hipError_t launch_off_to_ptr(const uint32_t* off, uint64_t base, uint64_t* ptrs,

@@ -37,17 +37,6 @@ namespace gss {
 // ---------------------------------------------------------------------------
 // Small helpers
 
-__global__ void k_fill_u64(uint64_t* dst, uint64_t n, uint64_t v) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    dst[i] = v;
-}
-
-__global__ void k_zero_counters(Counters* c) {
-  uint32_t* w = reinterpret_cast<uint32_t*>(c);
-  for (uint32_t i = threadIdx.x; i < sizeof(Counters) / 4; i += blockDim.x) w[i] = 0;
-}
-
 __global__ void k_ptr_to_off(const uint64_t* __restrict__ ptrs, uint64_t base,
                              uint32_t* __restrict__ off, uint64_t n) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -947,17 +936,6 @@ hipError_t launch_boundary_lcp(const Plan& p, uint32_t first, hipStream_t s) {
 hipError_t launch_retire(const Plan& p, uint32_t nretire, hipStream_t s) {
   if (!nretire) return hipSuccess;
   k_retire<<<grid_for(nretire, 1, 8192), 256, 0, s>>>(p, nretire);
-  return hipGetLastError();
-}
-
-hipError_t launch_zero_counters(const Plan& p, hipStream_t s) {
-  k_zero_counters<<<1, 64, 0, s>>>(p.ctr);
-  return hipGetLastError();
-}
-
-hipError_t launch_fill_u64(uint64_t* dst, uint64_t n, uint64_t v, hipStream_t s) {
-  if (!n) return hipSuccess;
-  k_fill_u64<<<grid_for(n, 256), 256, 0, s>>>(dst, n, v);
   return hipGetLastError();
 }
 
