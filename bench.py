@@ -132,8 +132,9 @@ def _best_time(fn, runs, prepare=None):
 def pipeline_rows(ss, dev, runs=5):
     """The steps either side of the sort on the same 10M URL-like set:
     device ingest of the newline file, --write formatting of the sorted
-    output, LCP-aware merge of two sorted halves, suffix sorting of a 10 MB
-    ACGT text, and the whole file-in / file-out pipeline with PCIe."""
+    output, LCP-aware merge of two sorted halves and of eight sorted runs,
+    suffix sorting of a 10 MB ACGT text, and the whole file-in / file-out
+    pipeline with PCIe."""
     import torch
 
     import gss
@@ -195,6 +196,29 @@ def pipeline_rows(ss, dev, runs=5):
                         "equals_full_sort": same,
                         "note": "two sorted 5M halves; 24 B/string = offset+LCP in, offset+LCP out"}
     del m, ml, a, b, al, bl
+
+    # K-way merge: eight sorted eighths of the input order.
+    k = 8
+    bounds = [n * r // k for r in range(k + 1)]
+    kruns, klcps = [], []
+    for r in range(k):
+        lo, hi = bounds[r], bounds[r + 1]
+        ro = torch.empty(hi - lo, dtype=torch.int32, device=dev)
+        rl = torch.empty(max(1, hi - lo - 1), dtype=torch.int64, device=dev)
+        gss.sort_device(work, off[lo:hi].contiguous(), ro, rl, text_bytes=tb)
+        kruns.append(ro)
+        klcps.append(rl[: max(0, hi - lo - 1)])
+
+    def merge_kway():
+        box["k"] = gss.merge_lcp_kway(work, kruns, klcps, text_bytes=tb)
+
+    t = _best_time(merge_kway, runs)
+    m, ml = box.pop("k")
+    same = bool(torch.equal(m, out)) and bool(torch.equal(ml, lcp))
+    res["merge_lcp_kway"] = {"seconds": t, "strings": n, "runs": k,
+                             "mstrings_per_s": n / t / 1e6, "equals_full_sort": same,
+                             "note": "eight sorted 1.25M runs merged in three rounds of 2-way merges"}
+    del m, ml, kruns, klcps
 
     # Suffix sorting of a 10 MB i.i.d. ACGT text.
     rng = np.random.default_rng(20242)

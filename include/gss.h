@@ -196,6 +196,24 @@ int gss_merge_lcp_device32(const unsigned char *d_text, uint64_t text_bytes,
 			   const uint32_t *d_b, const uint64_t *d_b_lcp, uint64_t nb,
 			   uint32_t *d_out, uint64_t *d_out_lcp, void *stream);
 
+/* K-way LCP merge of sorted runs of one text, replacing merge_lcp_3way
+ * (src/mergesort_lcp.cpp:399-709) for any number of runs: run r is
+ * d_runs[r] (run_len[r] device offsets) with its LCP array d_run_lcps[r]
+ * (run_len[r]-1 entries); the three arrays are host arrays of k entries.
+ * Equal strings come out in run order (run 0 first), as merge_lcp_3way's
+ * ties do, so merging the stable sorts of k parts gives the stable sort of
+ * their concatenation.  Adjacent runs merge pairwise in rounds (a balanced
+ * tree of the 2-way merge above, all queued on the stream) through two
+ * buffers of the total size (24 bytes per string) in the library's
+ * per-device scratch, which grows as needed and is kept.  d_out receives the
+ * merged offsets, d_out_lcp (optional) their LCP array. */
+int gss_merge_lcp_kway_device32(const unsigned char *d_text, uint64_t text_bytes,
+				const uint32_t *const *d_runs,
+				const uint64_t *const *d_run_lcps,
+				const uint64_t *run_len, uint32_t k,
+				uint32_t *d_out, uint64_t *d_out_lcp,
+				void *stream);
+
 /* ------------------------------------------------------------------------
  * Tuning knobs (no reference counterpart: the reference fixes its
  * thresholds at compile time, e.g. src/msd_ce.cpp:79 n < 32 and :216

@@ -20,6 +20,7 @@
 
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "gss_internal.h"
 #include "host.h"
@@ -634,6 +635,32 @@ int gss_ingest_device32(unsigned char* d_text, uint64_t text_bytes, int delim, u
   return GSS_OK;
 }
 
+}  // extern "C"
+
+namespace gss {
+namespace {
+
+// Queues one 2-way merge on `st` (no sync); `split` holds at least
+// merge_splits(na + nb) entries.
+uint64_t merge_splits(uint64_t total) { return (total + kMergeTile - 1) / kMergeTile + 1; }
+
+hipError_t merge_launch(const uint64_t* t64, uint64_t nwords, const uint32_t* a,
+                        const uint64_t* alcp, uint64_t na, const uint32_t* b,
+                        const uint64_t* blcp, uint64_t nb, uint32_t* out, uint64_t* olcp,
+                        uint64_t* split, hipStream_t st) {
+  const uint64_t nparts = merge_splits(na + nb) - 1;
+  k_merge_partition<<<(uint32_t)((nparts + 1 + 255) / 256), 256, 0, st>>>(t64, nwords, a, na, b,
+                                                                         nb, nparts, split);
+  k_merge<<<(uint32_t)nparts, kMergeThreads, 0, st>>>(t64, nwords, a, alcp, na, b, blcp, nb,
+                                                      split, out, olcp);
+  return hipGetLastError();
+}
+
+}  // namespace
+}  // namespace gss
+
+extern "C" {
+
 int gss_merge_lcp_device32(const unsigned char* d_text, uint64_t text_bytes, const uint32_t* d_a,
                            const uint64_t* d_a_lcp, uint64_t na, const uint32_t* d_b,
                            const uint64_t* d_b_lcp, uint64_t nb, uint32_t* d_out,
@@ -652,17 +679,99 @@ int gss_merge_lcp_device32(const unsigned char* d_text, uint64_t text_bytes, con
   int rc = scratch_for(&sc);
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(sc->mu);
-  const uint64_t nparts = (total + kMergeTile - 1) / kMergeTile;
-  rc = scratch_reserve(*sc, al256(8 * (nparts + 1)), st);
+  rc = scratch_reserve(*sc, al256(8 * merge_splits(total)), st);
   if (rc) return rc;
-  uint64_t* split = reinterpret_cast<uint64_t*>(sc->dev);
+  PL_HIP(merge_launch(reinterpret_cast<const uint64_t*>(d_text), (text_bytes + 7) / 8, d_a,
+                      d_a_lcp, na, d_b, d_b_lcp, nb, d_out, d_out_lcp,
+                      reinterpret_cast<uint64_t*>(sc->dev), st));
+  PL_HIP(hipStreamSynchronize(st));
+  return GSS_OK;
+}
+
+int gss_merge_lcp_kway_device32(const unsigned char* d_text, uint64_t text_bytes,
+                                const uint32_t* const* d_runs, const uint64_t* const* d_run_lcps,
+                                const uint64_t* run_len, uint32_t k, uint32_t* d_out,
+                                uint64_t* d_out_lcp, void* stream) {
+  if (k && (!d_runs || !d_run_lcps || !run_len))
+    return fail(GSS_E_INVALID, "NULL run table");
+  uint64_t total = 0;
+  for (uint32_t r = 0; r < k; ++r) {
+    if (run_len[r] && !d_runs[r]) return fail(GSS_E_INVALID, "NULL run pointer");
+    if (run_len[r] > 1 && !d_run_lcps[r])
+      return fail(GSS_E_INVALID, "each run of two or more strings needs its LCP array");
+    total += run_len[r];
+  }
+  if (total == 0) return GSS_OK;
+  if (!d_text || !d_out) return fail(GSS_E_INVALID, "NULL text or output pointer");
+  if (reinterpret_cast<uintptr_t>(d_text) & 7)
+    return fail(GSS_E_INVALID, "the device text must be 8-byte aligned");
+  if (total >= (1ull << 32)) return fail(GSS_E_TOO_LARGE, "at most 2^32-1 strings");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  // A run in the current round: offsets, LCPs (len - 1 of them), length.
+  struct Run {
+    const uint32_t* off;
+    const uint64_t* lcp;
+    uint64_t len;
+  };
+  std::vector<Run> cur;
+  for (uint32_t r = 0; r < k; ++r)
+    if (run_len[r]) cur.push_back(Run{d_runs[r], d_run_lcps[r], run_len[r]});
+  if (cur.size() == 1) {  // nothing to merge: copy the run
+    PL_HIP(hipMemcpyAsync(d_out, cur[0].off, 4 * total, hipMemcpyDeviceToDevice, st));
+    if (d_out_lcp && total > 1)
+      PL_HIP(hipMemcpyAsync(d_out_lcp, cur[0].lcp, 8 * (total - 1), hipMemcpyDeviceToDevice, st));
+    PL_HIP(hipStreamSynchronize(st));
+    return GSS_OK;
+  }
+  // Two ping-pong buffers for the intermediate rounds (a run at position p
+  // of the merged order keeps its offsets and LCPs at [p, p + len)) and the
+  // merge-path splits of one round, in the device's grow-only scratch.
+  // Every merge is queued on the stream; one sync at the end.
+  Scratch* sc = nullptr;
+  int rc = scratch_for(&sc);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(sc->mu);
+  const size_t b_off = al256(4 * total), b_lcp = al256(8 * total);
+  const size_t b_split = al256(8 * (merge_splits(total) + cur.size()));
+  rc = scratch_reserve(*sc, 2 * b_off + 2 * b_lcp + b_split, st);
+  if (rc) return rc;
+  char* base = sc->dev;
+  uint32_t* t_off[2] = {reinterpret_cast<uint32_t*>(base), reinterpret_cast<uint32_t*>(base + b_off)};
+  uint64_t* t_lcp[2] = {reinterpret_cast<uint64_t*>(base + 2 * b_off),
+                        reinterpret_cast<uint64_t*>(base + 2 * b_off + b_lcp)};
+  uint64_t* split = reinterpret_cast<uint64_t*>(base + 2 * b_off + 2 * b_lcp);
   const uint64_t* t64 = reinterpret_cast<const uint64_t*>(d_text);
   const uint64_t nwords = (text_bytes + 7) / 8;
-  k_merge_partition<<<(uint32_t)((nparts + 1 + 255) / 256), 256, 0, st>>>(t64, nwords, d_a, na, d_b,
-                                                                         nb, nparts, split);
-  k_merge<<<(uint32_t)nparts, kMergeThreads, 0, st>>>(t64, nwords, d_a, d_a_lcp, na, d_b, d_b_lcp,
-                                                      nb, split, d_out, d_out_lcp);
-  PL_HIP(hipGetLastError());
+  // Rounds of pairwise merges of adjacent runs (run 2i with 2i + 1), so
+  // that equal strings keep the order of their runs; the last round writes
+  // the output.  The rounds run in stream order, so one round's buffer and
+  // split table are free again two rounds later.
+  for (int round = 0; cur.size() > 1; ++round) {
+    const bool last = cur.size() == 2;
+    uint32_t* o_off = last ? d_out : t_off[round & 1];
+    uint64_t* o_lcp = last ? d_out_lcp : t_lcp[round & 1];
+    std::vector<Run> next;
+    uint64_t p = 0, sp = 0;
+    for (size_t r = 0; r < cur.size(); r += 2) {
+      if (r + 1 < cur.size()) {
+        const uint64_t len = cur[r].len + cur[r + 1].len;
+        PL_HIP(merge_launch(t64, nwords, cur[r].off, cur[r].lcp, cur[r].len, cur[r + 1].off,
+                            cur[r + 1].lcp, cur[r + 1].len, o_off + p,
+                            o_lcp ? o_lcp + p : nullptr, split + sp, st));
+        next.push_back(Run{o_off + p, o_lcp ? o_lcp + p : nullptr, len});
+        p += len;
+        sp += merge_splits(len);
+      } else {  // the odd run moves to this round's buffer unchanged
+        PL_HIP(hipMemcpyAsync(o_off + p, cur[r].off, 4 * cur[r].len, hipMemcpyDeviceToDevice, st));
+        if (cur[r].len > 1)
+          PL_HIP(hipMemcpyAsync(o_lcp + p, cur[r].lcp, 8 * (cur[r].len - 1),
+                                hipMemcpyDeviceToDevice, st));
+        next.push_back(Run{o_off + p, o_lcp + p, cur[r].len});
+        p += cur[r].len;
+      }
+    }
+    cur.swap(next);
+  }
   PL_HIP(hipStreamSynchronize(st));
   return GSS_OK;
 }

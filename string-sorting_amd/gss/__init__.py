@@ -104,6 +104,10 @@ _SIGNATURES = {
     "gss_merge_lcp_device32": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
                                          C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint64,
                                          C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gss_merge_lcp_kway_device32": (C.c_int, [C.c_void_p, C.c_uint64,
+                                              C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                              C.POINTER(C.c_uint64), C.c_uint32, C.c_void_p,
+                                              C.c_void_p, C.c_void_p]),
     "gss_autotune_device32": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
                                         C.POINTER(AutotuneResult), C.c_void_p]),
     "gss_autotune_enable": (C.c_int, [C.c_int]),
@@ -366,6 +370,30 @@ def merge_lcp(text, a, a_lcp, b, b_lcp, text_bytes=None, want_lcp=True, stream=N
                                         nb, out.data_ptr(), _ptr(ol), _stream_ptr(text, stream)),
            "gss_merge_lcp_device32")
     return out[: na + nb], (ol[: max(0, na + nb - 1)] if want_lcp else None)
+
+
+def merge_lcp_kway(text, runs, run_lcps, text_bytes=None, want_lcp=True, stream=None):
+    """K-way LCP merge of sorted runs (lists of int32 offset tensors and
+    their int64 LCP tensors of n-1 entries); equal strings keep run order.
+    Returns (offsets, lcp or None)."""
+    import torch
+
+    if len(runs) != len(run_lcps):
+        raise GssError("one LCP tensor per run")
+    tb = text.numel() if text_bytes is None else int(text_bytes)
+    k = len(runs)
+    lens = [r.numel() for r in runs]
+    total = sum(lens)
+    out = torch.empty(max(1, total), dtype=torch.int32, device=text.device)
+    ol = torch.empty(max(1, total - 1), dtype=torch.int64, device=text.device) if want_lcp else None
+    offs = (C.c_void_p * max(1, k))(*[r.data_ptr() if n else None for r, n in zip(runs, lens)])
+    lcps = (C.c_void_p * max(1, k))(*[l.data_ptr() if n > 1 else None
+                                      for l, n in zip(run_lcps, lens)])
+    nlen = (C.c_uint64 * max(1, k))(*lens)
+    _check(lib().gss_merge_lcp_kway_device32(text.data_ptr(), tb, offs, lcps, nlen, k,
+                                             out.data_ptr(), _ptr(ol), _stream_ptr(text, stream)),
+           "gss_merge_lcp_kway_device32")
+    return out[:total], (ol[: max(0, total - 1)] if want_lcp else None)
 
 
 class Server:

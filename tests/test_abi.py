@@ -123,6 +123,28 @@ def test_server_create_validates_before_touching_a_device():
     lib.gss_server_destroy(None)
 
 
+def test_kway_merge_validates_before_touching_a_device():
+    """The k-way merge checks its run table on the host: missing LCP arrays
+    for runs of two or more strings and NULL runs are refused; no strings at
+    all is a no-op."""
+    import ctypes as C
+
+    lib = gss.lib()
+    lens = (C.c_uint64 * 2)(5, 3)
+    offs = (C.c_void_p * 2)(1 << 20, 1 << 21)
+    lcps = (C.c_void_p * 2)(1 << 22, None)
+    assert lib.gss_merge_lcp_kway_device32(C.c_void_p(1 << 23), 64, offs, lcps, lens, 2,
+                                           C.c_void_p(1 << 24), None, None) < 0
+    assert "LCP array" in gss.last_error()
+    offs2 = (C.c_void_p * 2)(None, 1 << 21)
+    lcps2 = (C.c_void_p * 2)(1 << 22, 1 << 22)
+    assert lib.gss_merge_lcp_kway_device32(C.c_void_p(1 << 23), 64, offs2, lcps2, lens, 2,
+                                           C.c_void_p(1 << 24), None, None) < 0
+    assert "NULL run" in gss.last_error()
+    zero = (C.c_uint64 * 2)(0, 0)
+    assert lib.gss_merge_lcp_kway_device32(None, 0, offs, lcps, zero, 2, None, None, None) == 0
+
+
 def test_autotune_switches_without_device():
     """Enabling, resetting and counting the autotuner's cache touch no
     device; tuning rejects missing inputs before any device call."""

@@ -192,3 +192,48 @@ def test_merge_lcp_matches_oracle(sizes, oracle_mod):
     bo, blo = oracle_mod.sort_offsets(joined.text, b_in, lcp=True)
     mo, mlo = oracle_mod.merge_lcp(joined.text, ao, alo, bo, blo)
     assert np.array_equal(got, mo) and np.array_equal(ol.cpu().numpy().view(np.uint64), mlo)
+
+
+@pytest.mark.parametrize("sizes", [(7_000,), (9_000, 0, 4_000), (3_000, 3_000, 3_000),
+                                   (1, 20_000, 0, 5, 2_500), (800,) * 8, (0, 0)])
+def test_merge_lcp_kway_matches_oracle(sizes, oracle_mod):
+    """k sorted runs (k = 1..8, uneven and empty ones, duplicates across
+    runs) merge into the stable sort of their concatenation with exact LCPs:
+    equal strings keep run order, as merge_lcp_3way's ties do
+    (src/mergesort_lcp.cpp:399-709).  The k = 2 case equals the 2-way merge."""
+    import torch
+
+    parts = [gen.make(c, n=20_000, seed=70 + c) for c in range(4)]
+    joined, _ = gen.concat(parts)
+    rng = np.random.default_rng(sum(sizes) + len(sizes))
+    total = sum(sizes)
+    pick = rng.integers(0, joined.n, size=max(1, total))[:total]
+    if total > 20:
+        pick[total // 2: total // 2 + total // 10] = pick[: total // 10]  # cross-run duplicates
+    text = _dev(joined.text)
+    runs, lcps, ins = [], [], []
+    start = 0
+    for n in sizes:
+        part = joined.offsets[pick[start:start + n]]
+        ins.append(part)
+        start += n
+        if n:
+            o, lc = _sorted_run(text, part, joined.text_bytes)
+            runs.append(o[:n])
+            lcps.append(lc[: max(0, n - 1)])
+        else:
+            runs.append(torch.empty(0, dtype=torch.int32, device=device()))
+            lcps.append(torch.empty(0, dtype=torch.int64, device=device()))
+    log(f"k-way merge sizes={sizes}")
+    out, ol = gss.merge_lcp_kway(text, runs, lcps, text_bytes=joined.text_bytes)
+    got = out.cpu().numpy().view(np.uint32)
+    assert got.size == total
+    if total == 0:
+        return
+    want, want_lcp = oracle_mod.sort_offsets(joined.text, np.concatenate(ins), lcp=True)
+    assert np.array_equal(got, want)
+    assert np.array_equal(ol.cpu().numpy().view(np.uint64), want_lcp)
+    if len(sizes) == 2:
+        o2, l2 = gss.merge_lcp(text, runs[0], lcps[0], runs[1], lcps[1],
+                               text_bytes=joined.text_bytes)
+        assert np.array_equal(o2.cpu().numpy(), out.cpu().numpy())
