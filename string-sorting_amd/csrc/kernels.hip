@@ -7,9 +7,12 @@
 // strings that never span two segments:
 //
 //   gather_minmax  key = the 8 bytes at each string's depth (big-endian, NUL
-//                  padded), cached in `keys`; per-segment min/max key; the
-//                  pass alphabet (set of byte values seen).
-//   layout         per segment: its digit (first differing byte + k ranked
+//                  padded), cached in `keys` with the 8 bytes after it in
+//                  `ext` -- or, every other pass, taken as the previous
+//                  scatter left it in `keys`; per-tile min/max key and set
+//                  of byte values seen.
+//   segreduce      per segment: min/max key and byte set over its tiles.
+//   layout         the pass alphabet; per segment its digit (first differing byte + k ranked
 //                  characters, see digit_spec) and the place of its
 //                  digit-major count matrix in the flat array.
 //   histogram      per tile digit counts -> flat[base + d*ntiles + t].
@@ -19,7 +22,11 @@
 //                  (next pass, local, or final), retire runs, boundaries.
 //   scatter        stable partition of the offsets by digit: wave
 //                  match-ranking in LDS for the in-tile rank, staged in LDS
-//                  so the global writes are contiguous runs.
+//                  so the global writes are contiguous runs; after a
+//                  text-reading gather it also writes each string bound for
+//                  the next pass its key at the child's depth.
+//
+// Blocks of the tile kernels take tiles in XCD-aware order (xcd_tile).
 //   boundary_lcp   LCP of each new adjacent pair of child buckets.
 //
 // A segment whose keys are all equal costs the gather only: it keeps its
