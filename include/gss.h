@@ -221,7 +221,7 @@ int gss_merge_lcp_kway_device32(const unsigned char *d_text, uint64_t text_bytes
  * GSS_E_INVALID. */
 typedef struct gss_tuning {
 	uint32_t local_threshold; /* segments of at most this many strings finish
-				     inside one workgroup (64..8192)          */
+				     inside one workgroup (64..6144)          */
 	uint32_t radix_bits;      /* digit width of a global MSD pass (8..11)  */
 	uint32_t tile;            /* strings per workgroup in a global pass
 				     (1024..16384, multiple of 1024)           */
@@ -234,7 +234,7 @@ int gss_set_tuning(const gss_tuning *in);
  * string length, alphabet size and string count, each as a power-of-two
  * bucket, read from 512 evenly spaced strings.  gss_autotune_device32 sorts
  * a strided sample of the input (at most 2M strings) with every candidate
- * tuning -- local_threshold 1024/2048/4096/8192 x radix_bits 11/8 -- and
+ * tuning -- local_threshold 1024/2048/4096/6144 x radix_bits 11/8 -- and
  * returns the fastest (best_ms; default_ms is the current tuning's time, 0
  * when it is not a candidate), caching it for the signature.  It does not
  * change the current tuning.  With gss_autotune_enable(1), gss_sort_device32

@@ -29,7 +29,7 @@ constexpr uint64_t kTrialStrings = 1u << 21;  // strings sorted per candidate
 constexpr uint64_t kAutoMin = 1u << 16;       // smaller inputs keep the current tuning
 constexpr int kTimedTrials = 2;
 
-constexpr uint32_t kThresholds[] = {1024, 2048, 4096, 8192};
+constexpr uint32_t kThresholds[] = {1024, 2048, 4096, 6144};
 constexpr uint32_t kRadixBits[] = {11, 8};
 
 struct SigOut {

@@ -11,7 +11,7 @@ namespace gss {
 constexpr int kWave = 64;
 
 // Largest segment a single workgroup finishes (the local kernel's capacity).
-constexpr uint32_t kLocalCapMax = 8192;
+constexpr uint32_t kLocalCapMax = 6144;
 
 // Global MSD pass geometry: strings per tile = kTileThreads * kTileItems.
 constexpr int kTileThreads = 512;
@@ -106,7 +106,7 @@ struct LocalStats {
 };
 
 // Size classes of the local kernel: (threads, capacity).
-constexpr uint32_t kLocalClassCap[3] = {64, 1024, 8192};
+constexpr uint32_t kLocalClassCap[3] = {64, 1024, 6144};
 constexpr uint32_t kLocalClassThreads[3] = {64, 256, 1024};
 
 // Pointers into the workspace for one sort.

@@ -466,7 +466,7 @@ hipError_t launch_local_t(const Plan& p, int cls, uint32_t first, uint32_t nsegs
       k_local<256, 1024, COUNT><<<nsegs, 256, 0, s>>>(p, p.local[1] + first);
       break;
     case 2:
-      k_local<1024, 8192, COUNT><<<nsegs, 1024, 0, s>>>(p, p.local[2] + first);
+      k_local<1024, 6144, COUNT><<<nsegs, 1024, 0, s>>>(p, p.local[2] + first);
       break;
     default:
       return hipErrorInvalidValue;

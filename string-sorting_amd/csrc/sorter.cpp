@@ -64,7 +64,7 @@ struct Pending {
 };
 
 std::mutex g_mu;  // tuning, profile totals, default contexts
-gss_tuning g_tuning = {8192u, 11u, kTile};
+gss_tuning g_tuning = {6144u, 11u, kTile};
 std::atomic<bool> g_prof{false};    // HIP event brackets
 std::atomic<bool> g_levels{false};  // counting variant of the local kernels
 std::map<std::string, ProfTotal> g_prof_totals;
@@ -456,7 +456,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     GSS_HIP(hipEventRecord(c.side_done, st));
     GSS_HIP(hipStreamWaitEvent(c.side, c.side_done, 0));
     side_used = true;
-    static const char* kEarlyNames[3] = {"local64_early", "local1024_early", "local8192_early"};
+    static const char* kEarlyNames[3] = {"local64_early", "local1024_early", "local6144_early"};
     for (int cls = 2; cls >= 0; --cls) {
       if (upto[cls] <= launched[cls]) continue;
       if (upto[cls] > p.cap_local[cls]) return fail(GSS_E_INTERNAL, "local segment list overflow");
@@ -566,7 +566,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   // summed over the local key rounds, are filled in after the read-back).
   ProfToken phase_tok = prof_begin(c, "local_phase", 0, st);
   if (nretire) GSS_LAUNCH("retire", nretire, launch_retire(p, nretire, st));
-  static const char* kLocalNames[3] = {"local64", "local1024", "local8192"};
+  static const char* kLocalNames[3] = {"local64", "local1024", "local6144"};
   long local_tok[3] = {-1, -1, -1};  // pending profile entries of the local launches
   for (int cls = 2; cls >= 0; --cls) {
     if (nlocal[cls] > p.cap_local[cls]) return fail(GSS_E_INTERNAL, "local segment list overflow");
@@ -668,7 +668,7 @@ int sort_device32_tuned(const unsigned char* text, uint64_t text_bytes, const ui
 
 int check_tuning(const gss_tuning& t) {
   if (t.local_threshold < 64 || t.local_threshold > kLocalCapMax)
-    return fail(GSS_E_INVALID, "local_threshold must lie in [64, 8192]");
+    return fail(GSS_E_INVALID, "local_threshold must lie in [64, " + std::to_string(kLocalCapMax) + "]");
   if (t.radix_bits != 8 && t.radix_bits != 11)
     return fail(GSS_E_INVALID, "radix_bits must be 8 or 11");
   if (t.tile != kTile) return fail(GSS_E_INVALID, "tile must be 4096");

@@ -29,7 +29,7 @@ def test_autotune_picks_a_candidate():
     log(f"best threshold={r.best.local_threshold} radix={r.best.radix_bits} "
         f"{r.best_ms:.3f} ms (current tuning {r.default_ms:.3f} ms), sample {r.sample}")
     assert r.candidates == 8 and r.sample == ss.n
-    assert r.best.local_threshold in (1024, 2048, 4096, 8192) and r.best.radix_bits in (8, 11)
+    assert r.best.local_threshold in (1024, 2048, 4096, 6144) and r.best.radix_bits in (8, 11)
     assert 0 < r.best_ms <= r.default_ms
     after = gss.get_tuning()
     assert (after.local_threshold, after.radix_bits) == (before.local_threshold,

@@ -78,7 +78,7 @@ def test_deep_shared_prefix(plen, oracle_mod):
     assert_matches_oracle(ss, off, lcp, oracle_mod)
 
 
-@pytest.mark.parametrize("threshold", [64, 1024, 8192])
+@pytest.mark.parametrize("threshold", [64, 1024, 6144])
 @pytest.mark.parametrize("radix_bits", [8, 11])
 def test_tuning_knobs(threshold, radix_bits, oracle_mod):
     gss.set_tuning(local_threshold=threshold, radix_bits=radix_bits)
