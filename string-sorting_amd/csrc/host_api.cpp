@@ -11,12 +11,16 @@
 // unit-test/main.cpp:226-229).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cinttypes>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gss.h"
@@ -61,6 +65,161 @@ bool span_readable(uintptr_t lo, uintptr_t hi) {
   fclose(f);
   return ok;
 }
+
+// A fork-join pool of host threads for the pointer front end's bulk host
+// work (pointer scans and copies between the caller's pageable memory and
+// pinned staging buffers).  Workers never call HIP.  run(fn) calls
+// fn(worker, workers) on every worker, the calling thread being worker 0,
+// and returns when all have finished.
+class HostPool {
+ public:
+  explicit HostPool(int n) : n_(n < 1 ? 1 : n) {
+    for (int w = 1; w < n_; ++w) th_.emplace_back([this, w] { loop(w); });
+  }
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_go_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  int size() const { return n_; }
+  void run(const std::function<void(int, int)>& fn) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      job_ = &fn;
+      pending_ = n_ - 1;
+      ++gen_;
+    }
+    cv_go_.notify_all();
+    fn(0, n_);
+    std::unique_lock<std::mutex> lk(mu_);
+    cv_done_.wait(lk, [this] { return pending_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  void loop(int w) {
+    uint64_t seen = 0;
+    while (true) {
+      const std::function<void(int, int)>* job;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_go_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        job = job_;
+      }
+      (*job)(w, n_);
+      std::lock_guard<std::mutex> lk(mu_);
+      if (--pending_ == 0) cv_done_.notify_one();
+    }
+  }
+  const int n_;
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_go_, cv_done_;
+  const std::function<void(int, int)>* job_ = nullptr;
+  uint64_t gen_ = 0;
+  int pending_ = 0;
+  bool stop_ = false;
+};
+
+HostPool& host_pool() {
+  static HostPool pool(
+      (int)std::min<unsigned>(8u, std::max<unsigned>(1u, std::thread::hardware_concurrency() / 2)));
+  return pool;
+}
+
+// memcpy split over the pool (4 KiB-aligned pieces).
+void par_memcpy(void* dst, const void* src, uint64_t bytes) {
+  HostPool& pool = host_pool();
+  pool.run([&](int w, int nw) {
+    const uint64_t piece = ((bytes + nw - 1) / nw + 4095) & ~4095ull;
+    const uint64_t a = std::min<uint64_t>(bytes, piece * w), b = std::min<uint64_t>(bytes, a + piece);
+    if (b > a) memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a);
+  });
+}
+
+// Copies between the caller's pageable memory and the device through a ring
+// of pinned chunks: host threads fill (or drain) one chunk while the DMA of
+// the others runs.  Pageable copies go through the runtime's own staging at
+// a fraction of the link's rate; this keeps the link busy.  A slot is reused
+// only after the event of its last copy completed (also across calls).
+class Staging {
+ public:
+  static constexpr uint64_t kChunk = 16ull << 20;
+  static constexpr int kSlots = 4;
+
+  void h2d(void* dst, const void* src, uint64_t bytes, hipStream_t st) {
+    if (bytes < 2 * kChunk) {
+      check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), "hipMemcpy(H2D)");
+      return;
+    }
+    ensure();
+    int slot = 0;
+    for (uint64_t off = 0; off < bytes; off += kChunk, slot = (slot + 1) % kSlots) {
+      const uint64_t len = std::min(kChunk, bytes - off);
+      settle(slot);
+      par_memcpy(buf_ + slot * kChunk, static_cast<const char*>(src) + off, len);
+      check(hipMemcpyAsync(static_cast<char*>(dst) + off, buf_ + slot * kChunk, len,
+                           hipMemcpyHostToDevice, st),
+            "hipMemcpy(H2D)");
+      arm(slot, st, nullptr, 0);
+    }
+  }
+
+  // Returns once the data is in `dst` (the stream's earlier work included).
+  void d2h(void* dst, const void* src, uint64_t bytes, hipStream_t st) {
+    if (bytes < 2 * kChunk) {
+      check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), "hipMemcpy(D2H)");
+      check(hipStreamSynchronize(st), "hipStreamSynchronize");
+      return;
+    }
+    ensure();
+    int slot = 0;
+    for (uint64_t off = 0; off < bytes; off += kChunk, slot = (slot + 1) % kSlots) {
+      const uint64_t len = std::min(kChunk, bytes - off);
+      settle(slot);
+      check(hipMemcpyAsync(buf_ + slot * kChunk, static_cast<const char*>(src) + off, len,
+                           hipMemcpyDeviceToHost, st),
+            "hipMemcpy(D2H)");
+      arm(slot, st, static_cast<char*>(dst) + off, len);
+    }
+    for (int q = 0; q < kSlots; ++q) settle((slot + q) % kSlots);  // drain in order
+  }
+
+ private:
+  void ensure() {
+    if (buf_) return;
+    check(hipHostMalloc((void**)&buf_, kChunk * kSlots, hipHostMallocDefault), "hipHostMalloc");
+    for (int q = 0; q < kSlots; ++q)
+      check(hipEventCreateWithFlags(&ev_[q], hipEventDisableTiming), "hipEventCreate");
+  }
+  // Waits for the slot's last copy; a pending device-to-host chunk is then
+  // moved on to its destination.
+  void settle(int slot) {
+    if (!armed_[slot]) return;
+    check(hipEventSynchronize(ev_[slot]), "hipEventSynchronize");
+    armed_[slot] = false;
+    if (out_[slot]) par_memcpy(out_[slot], buf_ + slot * kChunk, out_len_[slot]);
+    out_[slot] = nullptr;
+  }
+  void arm(int slot, hipStream_t st, void* out, uint64_t len) {
+    check(hipEventRecord(ev_[slot], st), "hipEventRecord");
+    armed_[slot] = true;
+    out_[slot] = out;
+    out_len_[slot] = len;
+  }
+  unsigned char* buf_ = nullptr;
+  hipEvent_t ev_[kSlots] = {};
+  bool armed_[kSlots] = {};
+  void* out_[kSlots] = {};
+  uint64_t out_len_[kSlots] = {};
+};
+
+Staging g_staging;
 
 struct PtrBuffers {
   unsigned char* text = nullptr;
@@ -112,13 +271,29 @@ void sort_pointers(unsigned char** strings, size_t n, size_t* lcp_out) {
   hipStream_t st = nullptr;
   if (library_stream(&st)) die(gss_last_error());
 
+  // Lowest and highest pointer (a NULL shows up as lo == 0), split over the
+  // host pool.
   uintptr_t lo = UINTPTR_MAX, hi = 0;
-  for (size_t i = 0; i < n; ++i) {
-    const uintptr_t p = reinterpret_cast<uintptr_t>(strings[i]);
-    if (!p) die("NULL string pointer");
-    lo = p < lo ? p : lo;
-    hi = p > hi ? p : hi;
+  {
+    HostPool& pool = host_pool();
+    std::vector<uintptr_t> wlo(pool.size(), UINTPTR_MAX), whi(pool.size(), 0);
+    pool.run([&](int w, int nw) {
+      const size_t a = n * (size_t)w / nw, b = n * (size_t)(w + 1) / nw;
+      uintptr_t l = UINTPTR_MAX, h = 0;
+      for (size_t i = a; i < b; ++i) {
+        const uintptr_t p = reinterpret_cast<uintptr_t>(strings[i]);
+        l = p < l ? p : l;
+        h = p > h ? p : h;
+      }
+      wlo[w] = l;
+      whi[w] = h;
+    });
+    for (int w = 0; w < pool.size(); ++w) {
+      lo = std::min(lo, wlo[w]);
+      hi = std::max(hi, whi[w]);
+    }
   }
+  if (lo == 0) die("NULL string pointer");
   const uintptr_t end = hi + strlen(reinterpret_cast<const char*>(hi)) + 1;
   const uint64_t span = end - lo;
   const bool use_span = span < (1ull << 32) - 64 && span_readable(lo, end);
@@ -128,11 +303,8 @@ void sort_pointers(unsigned char** strings, size_t n, size_t* lcp_out) {
   if (use_span) {
     text_len = span;
     ensure_text(text_len + 64);
-    check(hipMemcpyAsync(g_buf.text, reinterpret_cast<const void*>(lo), span,
-                         hipMemcpyHostToDevice, st),
-          "hipMemcpy(text)");
-    check(hipMemcpyAsync(g_buf.ptrs, strings, 8 * n, hipMemcpyHostToDevice, st),
-          "hipMemcpy(pointers)");
+    g_staging.h2d(g_buf.text, reinterpret_cast<const void*>(lo), span, st);
+    g_staging.h2d(g_buf.ptrs, strings, 8 * n, st);
     check(launch_ptr_to_off(g_buf.ptrs, lo, g_buf.off, n, st), "ptr_to_off");
   } else {
     std::vector<uint32_t> starts(n);
@@ -175,11 +347,8 @@ void sort_pointers(unsigned char** strings, size_t n, size_t* lcp_out) {
           "off_to_orig_ptr");
     result = g_buf.ptrs2;
   }
-  check(hipMemcpyAsync(strings, result, 8 * n, hipMemcpyDeviceToHost, st), "hipMemcpy(result)");
-  if (lcp_out)
-    check(hipMemcpyAsync(lcp_out, g_buf.lcp, 8 * (n - 1), hipMemcpyDeviceToHost, st),
-          "hipMemcpy(lcp)");
-  check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  g_staging.d2h(strings, result, 8 * n, st);
+  if (lcp_out) g_staging.d2h(lcp_out, g_buf.lcp, 8 * (n - 1), st);
 }
 
 }  // namespace
