@@ -64,6 +64,24 @@ def test_pointer_span_text(oracle_mod):
     assert np.array_equal(got, oracle_mod.sort_offsets(ss.text, ss.offsets))
 
 
+def test_pointer_span_text_staged(oracle_mod):
+    """A span of over 32 MB and 32 MB of pointers go through the staged
+    copies (pinned chunks filled and drained by the host thread pool) both
+    ways, the LCP array too: the output is the input pointers in msd_CE2's
+    order and the LCPs are exact."""
+    ss = gen.make(1, n=4_200_000, seed=12)
+    base = ss.text.ctypes.data
+    ptrs = (np.uint64(base) + ss.offsets.astype(np.uint64)).astype(np.uint64)
+    lcp = np.zeros(ss.n - 1, dtype=np.uint64)
+    log(f"staged pointer sort n={ss.n} text={ss.text_bytes}")
+    gss.lib().gss_msd_lcp(ptrs.ctypes.data_as(C.POINTER(C.c_void_p)), ss.n,
+                          lcp.ctypes.data_as(C.POINTER(C.c_size_t)))
+    got = (ptrs - np.uint64(base)).astype(np.uint32)
+    want, want_lcp = oracle_mod.sort_offsets(ss.text, ss.offsets, lcp=True)
+    assert np.array_equal(got, want)
+    assert np.array_equal(lcp, want_lcp.astype(np.uint64))
+
+
 def test_pointer_packed_path(oracle_mod):
     """Strings in two mappings separated by an unmapped hole force the packed
     upload; the result must still be the caller's own pointers."""
