@@ -92,17 +92,20 @@ struct Counters {
   uint32_t flat_len;     // entries of the current pass's flat count array
   uint32_t error;        // nonzero: a list overflowed or a loop bound was hit
   uint32_t local_strings[3];  // strings handed to each local size class
+  uint32_t pad;               // whole 8-byte words: copied in 8-byte words
 };
+static_assert(sizeof(Counters) % 8 == 0, "Counters is copied in 8-byte words");
+static_assert(sizeof(BigSeg) % 8 == 0, "BigSeg is copied in 8-byte words");
 
 // Statistics of the local kernels, spread over kStatSlots slots (block b
-// updates slot b % kStatSlots, each slot in its own 256-byte line) so that
-// thousands of blocks do not serialise on one address; the host combines
-// the slots.  v[slot][c] for c < 3: sum over class-c rounds of the active
-// strings; v[slot][3]: most rounds one segment needed; v[slot][4 + c]: the
-// 8-byte key levels class-c strings advanced through (up to 4 per round).
+// updates slot b % kStatSlots, each slot in its own 128-byte cache line) so
+// that thousands of blocks do not serialise on one address; the host
+// combines the slots.  v[slot][c] for c < 3: sum over class-c rounds of the
+// active strings; v[slot][3]: most rounds one segment needed; v[slot][4 + c]:
+// the 8-byte key levels class-c strings advanced through (up to 4 per round).
 constexpr int kStatSlots = 64;
 struct LocalStats {
-  unsigned long long v[kStatSlots][32];
+  unsigned long long v[kStatSlots][16];
 };
 
 // Size classes of the local kernel: (threads, capacity).
@@ -384,6 +387,32 @@ hipError_t launch_retire(const Plan& p, uint32_t nretire, hipStream_t s);
 // also counts the 8-byte key levels into the statistics (slower: profiling).
 hipError_t launch_local(const Plan& p, int cls, uint32_t first, uint32_t nsegs, hipStream_t s,
                         bool count_levels);
+// Host-to-device uploads from fine-grained pinned host memory, launched
+// together: each range copies nwords 8-byte words from src to dst, the first
+// nzero of them cleared instead of read.  A host block must stay unchanged
+// until its launch has run.
+struct UploadRange {
+  const uint64_t* src;
+  uint64_t* dst;
+  uint32_t nzero, nwords;
+};
+constexpr int kUploadRanges = 6;
+struct UploadBatch {
+  UploadRange r[kUploadRanges];
+  uint32_t count = 0;
+};
+// Adds bytes [zero_bytes, bytes) of h_src (to the same offsets of d_dst)
+// and clears d_dst's first zero_bytes; both must be multiples of 8.
+// Returns -1 for a bad size or a full batch.
+int add_upload(UploadBatch& b, const void* h_src, void* d_dst, size_t zero_bytes, size_t bytes);
+// Launches the batch's ranges (if any) and empties it.
+hipError_t launch_upload(UploadBatch& b, hipStream_t s);
+// Writes a_bytes from device address a, then min(*count, cap) records of
+// rec_bytes from b (b may be null), to the fine-grained pinned block h_dst,
+// then seq to *h_flag once all of it is visible to the host.
+hipError_t launch_publish(const void* a, size_t a_bytes, const void* b, const uint32_t* count,
+                          uint32_t cap, size_t rec_bytes, void* h_dst, uint64_t* h_flag,
+                          uint64_t seq, hipStream_t s);
 hipError_t launch_ptr_to_off(const uint64_t* ptrs, uint64_t base, uint32_t* off,
                              uint64_t n, hipStream_t s);
 hipError_t launch_off_to_ptr(const uint32_t* off, uint64_t base, uint64_t* ptrs,

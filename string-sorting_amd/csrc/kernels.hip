@@ -76,6 +76,43 @@ __global__ void k_off_to_orig_ptr(const uint32_t* __restrict__ off,
   }
 }
 
+// Host-to-device upload of a sort's tables from fine-grained pinned host
+// memory, up to kUploadRanges ranges in one launch (blockIdx.y = range).
+// Words [0, nzero) of a range are cleared, words [nzero, nwords) read from
+// the host with system-scope loads, so a launch always sees what the host
+// wrote before it whatever the GPU caches hold of an earlier upload.  Being
+// a kernel on the sort's own stream, it needs no copy-engine hand-off
+// between the passes.
+__global__ __launch_bounds__(256) void k_upload(UploadBatch b) {
+  const UploadRange r = b.r[blockIdx.y];
+  const uint32_t stride = gridDim.x * 256;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < r.nwords; i += stride)
+    r.dst[i] = i < r.nzero
+                   ? 0ull
+                   : __hip_atomic_load(r.src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Device-to-host hand-over: words [0, na) of a, then the first
+// min(*count, cap) records (rec_words words each) of b, are written to the
+// fine-grained pinned block dst with system-scope stores; once every store
+// has completed, the sequence number goes to *flag.  The host polls the
+// flag instead of waiting for a copy and a stream: both would reach it tens
+// of microseconds later.  One workgroup.
+__global__ __launch_bounds__(1024) void k_publish(const uint64_t* __restrict__ a, uint32_t na,
+                                                  const uint64_t* __restrict__ b,
+                                                  const uint32_t* __restrict__ count, uint32_t cap,
+                                                  uint32_t rec_words, uint64_t* dst,
+                                                  uint64_t* flag, uint64_t seq) {
+  const uint32_t nb = b ? min(*count, cap) * rec_words : 0u;
+  for (uint32_t i = threadIdx.x; i < na + nb; i += 1024) {
+    const uint64_t v = i < na ? a[i] : b[i - na];
+    __hip_atomic_store(dst + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ __launch_bounds__(256) void k_copy_chunks(const CopyChunk* __restrict__ chunks) {
   const CopyChunk c = chunks[blockIdx.x];
   for (uint32_t i = threadIdx.x; i < c.words; i += 256) c.dst[i] = c.src[i];
@@ -962,6 +999,40 @@ hipError_t launch_off_to_orig_ptr(const uint32_t* off, const uint32_t* starts,
                                   const uint64_t* orig, uint64_t* ptrs, uint64_t n,
                                   hipStream_t s) {
   k_off_to_orig_ptr<<<grid_for(n, 256), 256, 0, s>>>(off, starts, orig, ptrs, n);
+  return hipGetLastError();
+}
+
+int add_upload(UploadBatch& b, const void* h_src, void* d_dst, size_t zero_bytes,
+               size_t bytes) {
+  if (bytes % 8 || zero_bytes % 8 || zero_bytes > bytes || bytes / 8 > 0xFFFFFFFFull ||
+      b.count >= (uint32_t)kUploadRanges)
+    return -1;
+  if (!bytes) return 0;
+  b.r[b.count++] = UploadRange{static_cast<const uint64_t*>(h_src), static_cast<uint64_t*>(d_dst),
+                               (uint32_t)(zero_bytes / 8), (uint32_t)(bytes / 8)};
+  return 0;
+}
+
+hipError_t launch_upload(UploadBatch& b, hipStream_t s) {
+  uint32_t most = 0;
+  for (uint32_t i = 0; i < b.count; ++i) most = b.r[i].nwords > most ? b.r[i].nwords : most;
+  const uint32_t count = b.count;
+  b.count = 0;
+  if (!most) return hipSuccess;
+  const uint32_t blocks = most >= 256u * 256u ? 256u : (most + 255) / 256;
+  k_upload<<<dim3(blocks, count), 256, 0, s>>>(b);
+  return hipGetLastError();
+}
+
+hipError_t launch_publish(const void* a, size_t a_bytes, const void* b, const uint32_t* count,
+                          uint32_t cap, size_t rec_bytes, void* h_dst, uint64_t* h_flag,
+                          uint64_t seq, hipStream_t s) {
+  if (a_bytes % 8 || rec_bytes % 8 || a_bytes / 8 + (uint64_t)cap * (rec_bytes / 8) > 0xFFFFFFFFull)
+    return hipErrorInvalidValue;
+  k_publish<<<1, 1024, 0, s>>>(static_cast<const uint64_t*>(a), (uint32_t)(a_bytes / 8),
+                               static_cast<const uint64_t*>(b), count, cap,
+                               (uint32_t)(rec_bytes / 8), static_cast<uint64_t*>(h_dst), h_flag,
+                               seq);
   return hipGetLastError();
 }
 

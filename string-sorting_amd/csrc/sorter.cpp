@@ -21,6 +21,7 @@ namespace gss {
 namespace {
 
 constexpr int kMaxDevices = 16;
+constexpr size_t kPubHead = 256;  // the hand-over block's sequence word, padded
 
 struct Workspace {
   char* base = nullptr;
@@ -35,18 +36,23 @@ struct Workspace {
   BigSeg* h_big = nullptr;
   Tile* h_tiles = nullptr;
   SegAux* h_aux = nullptr;
-  LocalStats* h_lstats = nullptr;
   LocalSeg* h_local = nullptr;  // initial local segments, 3 classes of cap_init
   uint32_t cap_init = 0;
-  // One pinned block per transfer direction of a pass: the segment aux table
-  // and the tile table go up in one copy (they are adjacent on the device
-  // too), the counters and both segment lists come back in one copy.
+  // Fine-grained (coherent) pinned blocks, one per direction of a pass.  Up:
+  // the segment aux table and the tile table, read by one upload launch
+  // (they are adjacent on the device too); h_init and h_local go up the same
+  // way.  Down: the hand-over block, a sequence word (bytes [0, 8)) and from
+  // kPubHead on the counters and the next segment list after a pass, or the
+  // local statistics and the counters at the end, written by the publish
+  // kernel; h_lstats points into it.
   char* h_up = nullptr;
   size_t up_tiles_off = 0;  // byte offset of the tiles behind the aux table
-  char* h_rb = nullptr;
-  size_t rb_bytes = 0;
-  // The start of a sort: zeroed local statistics, the counters and the
-  // initial segment list go up in one copy (adjacent on the device).
+  char* h_pub = nullptr;
+  uint64_t pub_seq = 0;
+  const LocalStats* h_lstats = nullptr;
+  // The start of a sort: the counters and the initial segment list go up in
+  // one upload that also clears the local statistics (adjacent on the
+  // device).
   char* h_init = nullptr;
   size_t init_ctr_off = 0, init_big_off = 0;
 };
@@ -113,6 +119,26 @@ gss_ctx* g_default[kMaxDevices] = {};
     if (e_ != hipSuccess)                                                          \
       return fail(GSS_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+// Waits until the hand-over block's sequence word reads seq.  The stream is
+// queried now and then, so a failed launch ends the wait with its error.
+int wait_published(const Workspace& w, hipStream_t st, uint64_t seq) {
+  const volatile uint64_t* flag = reinterpret_cast<const volatile uint64_t*>(w.h_pub);
+  for (uint32_t spin = 1;; ++spin) {
+    if (*flag == seq) break;
+    if (spin % 1024 == 0) {
+      const hipError_t e = hipStreamQuery(st);
+      if (e == hipSuccess) {
+        if (*flag == seq) break;
+        return fail(GSS_E_INTERNAL, "device hand-over did not arrive");
+      }
+      if (e != hipErrorNotReady)
+        return fail(GSS_E_DEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  return GSS_OK;
+}
 
 int init_context(Context& c) {
   GSS_HIP(hipGetDevice(&c.device));
@@ -198,14 +224,13 @@ void free_workspace(Workspace& w) {
   if (w.base) (void)hipFree(w.base);
   if (w.h_big) (void)hipHostFree(w.h_big);
   if (w.h_up) (void)hipHostFree(w.h_up);
-  if (w.h_rb) (void)hipHostFree(w.h_rb);
-  if (w.h_lstats) (void)hipHostFree(w.h_lstats);
+  if (w.h_pub) (void)hipHostFree(w.h_pub);
   if (w.h_init) (void)hipHostFree(w.h_init);
   w.h_init = nullptr;
   w.base = nullptr;
   w.h_big = nullptr;
   w.h_up = nullptr;
-  w.h_rb = nullptr;
+  w.h_pub = nullptr;
   w.h_tiles = nullptr;
   w.h_aux = nullptr;
   w.h_lstats = nullptr;
@@ -316,18 +341,19 @@ int ensure_workspace(Context& c, uint64_t n, uint32_t rb, uint32_t t) {
   GSS_HIP(hipHostMalloc((void**)&w.h_big, sizeof(BigSeg) * cap_big, hipHostMallocDefault));
   w.up_tiles_off = (size_t)(reinterpret_cast<char*>(p.tiles) - reinterpret_cast<char*>(p.aux));
   GSS_HIP(hipHostMalloc((void**)&w.h_up, w.up_tiles_off + sizeof(Tile) * cap_tiles,
-                        hipHostMallocDefault));
+                        hipHostMallocCoherent));
   w.h_aux = reinterpret_cast<SegAux*>(w.h_up);
   w.h_tiles = reinterpret_cast<Tile*>(w.h_up + w.up_tiles_off);
-  w.rb_bytes = (size_t)(reinterpret_cast<char*>(p.big[1]) - reinterpret_cast<char*>(p.ctr)) +
-               sizeof(BigSeg) * cap_big;
-  GSS_HIP(hipHostMalloc((void**)&w.h_rb, w.rb_bytes, hipHostMallocDefault));
-  GSS_HIP(hipHostMalloc((void**)&w.h_lstats, 2 * sizeof(LocalStats), hipHostMallocDefault));
   w.init_ctr_off = (size_t)(reinterpret_cast<char*>(p.ctr) - reinterpret_cast<char*>(p.lstats));
+  const size_t pub_bytes = kPubHead + std::max(sizeof(Counters) + sizeof(BigSeg) * cap_big,
+                                               w.init_ctr_off + sizeof(Counters));
+  GSS_HIP(hipHostMalloc((void**)&w.h_pub, pub_bytes, hipHostMallocCoherent));
+  memset(w.h_pub, 0, pub_bytes);
+  w.h_lstats = reinterpret_cast<const LocalStats*>(w.h_pub + kPubHead);
   w.init_big_off = (size_t)(reinterpret_cast<char*>(p.big[0]) - reinterpret_cast<char*>(p.lstats));
   const size_t init_bytes = w.init_big_off + sizeof(BigSeg) * cap_big;
-  GSS_HIP(hipHostMalloc((void**)&w.h_init, init_bytes, hipHostMallocDefault));
-  memset(w.h_init, 0, init_bytes);  // the statistics part stays zero
+  GSS_HIP(hipHostMalloc((void**)&w.h_init, init_bytes, hipHostMallocCoherent));
+  memset(w.h_init, 0, init_bytes);  // the statistics part is cleared on the device, never read
   return GSS_OK;
 }
 
@@ -338,7 +364,7 @@ int ensure_init_lists(Workspace& w, uint32_t nreq) {
   const uint32_t cap = std::max<uint32_t>(nreq, 64);
   // Three classes of initial local segments, then the initial retire runs.
   static_assert(sizeof(Retire) <= 2 * sizeof(LocalSeg), "layout of the initial lists");
-  GSS_HIP(hipHostMalloc((void**)&w.h_local, 5 * sizeof(LocalSeg) * cap, hipHostMallocDefault));
+  GSS_HIP(hipHostMalloc((void**)&w.h_local, 5 * sizeof(LocalSeg) * cap, hipHostMallocCoherent));
   w.cap_init = cap;
   return GSS_OK;
 }
@@ -426,18 +452,24 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
       w.h_big[nbig++] = BigSeg{b, size, 0, 0};
     }
   }
+  // The start of the sort goes up in one launch, together with the first
+  // pass's tiles when no early local launch needs the lists before them:
+  // the initial local lists, the retire runs, and (one range) cleared
+  // statistics, the counters and the big segments.
+  UploadBatch up;
+  auto add = [&](const void* src, void* dst, size_t zero_bytes, size_t bytes) -> int {
+    if (add_upload(up, src, dst, zero_bytes, bytes))
+      return fail(GSS_E_INTERNAL, "upload batch: bad range");
+    return GSS_OK;
+  };
   for (int cls = 0; cls < 3; ++cls)
-    if (h0.local[cls])
-      GSS_HIP(hipMemcpyAsync(p.local[cls], init[cls], sizeof(LocalSeg) * h0.local[cls],
-                             hipMemcpyHostToDevice, st));
-  if (h0.retire)
-    GSS_HIP(hipMemcpyAsync(p.retire, init_retire, sizeof(Retire) * h0.retire,
-                           hipMemcpyHostToDevice, st));
-  // Zeroed statistics, the counters and the big segments in one copy.
+    if ((rc = add(init[cls], p.local[cls], 0, sizeof(LocalSeg) * h0.local[cls]))) return rc;
+  if ((rc = add(init_retire, p.retire, 0, sizeof(Retire) * h0.retire))) return rc;
   memcpy(w.h_init + w.init_ctr_off, &h0, sizeof(Counters));
   memcpy(w.h_init + w.init_big_off, w.h_big, sizeof(BigSeg) * nbig);
-  GSS_HIP(hipMemcpyAsync(p.lstats, w.h_init, w.init_big_off + sizeof(BigSeg) * nbig,
-                         hipMemcpyHostToDevice, st));
+  if ((rc = add(w.h_init, p.lstats, w.init_ctr_off, w.init_big_off + sizeof(BigSeg) * nbig)))
+    return rc;
+  if (!nbig) GSS_HIP(launch_upload(up, st));
   // Local segments launched early run on the side stream with their own
   // statistics block; the side stream first waits for the uploads above.
   Plan p_side = p;
@@ -452,7 +484,9 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     bool any = false;
     for (int cls = 0; cls < 3; ++cls) any = any || upto[cls] > launched[cls];
     if (!any) return GSS_OK;
-    // The side stream waits for the work queued on the main stream so far.
+    // The side stream waits for the work queued on the main stream so far,
+    // the uploads still held back included.
+    GSS_HIP(launch_upload(up, st));
     GSS_HIP(hipEventRecord(c.side_done, st));
     GSS_HIP(hipStreamWaitEvent(c.side, c.side_done, 0));
     side_used = true;
@@ -500,8 +534,8 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
         w.h_aux[i] = SegAux{0, ntiles - first, 0, first};
       }
       if (pass < GSS_MAX_PASSES) stats.active[pass] = active;
-      GSS_HIP(hipMemcpyAsync(p.aux, w.h_up, w.up_tiles_off + sizeof(Tile) * ntiles,
-                             hipMemcpyHostToDevice, st));
+      if ((rc = add(w.h_up, p.aux, 0, w.up_tiles_off + sizeof(Tile) * ntiles))) return rc;
+      GSS_HIP(launch_upload(up, st));
       const BigSeg* segs = p.big[cur];
       BigSeg* next = p.big[cur ^ 1];
       static const char* kPassNames[10] = {"pass0", "pass1", "pass2", "pass3", "pass4",
@@ -522,15 +556,18 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
       GSS_LAUNCH("scatter", active, launch_scatter(p, segs, ntiles, rb, st));
       if (lcp) GSS_LAUNCH("boundary", 0, launch_boundary_lcp(p, bound_first, st));
       prof_end(c, pass_tok, st);
-      // Counters and the next segment list in one copy (they are adjacent).
-      const size_t next_off =
-          (size_t)(reinterpret_cast<char*>(next) - reinterpret_cast<char*>(p.ctr));
-      GSS_HIP(hipMemcpyAsync(w.h_rb, p.ctr, next_off + sizeof(BigSeg) * p.cap_big,
-                             hipMemcpyDeviceToHost, st));
-      GSS_HIP(hipStreamSynchronize(st));
-      memcpy(w.h_ctr, w.h_rb, sizeof(Counters));
+      // The counters and the used part of the next segment list come back
+      // through the hand-over block.
+      const uint64_t seq = ++w.pub_seq;
+      GSS_HIP(launch_publish(p.ctr, sizeof(Counters), next, &p.ctr->big_next, p.cap_big,
+                             sizeof(BigSeg), w.h_pub + kPubHead,
+                             reinterpret_cast<uint64_t*>(w.h_pub), seq, st));
+      rc = wait_published(w, st, seq);
+      if (rc) return rc;
+      memcpy(w.h_ctr, w.h_pub + kPubHead, sizeof(Counters));
       if (w.h_ctr->big_next <= p.cap_big)
-        memcpy(w.h_big, w.h_rb + next_off, sizeof(BigSeg) * w.h_ctr->big_next);
+        memcpy(w.h_big, w.h_pub + kPubHead + sizeof(Counters),
+               sizeof(BigSeg) * w.h_ctr->big_next);
       if (w.h_ctr->error)
         return fail(GSS_E_INTERNAL, "device pass failed (code " +
                                         std::to_string(w.h_ctr->error) + ")");
@@ -540,14 +577,17 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
       ++pass;
       if (nbig) {
         // More passes follow.  The local segments found so far are sorted on
-        // the side stream meanwhile when they hold at least as many strings
-        // as the next pass: a heavier pass would only contend with them for
-        // memory bandwidth.
+        // the side stream meanwhile when they hold at least half as many
+        // strings as the next pass.  Launched against a heavier pass they
+        // hold its workgroups off the CUs (one local workgroup fills a CU's
+        // LDS): on 10M URL strings, launching 2.9M local strings against a
+        // 7.1M-string pass made the sort 19% slower, while 100M Zipf words
+        // gain 3% from the half rule over an equal-size rule.
         uint64_t next_active = 0, waiting = 0;
         for (uint32_t i = 0; i < nbig; ++i) next_active += w.h_big[i].size;
         for (int cls = 0; cls < 3; ++cls) waiting += w.h_ctr->local_strings[cls];
         waiting -= early_strings;
-        if (waiting >= next_active) {
+        if (2 * waiting >= next_active) {
           const uint32_t upto[3] = {w.h_ctr->local[0], w.h_ctr->local[1], w.h_ctr->local[2]};
           const int erc = launch_early(upto);
           if (erc) return erc;
@@ -586,10 +626,15 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     GSS_HIP(hipEventRecord(c.side_done, c.side));
     GSS_HIP(hipStreamWaitEvent(st, c.side_done, 0));
   }
-  GSS_HIP(hipMemcpyAsync(w.h_ctr, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
-  GSS_HIP(hipMemcpyAsync(w.h_lstats, p.lstats, 2 * sizeof(LocalStats), hipMemcpyDeviceToHost,
-                         st));
-  GSS_HIP(hipStreamSynchronize(st));
+  // The local statistics and the counters behind them come back together.
+  const uint64_t seq = ++w.pub_seq;
+  GSS_HIP(launch_publish(p.lstats, w.init_ctr_off + sizeof(Counters), nullptr, nullptr, 0, 8,
+                         w.h_pub + kPubHead, reinterpret_cast<uint64_t*>(w.h_pub), seq, st));
+  rc = wait_published(w, st, seq);
+  if (rc) return rc;
+  // Event brackets are read once their stream has passed them.
+  if (!c.pending.empty()) GSS_HIP(hipStreamSynchronize(st));
+  memcpy(w.h_ctr, w.h_pub + kPubHead + w.init_ctr_off, sizeof(Counters));
   // A local launch's unit of work is one active string in one key round;
   // block 0 of the statistics holds the final launches, block 1 the early ones.
   // The profiled unit of a local launch is one string through one 8-byte key
