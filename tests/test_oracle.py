@@ -256,7 +256,9 @@ def test_reference_sortstring_equals_oracle_pipeline(mode, tmp_path, oracle_mod)
     out = tmp_path / "out.txt"
     args = [str(SORTSTRING), f"--write={out}"]
     args += {"plain": [], "raw": ["--raw"], "suffix": ["--suffix-sorting"]}[mode]
-    subprocess.run(args + ["msd_CE2", str(f)], check=True, capture_output=True, timeout=120)
+    # sortstring appends to ./sortstring_log: run it in the test's directory
+    subprocess.run(args + ["msd_CE2", str(f)], check=True, capture_output=True, timeout=120,
+                   cwd=tmp_path)
     text = np.frombuffer(data + b"\0" * 16, np.uint8).copy()
     if mode == "suffix":
         off = np.arange(len(data), dtype=np.uint32)
