@@ -18,6 +18,11 @@ int fail(int code, const std::string& msg);
 int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
                   uint32_t* out, uint64_t* lcp, uint64_t n, hipStream_t stream);
 
+// Waits for everything queued on `st` by polling it.  A blocking wait
+// sleeps once the stream has been busy for a while and wakes tens of
+// microseconds after the GPU finished, which a request's latency would pay.
+int wait_stream(hipStream_t st);
+
 // The default context's stream on the current device (created on first use).
 int library_stream(hipStream_t* out);
 

@@ -149,16 +149,17 @@ int run_batch(gss_server* s, Slot& sl) {
     }
   }
   if (g > sl.chunk_cap) return fail(GSS_E_INTERNAL, "batch chunk table overflow");
-  SRV_HIP(hipMemcpyAsync(sl.d_chunks, sl.h_chunks, sizeof(CopyChunk) * g, hipMemcpyHostToDevice,
-                         sl.stream));
+  UploadBatch up;  // the chunk table, read by a kernel from coherent pinned memory
+  if (add_upload(up, sl.h_chunks, sl.d_chunks, 0, sizeof(CopyChunk) * g))
+    return fail(GSS_E_INTERNAL, "batch chunk table upload");
+  SRV_HIP(launch_upload(up, sl.stream));
   SRV_HIP(launch_copy_chunks(sl.d_chunks, (uint32_t)ngather, sl.stream));
   const int rc = gss_ctx_sort_batch_device32(sl.ctx, s->text, s->text_bytes, sl.d_in, sl.d_out,
                                              want_lcp ? sl.d_lcp : nullptr, sl.req.data(),
                                              (uint32_t)sl.batch.size(), sl.stream);
   if (rc) return rc;
   SRV_HIP(launch_copy_chunks(sl.d_chunks + ngather, (uint32_t)(g - ngather), sl.stream));
-  SRV_HIP(hipStreamSynchronize(sl.stream));
-  return GSS_OK;
+  return wait_stream(sl.stream);
 }
 
 void complete(gss_server* s, Slot& sl, int rc) {
@@ -240,7 +241,7 @@ int init_slot(Slot& sl, uint64_t cap, uint32_t max_reqs) {
   p += align256(8 * cap);
   sl.d_chunks = reinterpret_cast<CopyChunk*>(p);
   SRV_HIP(hipHostMalloc((void**)&sl.h_chunks, sizeof(CopyChunk) * sl.chunk_cap,
-                        hipHostMallocDefault));
+                        hipHostMallocCoherent));
   sl.req.reserve(max_reqs + 1);
   sl.batch.reserve(max_reqs);
   return GSS_OK;

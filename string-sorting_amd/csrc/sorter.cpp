@@ -682,6 +682,15 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+int wait_stream(hipStream_t st) {
+  for (;;) {
+    const hipError_t e = hipStreamQuery(st);
+    if (e == hipSuccess) return GSS_OK;
+    if (e != hipErrorNotReady)
+      return fail(GSS_E_DEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+  }
+}
+
 int library_stream(hipStream_t* out) {
   Context* c = nullptr;
   const int rc = default_context(&c);
