@@ -625,7 +625,7 @@ def main():
         gss.lib().gss_msd(arr, ss.n)
         te = time.perf_counter() - t0
         e2e_ok = bool(np.array_equal((ptrs - np.uint64(base)).astype(np.uint32), got))
-        res["e2e"] = {"entry": "gss_msd (host pointers, H2D text+pointers, D2H pointers)",
+        res["e2e"] = {"entry": "gss_msd (host pointers: H2D text + 32-bit offsets, D2H offsets widened to pointers on the host)",
                       "seconds": te, "mstrings_per_s": ss.n / te / 1e6, "matches_device": e2e_ok}
 
     if not args.no_pipeline:

@@ -413,10 +413,6 @@ hipError_t launch_upload(UploadBatch& b, hipStream_t s);
 hipError_t launch_publish(const void* a, size_t a_bytes, const void* b, const uint32_t* count,
                           uint32_t cap, size_t rec_bytes, void* h_dst, uint64_t* h_flag,
                           uint64_t seq, hipStream_t s);
-hipError_t launch_ptr_to_off(const uint64_t* ptrs, uint64_t base, uint32_t* off,
-                             uint64_t n, hipStream_t s);
-hipError_t launch_off_to_ptr(const uint32_t* off, uint64_t base, uint64_t* ptrs,
-                             uint64_t n, hipStream_t s);
 hipError_t launch_off_to_orig_ptr(const uint32_t* off, const uint32_t* starts,
                                   const uint64_t* orig, uint64_t* ptrs, uint64_t n,
                                   hipStream_t s);

@@ -142,6 +142,23 @@ void par_memcpy(void* dst, const void* src, uint64_t bytes) {
   });
 }
 
+// The 32-bit distances of n pointers from base, split over the pool.
+void par_offsets(uint32_t* dst, unsigned char* const* ptrs, uint64_t n, uintptr_t base) {
+  host_pool().run([&](int w, int nw) {
+    const uint64_t a = n * (uint64_t)w / nw, b = n * (uint64_t)(w + 1) / nw;
+    for (uint64_t i = a; i < b; ++i)
+      dst[i] = (uint32_t)(reinterpret_cast<uintptr_t>(ptrs[i]) - base);
+  });
+}
+
+// base + off[i] for n 32-bit offsets, split over the pool.
+void par_pointers(unsigned char** dst, const uint32_t* off, uint64_t n, uintptr_t base) {
+  host_pool().run([&](int w, int nw) {
+    const uint64_t a = n * (uint64_t)w / nw, b = n * (uint64_t)(w + 1) / nw;
+    for (uint64_t i = a; i < b; ++i) dst[i] = reinterpret_cast<unsigned char*>(base + off[i]);
+  });
+}
+
 // Copies between the caller's pageable memory and the device through a ring
 // of pinned chunks: host threads fill (or drain) one chunk while the DMA of
 // the others runs.  Pageable copies go through the runtime's own staging at
@@ -168,6 +185,41 @@ class Staging {
             "hipMemcpy(H2D)");
       arm(slot, st, nullptr, 0);
     }
+  }
+
+  // Uploads the 32-bit distances from `base` of n string pointers: the host
+  // threads compute them into the pinned chunks (half the bytes of the
+  // pointers cross the link).
+  void h2d_offsets(uint32_t* dst, unsigned char* const* ptrs, uint64_t n, uintptr_t base,
+                   hipStream_t st) {
+    ensure();
+    constexpr uint64_t kPer = kChunk / 4;
+    int slot = 0;
+    for (uint64_t i = 0; i < n; i += kPer, slot = (slot + 1) % kSlots) {
+      const uint64_t m = std::min(kPer, n - i);
+      settle(slot);
+      par_offsets(reinterpret_cast<uint32_t*>(buf_ + slot * kChunk), ptrs + i, m, base);
+      check(hipMemcpyAsync(dst + i, buf_ + slot * kChunk, 4 * m, hipMemcpyHostToDevice, st),
+            "hipMemcpy(H2D)");
+      arm(slot, st, nullptr, 0);
+    }
+  }
+
+  // Downloads n 32-bit offsets and stores base + offset into ptrs; returns
+  // once all of them are there.
+  void d2h_pointers(unsigned char** ptrs, const uint32_t* src, uint64_t n, uintptr_t base,
+                    hipStream_t st) {
+    ensure();
+    constexpr uint64_t kPer = kChunk / 4;
+    int slot = 0;
+    for (uint64_t i = 0; i < n; i += kPer, slot = (slot + 1) % kSlots) {
+      const uint64_t m = std::min(kPer, n - i);
+      settle(slot);
+      check(hipMemcpyAsync(buf_ + slot * kChunk, src + i, 4 * m, hipMemcpyDeviceToHost, st),
+            "hipMemcpy(D2H)");
+      arm(slot, st, ptrs + i, m, base, true);
+    }
+    for (int q = 0; q < kSlots; ++q) settle((slot + q) % kSlots);  // drain in order
   }
 
   // Returns once the data is in `dst` (the stream's earlier work included).
@@ -203,20 +255,32 @@ class Staging {
     if (!armed_[slot]) return;
     check(hipEventSynchronize(ev_[slot]), "hipEventSynchronize");
     armed_[slot] = false;
-    if (out_[slot]) par_memcpy(out_[slot], buf_ + slot * kChunk, out_len_[slot]);
+    if (out_[slot] && widen_[slot])
+      par_pointers(static_cast<unsigned char**>(out_[slot]),
+                   reinterpret_cast<const uint32_t*>(buf_ + slot * kChunk), out_len_[slot],
+                   base_[slot]);
+    else if (out_[slot])
+      par_memcpy(out_[slot], buf_ + slot * kChunk, out_len_[slot]);
     out_[slot] = nullptr;
   }
-  void arm(int slot, hipStream_t st, void* out, uint64_t len) {
+  // out/len: where the chunk goes once its copy is done (bytes, or offsets
+  // to widen into pointers from base when widen is set).
+  void arm(int slot, hipStream_t st, void* out, uint64_t len, uintptr_t base = 0,
+           bool widen = false) {
     check(hipEventRecord(ev_[slot], st), "hipEventRecord");
     armed_[slot] = true;
     out_[slot] = out;
     out_len_[slot] = len;
+    base_[slot] = base;
+    widen_[slot] = widen;
   }
   unsigned char* buf_ = nullptr;
   hipEvent_t ev_[kSlots] = {};
   bool armed_[kSlots] = {};
   void* out_[kSlots] = {};
   uint64_t out_len_[kSlots] = {};
+  uintptr_t base_[kSlots] = {};
+  bool widen_[kSlots] = {};
 };
 
 Staging g_staging;
@@ -304,8 +368,7 @@ void sort_pointers(unsigned char** strings, size_t n, size_t* lcp_out) {
     text_len = span;
     ensure_text(text_len + 64);
     g_staging.h2d(g_buf.text, reinterpret_cast<const void*>(lo), span, st);
-    g_staging.h2d(g_buf.ptrs, strings, 8 * n, st);
-    check(launch_ptr_to_off(g_buf.ptrs, lo, g_buf.off, n, st), "ptr_to_off");
+    g_staging.h2d_offsets(g_buf.off, strings, n, lo, st);
   } else {
     std::vector<uint32_t> starts(n);
     uint64_t total = 0;
@@ -339,15 +402,13 @@ void sort_pointers(unsigned char** strings, size_t n, size_t* lcp_out) {
                                lcp_out ? g_buf.lcp : nullptr, n, st);
   if (rc) die(std::string("device sort failed: ") + gss_last_error());
 
-  uint64_t* result = g_buf.ptrs;
-  if (use_span) {
-    check(launch_off_to_ptr(g_buf.out, lo, g_buf.ptrs, n, st), "off_to_ptr");
+  if (use_span) {  // the sorted offsets come back and are widened on the host
+    g_staging.d2h_pointers(strings, g_buf.out, n, lo, st);
   } else {
     check(launch_off_to_orig_ptr(g_buf.out, g_buf.off, g_buf.ptrs, g_buf.ptrs2, n, st),
           "off_to_orig_ptr");
-    result = g_buf.ptrs2;
+    g_staging.d2h(strings, g_buf.ptrs2, 8 * n, st);
   }
-  g_staging.d2h(strings, result, 8 * n, st);
   if (lcp_out) g_staging.d2h(lcp_out, g_buf.lcp, 8 * (n - 1), st);
 }
 

@@ -44,20 +44,6 @@ namespace gss {
 // ---------------------------------------------------------------------------
 // Small helpers
 
-__global__ void k_ptr_to_off(const uint64_t* __restrict__ ptrs, uint64_t base,
-                             uint32_t* __restrict__ off, uint64_t n) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    off[i] = (uint32_t)(ptrs[i] - base);
-}
-
-__global__ void k_off_to_ptr(const uint32_t* __restrict__ off, uint64_t base,
-                             uint64_t* __restrict__ ptrs, uint64_t n) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    ptrs[i] = base + off[i];
-}
-
 // Packed texts: offsets were assigned in input order, so the input index of
 // a sorted offset is found by binary search in the ascending start array.
 __global__ void k_off_to_orig_ptr(const uint32_t* __restrict__ off,
@@ -980,18 +966,6 @@ hipError_t launch_boundary_lcp(const Plan& p, uint32_t first, hipStream_t s) {
 hipError_t launch_retire(const Plan& p, uint32_t nretire, hipStream_t s) {
   if (!nretire) return hipSuccess;
   k_retire<<<grid_for(nretire, 1, 8192), 256, 0, s>>>(p, nretire);
-  return hipGetLastError();
-}
-
-hipError_t launch_ptr_to_off(const uint64_t* ptrs, uint64_t base, uint32_t* off, uint64_t n,
-                             hipStream_t s) {
-  k_ptr_to_off<<<grid_for(n, 256), 256, 0, s>>>(ptrs, base, off, n);
-  return hipGetLastError();
-}
-
-hipError_t launch_off_to_ptr(const uint32_t* off, uint64_t base, uint64_t* ptrs, uint64_t n,
-                             hipStream_t s) {
-  k_off_to_ptr<<<grid_for(n, 256), 256, 0, s>>>(off, base, ptrs, n);
   return hipGetLastError();
 }
 
