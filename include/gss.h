@@ -60,6 +60,8 @@ void gss_msd_lcp(unsigned char **strings, size_t n, size_t *lcp);
  *
  * Returns 0 on success, a negative GSS_E* code otherwise (gss_last_error()
  * explains).  The call returns once the result is complete on the stream.
+ * The calling thread polls while the passes run (the host decides each
+ * pass from the previous one's counters): it busy-waits rather than sleeps.
  */
 int gss_sort_device32(const unsigned char *d_text, uint64_t text_bytes,
 		      const uint32_t *d_off_in, uint32_t *d_off_out,

@@ -112,3 +112,29 @@ def test_contexts_run_concurrently(oracle_mod):
     for c in ctxs:
         c.close()
     torch.cuda.synchronize()
+
+
+def test_interleaved_shapes_on_one_context(oracle_mod):
+    """Single sorts and batches of different shapes back to back on the
+    default context: every call re-fills the staged tables and hands its
+    counters back through the same host block, so a stale table or a
+    hand-over read too early would show up as a wrong order here."""
+    plan = [("one", 0, 70000), ("batch", [1, 2, 65, 7000, 1]), ("one", 1, 5000),
+            ("batch", [30000, 0, 12, 6145]), ("one", 2, 2), ("one", 3, 90000),
+            ("batch", [1] * 9 + [40000]), ("one", 1, 1025), ("batch", [64, 64, 9000, 3]),
+            ("one", 0, 6144), ("one", 2, 50000), ("batch", [20000, 20000])]
+    for step, item in enumerate(plan):
+        if item[0] == "one":
+            _, cfg, n = item
+            ss = gen.make(cfg, n=n, seed=100 + step)
+            log(f"interleaved step {step}: {gen.NAMES[cfg]} n={n}")
+            off, lcp = gpu_sort(ss)
+            want, want_lcp = oracle_mod.sort_offsets(ss.text, ss.offsets, "msd_ce2", lcp=True)
+            assert np.array_equal(off, want), f"step {step}"
+            assert np.array_equal(lcp, want_lcp), f"step {step} LCP"
+        else:
+            sets = _mixed_requests(200 + step, item[1])
+            joined, req = gen.concat(sets)
+            log(f"interleaved step {step}: batch of {len(sets)}, {joined.n} strings")
+            o, l = _batch_sort(joined, req)
+            _check_requests(joined, req, sets, o, l, oracle_mod)
