@@ -83,20 +83,34 @@ __global__ __launch_bounds__(256) void k_upload(UploadBatch b) {
 // fine-grained pinned block dst with system-scope stores; once every store
 // has completed, the sequence number goes to *flag.  The host polls the
 // flag instead of waiting for a copy and a stream: both would reach it tens
-// of microseconds later.  One workgroup.
-__global__ __launch_bounds__(1024) void k_publish(const uint64_t* __restrict__ a, uint32_t na,
-                                                  const uint64_t* __restrict__ b,
-                                                  const uint32_t* __restrict__ count, uint32_t cap,
-                                                  uint32_t rec_words, uint64_t* dst,
-                                                  uint64_t* flag, uint64_t seq) {
-  const uint32_t nb = b ? min(*count, cap) * rec_words : 0u;
-  for (uint32_t i = threadIdx.x; i < na + nb; i += 1024) {
-    const uint64_t v = i < na ? a[i] : b[i - na];
-    __hip_atomic_store(dst + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+// of microseconds later.  Each lane loads kPublishWords words before it
+// stores any (one memory round trip per 4096 words), each wave's release
+// fence waits for its own stores, and the barrier puts the flag after all.
+constexpr int kPublishThreads = 256;
+constexpr int kPublishWords = 16;
+
+__global__ __launch_bounds__(kPublishThreads) void k_publish(
+    const uint64_t* __restrict__ a, uint32_t na, const uint64_t* __restrict__ b,
+    const uint32_t* __restrict__ count, uint32_t cap, uint32_t rec_words, uint64_t* dst,
+    uint64_t* flag, uint64_t seq) {
+  const uint32_t total = na + (b ? min(*count, cap) * rec_words : 0u);
+  for (uint32_t i0 = 0; i0 < total; i0 += kPublishThreads * kPublishWords) {
+    uint64_t v[kPublishWords];
+#pragma unroll
+    for (int q = 0; q < kPublishWords; ++q) {
+      // Clamped to the last word, so every load is issued unconditionally.
+      const uint32_t i = min(i0 + q * kPublishThreads + threadIdx.x, total - 1);
+      v[q] = i < na ? a[i] : b[i - na];
+    }
+#pragma unroll
+    for (int q = 0; q < kPublishWords; ++q) {
+      const uint32_t i = i0 + q * kPublishThreads + threadIdx.x;
+      if (i < total) __hip_atomic_store(dst + i, v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
-  __threadfence_system();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
   __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  if (threadIdx.x == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 __global__ __launch_bounds__(256) void k_copy_chunks(const CopyChunk* __restrict__ chunks) {
@@ -1003,7 +1017,7 @@ hipError_t launch_publish(const void* a, size_t a_bytes, const void* b, const ui
                           uint64_t seq, hipStream_t s) {
   if (a_bytes % 8 || rec_bytes % 8 || a_bytes / 8 + (uint64_t)cap * (rec_bytes / 8) > 0xFFFFFFFFull)
     return hipErrorInvalidValue;
-  k_publish<<<1, 1024, 0, s>>>(static_cast<const uint64_t*>(a), (uint32_t)(a_bytes / 8),
+  k_publish<<<1, kPublishThreads, 0, s>>>(static_cast<const uint64_t*>(a), (uint32_t)(a_bytes / 8),
                                static_cast<const uint64_t*>(b), count, cap,
                                (uint32_t)(rec_bytes / 8), static_cast<uint64_t*>(h_dst), h_flag,
                                seq);
