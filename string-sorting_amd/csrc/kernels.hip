@@ -386,17 +386,23 @@ __global__ __launch_bounds__(kTileThreads) void k_histogram(Plan p) {
   __shared__ uint32_t h[NB];
   __shared__ uint8_t rank[256];
   const Tile t = p.tiles[xcd_tile(blockIdx.x, gridDim.x)];
-  const uint64_t mn = p.segmm[2 * t.seg], mx = p.segmm[2 * t.seg + 1];
-  if (mn == mx) return;
-  const DigitSpec ds = digit_spec<RB>(mn, mx, build_rank(p.alpha, rank));
-  // All keys are loaded before the first is used (slots past the tile read
-  // its last key and are not counted), so the loads overlap.
+  // Everything that needs only the tile is loaded before the segment's
+  // min/max can end the block (a load after that return cannot be hoisted
+  // above it): the keys (slots past the tile read its last key and are not
+  // counted), the alphabet and the segment's count-matrix place.
   uint64_t key[kTileItems];
 #pragma unroll
   for (int j = 0; j < kTileItems; ++j) {
     const uint32_t i = j * kTileThreads + threadIdx.x;
     key[j] = p.keys[t.begin + (i < t.len ? i : t.len - 1)];
   }
+  uint32_t alpha[8];
+#pragma unroll
+  for (int w = 0; w < 8; ++w) alpha[w] = p.alpha[w];
+  const SegAux ax = p.aux[t.seg];
+  const uint64_t mn = p.segmm[2 * t.seg], mx = p.segmm[2 * t.seg + 1];
+  if (mn == mx) return;
+  const DigitSpec ds = digit_spec<RB>(mn, mx, build_rank(alpha, rank));
   for (uint32_t d = threadIdx.x; d < ds.nbu; d += kTileThreads) h[d] = 0;
   __syncthreads();
 #pragma unroll
@@ -405,7 +411,6 @@ __global__ __launch_bounds__(kTileThreads) void k_histogram(Plan p) {
     if (i < t.len) atomicAdd(&h[digit_of(key[j], ds, rank)], 1u);
   }
   __syncthreads();
-  const SegAux ax = p.aux[t.seg];
   uint32_t* col = p.flat + ax.flat_base + t.tidx;
   for (uint32_t d = threadIdx.x; d < ds.nbu; d += kTileThreads) col[(uint64_t)d * ax.ntiles] = h[d];
 }
@@ -552,6 +557,12 @@ __global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, Big
   constexpr uint32_t DPT = NB / 256;
   __shared__ uint32_t s_first;
   const uint32_t si = blockIdx.x;
+  // The segment's count-matrix place and the alphabet are loaded with its
+  // bounds, before its min/max can end the block.
+  const SegAux ax = p.aux[si];
+  uint32_t alpha[8];
+#pragma unroll
+  for (int w = 0; w < 8; ++w) alpha[w] = p.alpha[w];
   const BigSeg sg = segs[si];
   const uint64_t mn = p.segmm[2 * si], mx = p.segmm[2 * si + 1];
   if (mn == mx) {
@@ -564,10 +575,9 @@ __global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, Big
     return;
   }
   const uint32_t nbuf = sg.buf == 1 ? 2u : 1u;
-  const bool has_nul = p.alpha[0] & 1u;  // NUL, if present, has rank 0
-  const DigitSpec ds = digit_spec<RB>(mn, mx, alpha_size(p.alpha));
+  const bool has_nul = alpha[0] & 1u;  // NUL, if present, has rank 0
+  const DigitSpec ds = digit_spec<RB>(mn, mx, alpha_size(alpha));
   const uint32_t fb = ds.fb0 + ds.k;  // bytes every child shares in full
-  const SegAux ax = p.aux[si];
   const uint32_t* col0 = p.flat + ax.flat_base;
   // Bucket of digit d: [start(d), start(d+1)), start(nbu) = segment size.
   // The DPT + 1 starts a thread needs are loaded before any is used (clamped
@@ -733,9 +743,30 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
   __shared__ uint8_t rank[256];
 
   const Tile t = p.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const bool carry = !p.keys_carried;  // hand the next pass its keys
+  const int wid = threadIdx.x / kWave;
+  const int lane = lane_id();
+  // Everything that needs only the tile is loaded before the segment's
+  // min/max can end the block (a load after that return cannot be hoisted
+  // above it): every item's key and next 8 bytes (slots past the tile read
+  // its last string and are masked out), the alphabet and the segment's
+  // count-matrix place.  The offsets follow once the segment names the
+  // buffer that holds them.
+  uint64_t key[kTileItems];
+  uint64_t nk[kTileItems];  // key of the string at its child's depth (carry)
+#pragma unroll
+  for (int j = 0; j < kTileItems; ++j) {
+    const uint32_t i = wid * IPW + j * kWave + lane;
+    const uint32_t at = t.begin + (i < t.len ? i : t.len - 1);
+    key[j] = p.keys[at];
+    nk[j] = carry ? p.ext[at] : 0ull;
+  }
+  uint32_t alpha[8];
+#pragma unroll
+  for (int w = 0; w < 8; ++w) alpha[w] = p.alpha[w];
+  const SegAux ax = p.aux[t.seg];
   const BigSeg sg = segs[t.seg];
   const uint64_t mn = p.segmm[2 * t.seg], mx = p.segmm[2 * t.seg + 1];
-  const bool carry = !p.keys_carried;  // hand the next pass its keys
   if (mn == mx) {
     // The whole segment stays where it is and, unless its strings ended,
     // goes 8 bytes deeper: its next keys are the 8 bytes after these.
@@ -744,33 +775,25 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
         p.keys_next[t.begin + i] = p.ext[t.begin + i];
     return;
   }
-  const DigitSpec ds = digit_spec<RB>(mn, mx, build_rank(p.alpha, rank));
+  const DigitSpec ds = digit_spec<RB>(mn, mx, build_rank(alpha, rank));
   const uint32_t nbits = digit_bits(ds.nbu);
   const uint32_t fb = ds.fb0 + ds.k;  // bytes the children share: their keys start there
   const uint32_t* __restrict__ src = p.idx[sg.buf];
   uint32_t* __restrict__ dst = p.idx[sg.buf == 1 ? 2 : 1];
+  uint32_t off[kTileItems];
+#pragma unroll
+  for (int j = 0; j < kTileItems; ++j) {
+    const uint32_t i = wid * IPW + j * kWave + lane;
+    off[j] = src[t.begin + (i < t.len ? i : t.len - 1)];
+  }
 
   static_assert(W == 8, "one 16-byte row of wave counters per digit");
   uint4* const rows = reinterpret_cast<uint4*>(&s_whist[0][0]);
   for (uint32_t d = threadIdx.x; d < NB; d += kTileThreads) rows[d] = make_uint4(0, 0, 0, 0);
   __syncthreads();
 
-  const int wid = threadIdx.x / kWave;
-  const int lane = lane_id();
   const uint64_t lt = lanemask_lt();
-  uint32_t off[kTileItems], dig[kTileItems], rnk[kTileItems];
-  uint64_t nk[kTileItems];  // key of the string at its child's depth (carry)
-  // Every item's loads are issued before any is used (slots past the tile
-  // read the tile's last string and are masked out), so they overlap.
-  uint64_t key[kTileItems];
-#pragma unroll
-  for (int j = 0; j < kTileItems; ++j) {
-    const uint32_t i = wid * IPW + j * kWave + lane;
-    const uint32_t at = t.begin + (i < t.len ? i : t.len - 1);
-    off[j] = src[at];
-    key[j] = p.keys[at];
-    nk[j] = carry ? p.ext[at] : 0ull;
-  }
+  uint32_t dig[kTileItems], rnk[kTileItems];
 #pragma unroll
   for (int j = 0; j < kTileItems; ++j) {
     dig[j] = digit_of(key[j], ds, rank);
@@ -794,7 +817,6 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
 
   // This tile's place in each digit's bucket and the bucket's kind, loaded
   // before the scan below so that the strided reads overlap it.
-  const SegAux ax = p.aux[t.seg];
   const uint32_t* col = p.flat + ax.flat_base;
   const uint32_t* bs = p.bstart + (uint64_t)t.seg * NB;
   const uint32_t base0 = col[0];
