@@ -15,7 +15,8 @@
 //   3. sort    every sub-segment by its chosen key, stably: LSD radix passes
 //              over the key bytes that vary inside some sub-segment, then
 //              over the sub-segment id (wave match-ranking with per-wave LDS
-//              digit counters); when no sub-segment exceeds 16 strings each is
+//              digit counters); when no sub-segment exceeds insertion_max
+//              strings (16, or 64 in the largest class) each is
 //              insertion-sorted by one thread;
 //   4. split   into equal-key groups.  A singleton, or a group whose key holds
 //              the NUL (identical strings), is final -- the retirement rule
@@ -34,6 +35,16 @@ namespace gss {
 
 constexpr int kLocalWords = 4;
 
+// Largest group a round still insertion-sorts (one thread per group, the
+// groups side by side) instead of running the LSD passes over all items.
+// 16 in the small classes, whose few threads would wait on one long
+// insertion sort; 64 in the 1024-thread class, where hundreds of groups sort
+// at once and one 64-string insertion sort (about 1,000 dependent LDS steps)
+// costs about what the passes would, so no round gets slower than with the
+// passes.  On 10M URL strings 1.78 -> 1.71 ms, the other workloads
+// unchanged; 96 and 128 gained nothing more.
+constexpr uint32_t insertion_max(uint32_t cap) { return cap > kLocalClassCap[1] ? 64u : 16u; }
+
 template <int BLOCK, int CAP>
 struct LocalSmem {
   static constexpr int W = BLOCK / kWave;
@@ -47,7 +58,7 @@ struct LocalSmem {
   uint8_t ss_stop[(SS + 3) & ~3];  // bit q: key q varies or holds a NUL in the group
   alignas(8) uint16_t whist[256][W];  // per-wave digit counters, digit-major
   uint64_t edge[W][kLocalWords];  // keys of each wave's last item
-  uint32_t wvm[W];   // per wave: varying key bytes | 0x100 if a group is > 16
+  uint32_t wvm[W];   // per wave: varying key bytes | 0x100 if a group is > insertion_max
   uint32_t wtot[W];  // per wave: survivors | their group heads << 16
   uint32_t scan[W + 1];
 };
@@ -233,7 +244,7 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
 
     // 2. Per sub-segment: which keys differ between adjacent members (two
     //    members differ somewhere only if some adjacent pair does) or hold a
-    //    NUL.  Also note whether any sub-segment exceeds 16 strings.
+    //    NUL.  Also note whether any sub-segment exceeds insertion_max strings.
     bool big = false;
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
@@ -257,7 +268,7 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
         if (gv[j] != 0xFFFFu) {
           if (stop && tag != prev_tag)
             atomicOr((uint32_t*)&s.ss_stop[0] + (gv[j] >> 2), stop << (8 * (gv[j] & 3)));
-          big = big || (s.ss_start[gv[j] + 1] - s.ss_start[gv[j]]) > 16;
+          big = big || (s.ss_start[gv[j] + 1] - s.ss_start[gv[j]]) > insertion_max(CAP);
         }
       }
     }
