@@ -140,7 +140,10 @@ gss_server *gss_server_create(const unsigned char *d_text, uint64_t text_bytes,
 /* Queues a sort of the n strings at device offsets d_off (into the server's
  * text).  The sorted offsets go to d_out (device, n entries, may equal d_off)
  * and, when d_lcp is not NULL, the n-1 LCPs to d_lcp.  The arrays must stay
- * valid until the request completes.  Returns the request id (>= 0), or a
+ * valid until the request completes.  The server's own streams read d_off
+ * and write d_out / d_lcp without waiting for any other stream: device work
+ * that writes these arrays (or reads d_out) must be complete before the
+ * call.  Returns the request id (>= 0), or a
  * negative GSS_E* code: GSS_E_TOO_LARGE when n exceeds one slot's batch
  * capacity (that request is not admitted; sort it with gss_sort_device32). */
 int64_t gss_server_submit(gss_server *s, const uint32_t *d_off, uint32_t *d_out,

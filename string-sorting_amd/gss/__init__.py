@@ -400,7 +400,9 @@ class Server:
     """The served request loop (gss_server_* in gss.h) over one device text.
 
     ``submit`` queues a sort of a uint32 offset tensor (a contiguous device
-    tensor, e.g. a slice of a pool) into an output tensor of the same size;
+    tensor, e.g. a slice of a pool) into an output tensor of the same size
+    (the server's streams do not wait for torch's: work that writes these
+    tensors must have finished, e.g. after ``torch.cuda.synchronize()``);
     slots take queued requests in order, fold as many as fit into one batch
     and complete them together.  ``wait`` returns the request's latency in ms.
     """

@@ -64,13 +64,14 @@ def test_served_stream_matches_alone_and_oracle(stream, pools_dev, oracle_mod):
         assert cap < sum(n for _, n in reqs), "capacity must force several batches"
         outs, lcps, ids = [], [], []
         for k, (a, n) in enumerate(reqs):
-            o = torch.full((max(1, n),), -1, dtype=torch.int32, device=text.device)
-            l = None
-            if k % 2 == 1:
-                l = torch.full((max(1, n - 1),), -7, dtype=torch.int64, device=text.device)
-            outs.append(o)
-            lcps.append(l)
-            ids.append(srv.submit(off[a:a + n], o[:n], l))
+            outs.append(torch.full((max(1, n),), -1, dtype=torch.int32, device=text.device))
+            lcps.append(torch.full((max(1, n - 1),), -7, dtype=torch.int64, device=text.device)
+                        if k % 2 == 1 else None)
+        # The server's streams do not wait for torch's: the fills above must
+        # be done before a request is handed over.
+        torch.cuda.synchronize()
+        for k, (a, n) in enumerate(reqs):
+            ids.append(srv.submit(off[a:a + n], outs[k][:n], lcps[k]))
             if k >= 20:  # the first 20 arrive at once, the rest spread out
                 time.sleep(float(rng.uniform(0, 0.002)))
         lat = [srv.wait(i) for i in ids]
