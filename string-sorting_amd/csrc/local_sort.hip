@@ -16,8 +16,9 @@
 //              over the key bytes that vary inside some sub-segment, then
 //              over the sub-segment id (wave match-ranking with per-wave LDS
 //              digit counters); when no sub-segment exceeds insertion_max
-//              strings (16, or 64 in the largest class) each is
-//              insertion-sorted by one thread;
+//              strings each is insertion-sorted by one thread (16), or, in
+//              the largest class, every item is ranked inside its group by
+//              counting (128);
 //   4. split   into equal-key groups.  A singleton, or a group whose key holds
 //              the NUL (identical strings), is final -- the retirement rule
 //              of multikey_cache (src/multikey_cache.cpp:207-211) and of
@@ -35,15 +36,16 @@ namespace gss {
 
 constexpr int kLocalWords = 4;
 
-// Largest group a round still insertion-sorts (one thread per group, the
-// groups side by side) instead of running the LSD passes over all items.
-// 16 in the small classes, whose few threads would wait on one long
-// insertion sort; 64 in the 1024-thread class, where hundreds of groups sort
-// at once and one 64-string insertion sort (about 1,000 dependent LDS steps)
-// costs about what the passes would, so no round gets slower than with the
-// passes.  On 10M URL strings 1.78 -> 1.71 ms, the other workloads
-// unchanged; 96 and 128 gained nothing more.
-constexpr uint32_t insertion_max(uint32_t cap) { return cap > kLocalClassCap[1] ? 64u : 16u; }
+// Largest group a round still sorts without the LSD passes over all items:
+// 16 in the small classes, which insertion-sort each group in one thread
+// (their few threads would wait on one long insertion sort), 128 in the
+// 1024-thread class, which ranks every item inside its group by counting
+// (independent LDS reads, broadcast across the lanes of a group), cheaper
+// than the passes up to that size.  10M URL strings: 1.79 ms with 16 for all
+// classes, 1.72 ms with a 64-string insertion sort, 1.68 ms counting up to
+// 128 (256 no better, 512 slower); the other workloads unchanged.
+constexpr uint32_t kCountMax = 128;
+constexpr uint32_t insertion_max(uint32_t cap) { return cap > kLocalClassCap[1] ? kCountMax : 16u; }
 
 template <int BLOCK, int CAP>
 struct LocalSmem {
@@ -345,7 +347,43 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
 
     // 5. Sort every sub-segment by its key (stable).
     if (vm) {
-      if (!big) {
+      if (!big && kClass == 2) {
+        // Every item's stable rank inside its group by counting: it reads
+        // the group's keys in order (the lanes of one group read the same
+        // word at each step, which the LDS broadcasts), then all items move.
+        // The reads are independent, unlike an insertion sort's chain.
+        uint64_t ck[IPT];
+        uint32_t co[IPT], cd[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+          ck[j] = 0;
+          co[j] = 0;
+          cd[j] = 0xFFFFFFFFu;
+          const uint32_t i = base + j * kWave;
+          if ((uint32_t)j < jn && i < nact) {
+            const uint32_t g = s.sid[i];
+            const uint32_t a = s.ss_start[g], en = s.ss_start[g + 1];
+            const uint64_t k = s.key[i];
+            uint32_t r = 0;
+            for (uint32_t m = a; m < en; ++m) {
+              const uint64_t km = s.key[m];
+              r += (km < k || (km == k && m < i)) ? 1u : 0u;
+            }
+            ck[j] = k;
+            co[j] = s.off[i];
+            cd[j] = a + r;
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+          if (cd[j] != 0xFFFFFFFFu) {
+            s.key[cd[j]] = ck[j];
+            s.off[cd[j]] = co[j];
+          }
+        }
+        __syncthreads();
+      } else if (!big) {
         for (uint32_t g = threadIdx.x; g < nss; g += BLOCK) {
           const uint32_t a = s.ss_start[g], en = s.ss_start[g + 1];
           for (uint32_t i = a + 1; i < en; ++i) {
