@@ -26,8 +26,13 @@
 //                  text-reading gather it also writes each string bound for
 //                  the next pass its key at the child's depth.
 //
-// Blocks of the tile kernels take tiles in XCD-aware order (xcd_tile).
 //   boundary_lcp   LCP of each new adjacent pair of child buckets.
+//
+// Blocks of the tile kernels take tiles in XCD-aware order (xcd_tile).
+// Between passes the host and the device hand over through two small
+// kernels instead of copies: upload reads the host-built tile table from
+// fine-grained pinned memory, publish writes the counters and the next
+// segment list back and then a sequence word that the host polls.
 //
 // A segment whose keys are all equal costs the gather only: it keeps its
 // place and buffer and moves 8 bytes deeper (or retires if the key holds the
