@@ -169,6 +169,124 @@ __device__ __forceinline__ void local_radix_pass(LocalSmem<BLOCK, CAP>& s, uint3
   __syncthreads();
 }
 
+// A round with one group (a segment's first sorting round): stable passes
+// by its two most significant varying bytes leave sub-buckets of strings
+// equal on both, and every item is then ranked inside its sub-bucket by
+// counting (the sub-bucket's items that order before it).  A sub-bucket
+// starts at a change of the two bytes; each item finds the start by ballots
+// within its wave and the last start of the waves before it.  Returns false,
+// the items sorted by the two bytes only, when a sub-bucket holds more than
+// kCountMax strings (the caller's LSD passes then sort the group: they are
+// stable, so the order these passes left does not matter).
+template <int BLOCK, int CAP>
+__device__ __forceinline__ bool local_two_byte_count(LocalSmem<BLOCK, CAP>& s, uint32_t nact,
+                                                     uint32_t ipw, uint32_t base, uint32_t jn,
+                                                     uint32_t vm, uint32_t count_max) {
+  constexpr int IPT = CAP / BLOCK;
+  constexpr int W = BLOCK / kWave;
+  const int top = 31 - __builtin_clz(vm);
+  const uint32_t rest = vm & ~(1u << top);
+  if (rest) local_radix_pass<BLOCK, CAP>(s, nact, base, jn, 31 - __builtin_clz(rest));
+  local_radix_pass<BLOCK, CAP>(s, nact, base, jn, top);
+  if (!rest) return true;  // one varying byte: that pass sorted the group
+  const int sec = 31 - __builtin_clz(rest);
+  const uint32_t wid = threadIdx.x / kWave;
+  const uint32_t lane = lane_id();
+  const uint32_t slot0 = wid * ipw;  // first item of this wave
+  auto tag = [&](uint64_t k) -> uint32_t {
+    return ((uint32_t)(k >> (8 * top)) & 0xFFu) << 8 | ((uint32_t)(k >> (8 * sec)) & 0xFFu);
+  };
+  // Last sub-bucket start of this wave, then of the waves before it.
+  uint32_t last = 0;
+  bool has = false;
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t i = base + j * kWave;
+    const bool v = (uint32_t)j < jn && i < nact;
+    const bool head = v && (i == 0 || tag(s.key[i]) != tag(s.key[i > 0 ? i - 1 : 0]));
+    const uint64_t bh = __ballot(head);
+    if (bh) {
+      last = slot0 + j * kWave + (63 - __builtin_clzll(bh));
+      has = true;
+    }
+  }
+  if (lane == 0) s.wtot[wid] = has ? last : 0xFFFFFFFFu;
+  __syncthreads();
+  uint32_t carry = 0;  // item 0 starts a sub-bucket, so wave 0 always has one
+  for (uint32_t w = 0; w < wid; ++w) {
+    const uint32_t h = s.wtot[w];
+    if (h != 0xFFFFFFFFu) carry = h;
+  }
+  // Each item's sub-bucket start goes to sid, which holds 0 for every item of
+  // a one-group round; no item may lie kCountMax or more past its start
+  // (checked before any counting, so a big run of equal strings costs one
+  // scan of starts, not a quadratic count).  Then each item counts its
+  // sub-bucket and sid takes its destination: nothing is held in registers
+  // across the items.
+  bool over = false;
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t i = base + j * kWave;
+    const bool v = (uint32_t)j < jn && i < nact;
+    const bool head = v && (i == 0 || tag(s.key[i]) != tag(s.key[i > 0 ? i - 1 : 0]));
+    const uint64_t bh = __ballot(head);
+    const uint64_t le = bh & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+    const uint32_t lo = le ? slot0 + j * kWave + (63 - __builtin_clzll(le)) : carry;
+    if (bh) carry = slot0 + j * kWave + (63 - __builtin_clzll(bh));
+    if (v) {
+      over = over || i - lo >= count_max;
+      s.sid[i] = (uint16_t)lo;
+    }
+  }
+  if (!__syncthreads_or(over)) {
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t i = base + j * kWave;
+      if ((uint32_t)j < jn && i < nact) {
+        const uint32_t lo = s.sid[i];
+        const uint64_t k = s.key[i];
+        const uint32_t t = tag(k);
+        uint32_t r = 0;
+        for (uint32_t m = lo; m < nact; ++m) {
+          const uint64_t km = s.key[m];
+          if (tag(km) != t) break;
+          r += (km < k || (km == k && m < i)) ? 1u : 0u;
+        }
+        s.sid[i] = (uint16_t)(lo + r);
+      }
+    }
+  }
+  const bool bail = __syncthreads_or(over);  // block-uniform, as above
+  uint32_t cd[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t i = base + j * kWave;
+    cd[j] = (uint32_t)j < jn && i < nact ? s.sid[i] : 0xFFFFFFFFu;
+  }
+  if (!bail) {
+    // Keys, then offsets, each read whole before any is written.
+    uint64_t ck[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) ck[j] = cd[j] != 0xFFFFFFFFu ? s.key[base + j * kWave] : 0ull;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < IPT; ++j)
+      if (cd[j] != 0xFFFFFFFFu) s.key[cd[j]] = ck[j];
+    uint32_t co[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) co[j] = cd[j] != 0xFFFFFFFFu ? s.off[base + j * kWave] : 0u;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < IPT; ++j)
+      if (cd[j] != 0xFFFFFFFFu) s.off[cd[j]] = co[j];
+  }
+#pragma unroll
+  for (int j = 0; j < IPT; ++j)
+    if (cd[j] != 0xFFFFFFFFu) s.sid[base + j * kWave] = 0;  // the one group again
+  __syncthreads();
+  return !bail;
+}
+
 // COUNT: also count the 8-byte key levels the strings advance through (for
 // profiling only: the count costs registers).
 template <int BLOCK, int CAP, bool COUNT>
@@ -400,7 +518,8 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
           }
         }
         __syncthreads();
-      } else {
+      } else if (!(kClass == 2 && nss == 1 &&
+                   local_two_byte_count<BLOCK, CAP>(s, nact, ipw, base, jn, vm, kCountMax))) {
         for (int b = 0; b < 8; ++b)
           if ((vm >> b) & 1u) local_radix_pass<BLOCK, CAP>(s, nact, base, jn, b);
         if (nss > 1) local_radix_pass<BLOCK, CAP>(s, nact, base, jn, 8);

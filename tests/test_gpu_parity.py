@@ -45,6 +45,18 @@ EDGE_CASES = {
                    + bytes([0xFF - (j % 7)]) for j in range(5000)],
     "key_boundaries": [b"x" * L for L in range(0, 40)] + [b"x" * L + b"y" for L in range(0, 40)],
     "modular_9": [bytes([ord("a") + (i % 30)]) * 9 for i in range(8001)],
+    # One local segment whose first sorting key varies in one byte only.
+    "one_varying_byte": [b"ppp" + bytes([ord("a") + (i * 7) % 26]) + b"qqqq" + str(i % 997).encode()
+                         for i in range(5000)],
+    # Two varying bytes, four combinations: sub-buckets far above the
+    # counting bound, so the round falls back to the LSD passes.
+    "two_bytes_few": [b"pp" + b"ab"[i % 2:i % 2 + 1] + b"qq" + b"xy"[(i // 2) % 2:(i // 2) % 2 + 1]
+                      + b"rr" + str((i * 31) % 1009).encode() for i in range(5000)],
+    # Two varying bytes, 676 combinations and exact duplicates: counting
+    # inside small sub-buckets, ties kept in input order.
+    "two_bytes_many": [b"pp" + bytes([ord("a") + (i * 11) % 26]) + b"qq"
+                       + bytes([ord("a") + (i * 5 // 26) % 26]) + b"rr" + str(i % 3).encode()
+                       for i in range(5000)],
 }
 
 
