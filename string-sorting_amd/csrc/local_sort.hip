@@ -16,9 +16,9 @@
 //              over the key bytes that vary inside some sub-segment, then
 //              over the sub-segment id (wave match-ranking with per-wave LDS
 //              digit counters); when no sub-segment exceeds insertion_max
-//              strings each is insertion-sorted by one thread (16), or, in
-//              the largest class, every item is ranked inside its group by
-//              counting (128);
+//              strings each is insertion-sorted by one thread (16, middle
+//              class), or every item is ranked inside its group by counting
+//              (128 in the largest class, always in the one-wave class);
 //   4. split   into equal-key groups.  A singleton, or a group whose key holds
 //              the NUL (identical strings), is final -- the retirement rule
 //              of multikey_cache (src/multikey_cache.cpp:207-211) and of
@@ -36,16 +36,24 @@ namespace gss {
 
 constexpr int kLocalWords = 4;
 
-// Largest group a round still sorts without the LSD passes over all items:
-// 16 in the small classes, which insertion-sort each group in one thread
-// (their few threads would wait on one long insertion sort), 128 in the
-// 1024-thread class, which ranks every item inside its group by counting
-// (independent LDS reads, broadcast across the lanes of a group), cheaper
-// than the passes up to that size.  10M URL strings: 1.79 ms with 16 for all
-// classes, 1.72 ms with a 64-string insertion sort, 1.68 ms counting up to
-// 128 (256 no better, 512 slower); the other workloads unchanged.
+// Largest group a round still sorts without the LSD passes over all items.
+// The 1024-thread class and the one-wave class rank every item inside its
+// group by counting (independent LDS reads, broadcast across the lanes of a
+// group): up to 128 strings in the former, cheaper than the passes up to
+// that size, and always in the latter, whose 64 strings fit one wave (DNA
+// 32M, whose time is in that class, 5.22 -> 4.53 ms).  The middle class
+// insertion-sorts groups of up to 16 strings in one thread each (counting
+// there gained nothing).  10M URL strings: 1.79 ms with 16 for all classes,
+// 1.72 ms with a 64-string insertion sort, 1.68 ms counting up to 128 (256
+// no better, 512 slower).
 constexpr uint32_t kCountMax = 128;
-constexpr uint32_t insertion_max(uint32_t cap) { return cap > kLocalClassCap[1] ? kCountMax : 16u; }
+constexpr uint32_t insertion_max(uint32_t cap) {
+  return cap > kLocalClassCap[1] ? kCountMax : (cap <= kLocalClassCap[0] ? cap : 16u);
+}
+// Classes that count ranks rather than insertion-sort.
+constexpr bool counts_ranks(uint32_t cap) {
+  return cap > kLocalClassCap[1] || cap <= kLocalClassCap[0];
+}
 
 template <int BLOCK, int CAP>
 struct LocalSmem {
@@ -465,7 +473,7 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
 
     // 5. Sort every sub-segment by its key (stable).
     if (vm) {
-      if (!big && kClass == 2) {
+      if (!big && counts_ranks(CAP)) {
         // Every item's stable rank inside its group by counting: it reads
         // the group's keys in order (the lanes of one group read the same
         // word at each step, which the LDS broadcasts), then all items move.
