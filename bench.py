@@ -46,7 +46,7 @@ HBM_PEAK_GBPS = 8000.0
 # and 8 text bytes, write the partitioned (8-byte key, offset) pair.
 BYTES_PER_ROUND_STRING = 2 * (4 + 8)  # per string and 8-byte key level
 KERNELS = ["gather", "segreduce", "layout", "histogram", "flat_scan", "segscan", "scatter",
-           "boundary", "retire", "local64", "local1024", "local6144", "local64_early",
+           "boundary", "retire", "local128", "local1024", "local6144", "local128_early",
            "local1024_early", "local6144_early"]
 PASSES = [f"pass{r}" for r in range(9)] + ["pass9+", "local_phase"]
 
@@ -57,7 +57,7 @@ def log(msg):
 
 # Library profile names -> device symbol fragments in the rocprofv3 tables.
 SYMBOLS = {"gather": "k_gather_minmax", "histogram": "k_histogram", "scatter": "k_scatter",
-           "segscan": "k_segscan", "local64": "k_local<64, 64, false>",
+           "segscan": "k_segscan", "local128": "k_local<64, 128, false>",
            "local1024": "k_local<256, 1024, false>", "local6144": "k_local<1024, 6144, false>"}
 PROFILE = ROOT / "profiles" / "url_like_10M_kernels.md"
 

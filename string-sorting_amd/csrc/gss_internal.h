@@ -109,7 +109,7 @@ struct LocalStats {
 };
 
 // Size classes of the local kernel: (threads, capacity).
-constexpr uint32_t kLocalClassCap[3] = {64, 1024, 6144};
+constexpr uint32_t kLocalClassCap[3] = {128, 1024, 6144};
 constexpr uint32_t kLocalClassThreads[3] = {64, 256, 1024};
 
 // Pointers into the workspace for one sort.

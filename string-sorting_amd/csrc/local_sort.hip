@@ -40,8 +40,8 @@ constexpr int kLocalWords = 4;
 // The 1024-thread class and the one-wave class rank every item inside its
 // group by counting (independent LDS reads, broadcast across the lanes of a
 // group): up to 128 strings in the former, cheaper than the passes up to
-// that size, and always in the latter, whose 64 strings fit one wave (DNA
-// 32M, whose time is in that class, 5.22 -> 4.53 ms).  The middle class
+// that size, and always in the latter, one wave over at most 128 strings
+// (DNA 32M, whose time is in that class, 5.22 -> 4.53 ms).  The middle class
 // insertion-sorts groups of up to 16 strings in one thread each (counting
 // there gained nothing).  10M URL strings: 1.79 ms with 16 for all classes,
 // 1.72 ms with a 64-string insertion sort, 1.68 ms counting up to 128 (256
@@ -636,7 +636,7 @@ template <bool COUNT>
 hipError_t launch_local_t(const Plan& p, int cls, uint32_t first, uint32_t nsegs, hipStream_t s) {
   switch (cls) {
     case 0:
-      k_local<64, 64, COUNT><<<nsegs, 64, 0, s>>>(p, p.local[0] + first);
+      k_local<64, 128, COUNT><<<nsegs, 64, 0, s>>>(p, p.local[0] + first);
       break;
     case 1:
       k_local<256, 1024, COUNT><<<nsegs, 256, 0, s>>>(p, p.local[1] + first);

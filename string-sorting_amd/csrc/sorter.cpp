@@ -490,7 +490,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     GSS_HIP(hipEventRecord(c.side_done, st));
     GSS_HIP(hipStreamWaitEvent(c.side, c.side_done, 0));
     side_used = true;
-    static const char* kEarlyNames[3] = {"local64_early", "local1024_early", "local6144_early"};
+    static const char* kEarlyNames[3] = {"local128_early", "local1024_early", "local6144_early"};
     for (int cls = 2; cls >= 0; --cls) {
       if (upto[cls] <= launched[cls]) continue;
       if (upto[cls] > p.cap_local[cls]) return fail(GSS_E_INTERNAL, "local segment list overflow");
@@ -606,7 +606,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   // summed over the local key rounds, are filled in after the read-back).
   ProfToken phase_tok = prof_begin(c, "local_phase", 0, st);
   if (nretire) GSS_LAUNCH("retire", nretire, launch_retire(p, nretire, st));
-  static const char* kLocalNames[3] = {"local64", "local1024", "local6144"};
+  static const char* kLocalNames[3] = {"local128", "local1024", "local6144"};
   long local_tok[3] = {-1, -1, -1};  // pending profile entries of the local launches
   for (int cls = 2; cls >= 0; --cls) {
     if (nlocal[cls] > p.cap_local[cls]) return fail(GSS_E_INTERNAL, "local segment list overflow");

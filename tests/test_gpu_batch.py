@@ -56,7 +56,7 @@ def _check_requests(joined, req, sets, o, l, oracle):
 
 
 @pytest.mark.parametrize("sizes", [
-    [0, 1, 2, 3, 64, 65, 1000, 1024, 1025, 5000, 6144, 6145, 8192, 8193, 30000],
+    [0, 1, 2, 3, 64, 65, 128, 129, 1000, 1024, 1025, 5000, 6144, 6145, 8192, 8193, 30000],
     [40000, 1, 0, 70000, 12, 9000],
     [1] * 50 + [2] * 50,
 ])
