@@ -10,8 +10,13 @@ Besides the contract fields the JSON line carries:
   roofline      the dominant kernel's algorithmic bytes per launch / its
                 average launch time (HIP events on the launch stream),
                 against the 8 TB/s HBM peak;
-  cpu_baseline  the reference's msd_CE2 (compiled from its sources into
-                oracle/_ref) timed on this host's cores on the same workload;
+  cpu_baseline  the reference's own single-core routines (msd_CE2, msd_CE6,
+                msd_ci_adaptive, msd_A2_adaptive, multikey_cache8, burstsortA,
+                compiled from its sources into oracle/_ref) on one core of
+                this host, on the same 10M strings: `value` is the fastest,
+                7 runs, trimmed mean of the middle 5; msd_CE2 beside it, and
+                parallel_msd_radix_sort on the host's cores (CPU model and
+                thread count stated), plus the 1M uniform a-z set;
   hbm_gbps      algorithmic bytes of the whole sort / sort time;
   e2e           the routine.h pointer path (gss_msd), PCIe transfers included;
   pipeline      the steps either side of the sort on the same set: device
@@ -87,7 +92,7 @@ def parse():
     ap.add_argument("--lcp", action="store_true", help="also emit the LCP array in the timed steps")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-e2e", action="store_true", help="skip the pointer-path timing")
-    ap.add_argument("--cpu-runs", type=int, default=2)
+    ap.add_argument("--cpu-runs", type=int, default=CPU_RUNS)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device twice: the "
                          "reference dispatch that scales FETCH_SIZE / WRITE_SIZE counters")
@@ -368,7 +373,8 @@ def served_stream(args, dev):
             t_due = start + due[i]
             while time.perf_counter() < t_due:
                 pass
-            rid = L.gss_server_submit(srv.handle, in_ptr[i], out_ptr["served"][i], None, ns[i])
+            rid = L.gss_server_submit(srv.handle, in_ptr[i], out_ptr["served"][i], None, ns[i],
+                                      None)
             if rid < 0:
                 raise RuntimeError(f"submit of request {i} failed: {gss.last_error()}")
             ids[i] = rid
@@ -421,43 +427,145 @@ def served_stream(args, dev):
     }
 
 
-def cpu_baseline(ss, runs):
-    """msd_CE2 of the reference (oracle/_ref/libref.so) on the same strings,
-    one core.  Falls back to the oracle's C restatement when the reference
-    build is absent."""
+# The reference's best single-core routines for this path and its multicore
+# baseline (BASELINE.md's plan): msd_CE2 is the parity oracle, the others
+# are the fastest of their families on these inputs.
+CPU_ROUTINES = ["msd_CE2", "msd_CE6", "msd_ci_adaptive", "msd_A2_adaptive", "multikey_cache8",
+                "burstsortA"]
+CPU_RUNS = 7  # the reference's report: 7 runs, trimmed mean of the middle 5
+
+
+def _trimmed_mean(times):
+    """Mean of the middle 5 of 7 runs (report/htmlreport.xsl:265-277); with
+    fewer runs, the mean of all but the fastest and slowest when possible."""
+    t = sorted(times)
+    if len(t) >= 3:
+        t = t[1:-1]
+    return sum(t) / len(t)
+
+
+def _cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _time_routine(name, ptrs0, runs):
+    """Wall times (perf_counter_ns, seconds) of reference routine `name` on
+    fresh copies of the pointer array; the copy is outside the timing, as
+    sortstring's pointer build is (src/sortstring.c:424-451)."""
     import oracle
 
-    base = np.uint64(ss.text.ctypes.data)
-    ptrs0 = (base + ss.offsets.astype(np.uint64)).astype(np.uint64)
-    if oracle.ref_available():
-        kind, fn = "reference", lambda p: oracle.ref_sort_pointers("msd_CE2", p)
-        name = "msd_CE2 (reference sources, gcc -O3)"
-    else:
-        lib = oracle.lib()
-        kind, fn = "port", lambda p: lib.or_msd_ce2(p.ctypes.data, p.shape[0])
-        name = "msd_CE2 restatement (oracle.c)"
+    fn = oracle.ref_routine(name)
     times = []
     for _ in range(runs):
         p = ptrs0.copy()
-        t0 = time.perf_counter()
-        fn(p)
-        times.append(time.perf_counter() - t0)
-    best = min(times)
-    out = {"value": ss.n / best / 1e6, "unit": "Mstrings/s", "cores": 1, "kind": kind,
-           "sample": f"{name} on the full {ss.n}-string URL-like set, best of {runs} runs "
-                     f"({', '.join(f'{t:.2f}' for t in times)} s)"}
-    if oracle.ref_available():
-        try:
-            threads = min(16, os.cpu_count() or 1)
-            os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+        t0 = time.perf_counter_ns()
+        fn(p.ctypes.data, p.shape[0])
+        times.append((time.perf_counter_ns() - t0) * 1e-9)
+    return times, p
+
+
+def cpu_baseline(ss, runs=CPU_RUNS):
+    """The reference's own routines (oracle/_ref/libref.so, compiled from its
+    sources) on this host, in this process:
+
+    * single core (the process pinned to one CPU with sched_setaffinity):
+      every routine of CPU_ROUTINES once on the full URL-like set to screen,
+      then `runs` runs of the fastest and of msd_CE2, trimmed mean;
+    * parallel_msd_radix_sort on all CPUs this process may use, `runs` runs;
+    * the same on the 1,000,000-string uniform a-z set (config 0), every
+      routine `runs` times, on a nanosecond clock.
+
+    `value` is the fastest single-core routine on the URL-like set.  Falls
+    back to the oracle's C restatement of msd_CE2 when the reference build
+    is absent."""
+    import oracle
+
+    import gss.gen as gen
+
+    base = np.uint64(ss.text.ctypes.data)
+    ptrs0 = (base + ss.offsets.astype(np.uint64)).astype(np.uint64)
+    if not oracle.ref_available():
+        lib = oracle.lib()
+        times = []
+        for _ in range(runs):
             p = ptrs0.copy()
-            t0 = time.perf_counter()
-            oracle.ref_sort_pointers("parallel_msd_radix_sort", p)
-            tp = time.perf_counter() - t0
-            out["multicore"] = {"routine": "parallel_msd_radix_sort", "threads": threads,
-                                "value": ss.n / tp / 1e6, "seconds": tp}
-        except Exception as e:  # the baseline is a report, not the target
-            out["multicore"] = {"error": str(e)}
+            t0 = time.perf_counter_ns()
+            lib.or_msd_ce2(p.ctypes.data, p.shape[0])
+            times.append((time.perf_counter_ns() - t0) * 1e-9)
+        t = _trimmed_mean(times)
+        return {"value": ss.n / t / 1e6, "unit": "Mstrings/s", "cores": 1, "kind": "port",
+                "sample": f"msd_CE2 restatement (oracle.c) on the full {ss.n}-string URL-like "
+                          f"set, trimmed mean of {runs} runs"}
+    allowed = sorted(os.sched_getaffinity(0))
+    cpu_model = _cpu_model()
+    want = np.sort(ss.offsets)
+    out = {"unit": "Mstrings/s", "cores": 1, "kind": "reference", "cpu_model": cpu_model,
+           "nproc": len(allowed), "os_cpu_count": os.cpu_count()}
+    try:
+        os.sched_setaffinity(0, {allowed[0]})
+        screen = {}
+        for name in CPU_ROUTINES:
+            t, p = _time_routine(name, ptrs0, 1)
+            got = (p - base).astype(np.uint32)
+            ok = bool(np.array_equal(np.sort(got), want)) and oracle.check_result(ss.text, got)[2] == 0
+            screen[name] = {"seconds": t[0], "mstrings_per_s": ss.n / t[0] / 1e6, "check": ok}
+            log(f"  {name}: {t[0]:.3f} s (sorted: {ok})")
+        best = min((k for k in screen if screen[k]["check"]), key=lambda k: screen[k]["seconds"])
+        timed = {}
+        for name in dict.fromkeys([best, "msd_CE2"]):
+            t, _ = _time_routine(name, ptrs0, runs)
+            timed[name] = {"trimmed_mean_s": _trimmed_mean(t), "runs_s": t,
+                           "mstrings_per_s": ss.n / _trimmed_mean(t) / 1e6}
+        small = gen.make(0)
+        sbase = np.uint64(small.text.ctypes.data)
+        sptrs = (sbase + small.offsets.astype(np.uint64)).astype(np.uint64)
+        config0 = {}
+        for name in CPU_ROUTINES:
+            t, _ = _time_routine(name, sptrs, runs)
+            config0[name] = {"trimmed_mean_ms": _trimmed_mean(t) * 1e3,
+                             "mstrings_per_s": small.n / _trimmed_mean(t) / 1e6}
+    finally:
+        os.sched_setaffinity(0, allowed)
+    out.update({
+        "value": timed[best]["mstrings_per_s"],
+        "routine": best,
+        "sample": (f"{best} (reference sources, gcc -O3), one core of {cpu_model}, on the full "
+                   f"{ss.n}-string URL-like set: trimmed mean of the middle 5 of {runs} runs; "
+                   f"screened against {', '.join(CPU_ROUTINES)} (one run each)"),
+        "screen": screen, "timed": timed,
+        "msd_CE2": timed["msd_CE2"]["mstrings_per_s"],
+        "config0_uniform_1M": config0,
+    })
+    # The multicore baseline on every CPU this process may use (bounded by
+    # OMP_NUM_THREADS when the environment sets it: a host shared between
+    # jobs gives each its share that way).  OpenMP reads its thread count
+    # when libref.so loads, so the count is set through the OpenMP runtime
+    # the library uses.
+    try:
+        import ctypes as C
+
+        env_threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        threads = min(len(allowed), env_threads) if env_threads > 0 else len(allowed)
+        gomp = C.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(threads)
+        t, p = _time_routine("parallel_msd_radix_sort", ptrs0, runs)
+        got = (p - base).astype(np.uint32)
+        t0, _ = _time_routine("parallel_msd_radix_sort", sptrs, runs)
+        out["multicore"] = {"routine": "parallel_msd_radix_sort", "threads": threads,
+                            "cpu_model": cpu_model, "nproc": len(allowed),
+                            "omp_num_threads_env": env_threads or None,
+                            "value": ss.n / _trimmed_mean(t) / 1e6,
+                            "trimmed_mean_s": _trimmed_mean(t), "runs_s": t,
+                            "config0_mstrings_per_s": small.n / _trimmed_mean(t0) / 1e6,
+                            "sorted": bool(np.array_equal(np.sort(got), want))}
+    except Exception as e:  # the baseline is a report, not the target
+        out["multicore"] = {"error": str(e)}
     return out
 
 

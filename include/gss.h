@@ -49,10 +49,12 @@ void gss_msd_lcp(unsigned char **strings, size_t n, size_t *lcp);
  * device).  The text and the 32-bit string offsets already sit in device
  * memory; this is the path the throughput figures are quoted on.
  *
- *   d_text     device pointer, 8-byte aligned, text_bytes readable bytes;
- *              every string starts at an offset < text_bytes and its NUL
- *              lies inside the text (a string running off the end is cut
- *              there, as if a NUL followed).
+ *   d_text     device pointer, 8-byte aligned, text_bytes >= 1 bytes (the
+ *              last, partial 8-byte word may be read, never written, in
+ *              full); every string starts at an offset < text_bytes.  A
+ *              string running off the end is cut there, as if a NUL
+ *              followed: bytes at or beyond text_bytes are never part of
+ *              a string, also when they share the text's last word.
  *   d_off_in   n string offsets into d_text (any order, may repeat).
  *   d_off_out  n offsets, receives the sorted order (may equal d_off_in).
  *   d_lcp      NULL, or n-1 uint64 LCP values as for gss_msd_lcp.
@@ -132,22 +134,32 @@ typedef struct gss_server_stats {
 
 /* Creates a server on the current device for the text d_text (device, 8-byte
  * aligned, text_bytes bytes, as for gss_sort_device32), which must stay valid
- * until gss_server_destroy.  One slot's batch capacity is the largest string
- * count whose buffers and workspace, times `slots`, fit hbm_budget (under the
- * tuning current at creation).  Returns NULL on error (gss_last_error()). */
+ * until gss_server_destroy.  The tuning current at creation is kept: every
+ * batch of this server uses it, whatever gss_set_tuning does later.  One
+ * slot's batch capacity is the largest string count whose buffers and
+ * workspace under that tuning, times `slots`, fit hbm_budget.  Returns NULL
+ * on error (gss_last_error()). */
 gss_server *gss_server_create(const unsigned char *d_text, uint64_t text_bytes,
 			      const gss_server_config *cfg);
 /* Queues a sort of the n strings at device offsets d_off (into the server's
  * text).  The sorted offsets go to d_out (device, n entries, may equal d_off)
  * and, when d_lcp is not NULL, the n-1 LCPs to d_lcp.  The arrays must stay
- * valid until the request completes.  The server's own streams read d_off
- * and write d_out / d_lcp without waiting for any other stream: device work
- * that writes these arrays (or reads d_out) must be complete before the
- * call.  Returns the request id (>= 0), or a
- * negative GSS_E* code: GSS_E_TOO_LARGE when n exceeds one slot's batch
- * capacity (that request is not admitted; sort it with gss_sort_device32). */
+ * valid until the request completes.
+ *
+ * Stream order: ready_event, when not NULL, is a hipEvent_t recorded after
+ * the device work that writes d_off (or still reads d_out / d_lcp); the
+ * request's device work waits for that work.  The event is captured as
+ * recorded at the time of the call, so the caller may record it again
+ * right away.  With ready_event NULL the server waits for no other stream,
+ * and such work must be complete before the call.  Either way the results
+ * are complete on the device when gss_server_wait returns (or
+ * gss_server_done reports 1).
+ *
+ * Returns the request id (>= 0), or a negative GSS_E* code: GSS_E_TOO_LARGE
+ * when n exceeds one slot's batch capacity (that request is not admitted;
+ * sort it with gss_sort_device32). */
 int64_t gss_server_submit(gss_server *s, const uint32_t *d_off, uint32_t *d_out,
-			  uint64_t *d_lcp, uint64_t n);
+			  uint64_t *d_lcp, uint64_t n, void *ready_event);
 /* Waits for request `id` and releases its record; *latency_ms (optional) =
  * completion - submission.  Returns the request's status (0 = sorted). */
 int gss_server_wait(gss_server *s, int64_t id, double *latency_ms);

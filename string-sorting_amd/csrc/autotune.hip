@@ -41,7 +41,7 @@ struct SigOut {
 
 // One block of kSigStrings threads: thread i reads string i * n / m.
 __global__ __launch_bounds__(kSigStrings) void k_signature(const uint64_t* __restrict__ t64,
-                                                          uint64_t nwords,
+                                                          uint64_t tlen,
                                                           const uint32_t* __restrict__ off,
                                                           uint64_t n, SigOut* __restrict__ res) {
   __shared__ uint32_t s_alpha[8];
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(kSigStrings) void k_signature(const uint64_t* __res
     uint32_t len = 0;
     bool ended = false;
     for (uint32_t k = 0; k < kSigLenCap && !ended; k += 8) {
-      const uint64_t key = load_key(t64, nwords, o + k);  // big-endian, zero after NUL
+      const uint64_t key = load_key(t64, tlen, o + k);  // big-endian, zero after NUL
       const uint32_t l = key_len(key);                    // bytes before the NUL, or 8
       len += l;
       ended = l < 8;
@@ -127,7 +127,7 @@ int signature(const unsigned char* text, uint64_t text_bytes, const uint32_t* in
     g_state.device = dev;
   }
   k_signature<<<1, kSigStrings, 0, st>>>(reinterpret_cast<const uint64_t*>(text),
-                                         (text_bytes + 7) / 8, in, n, g_state.d_sig);
+                                         text_bytes, in, n, g_state.d_sig);
   AT_HIP(hipGetLastError());
   AT_HIP(hipMemcpyAsync(g_state.h_sig, g_state.d_sig, sizeof(SigOut), hipMemcpyDeviceToHost, st));
   AT_HIP(hipStreamSynchronize(st));
@@ -225,6 +225,7 @@ int gss_autotune_device32(const unsigned char* d_text, uint64_t text_bytes, cons
   *out = gss_autotune_result{};
   if (!d_text || !d_off) return fail(GSS_E_INVALID, "NULL text or offsets");
   if (n < 2) return fail(GSS_E_INVALID, "nothing to tune on fewer than two strings");
+  if (text_bytes == 0) return fail(GSS_E_INVALID, "empty text: every string starts inside it");
   if (reinterpret_cast<uintptr_t>(d_text) & 7)
     return fail(GSS_E_INVALID, "the device text must be 8-byte aligned");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);

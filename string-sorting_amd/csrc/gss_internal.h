@@ -115,7 +115,7 @@ constexpr uint32_t kLocalClassThreads[3] = {64, 256, 1024};
 // Pointers into the workspace for one sort.
 struct Plan {
   const uint64_t* text64;
-  uint64_t nwords;       // readable 8-byte words of the text
+  uint64_t text_bytes;   // text length: bytes at or beyond it read as 0 (>= 1)
   uint32_t* idx[3];      // index buffers: [0] = input, [1], [2] ping-pong
   uint32_t* out;         // sorted offsets
   uint64_t* lcp;         // NULL or n-1 LCPs
@@ -152,13 +152,17 @@ struct Plan {
 // ---------------------------------------------------------------------------
 // Device primitives
 
-// Aligned text word `w` (little-endian), 0 beyond the text.  The address is
-// clamped instead of branched around, so a run of these loads issues back to
-// back and waits once.  nwords >= 1.
+// Aligned text word `w` (little-endian) of a text of tlen >= 1 bytes; bytes
+// at or beyond tlen read as 0, also inside the last, partial word, so a
+// string that runs off the end is cut there as if a NUL followed.  The
+// address is clamped instead of branched around, so a run of these loads
+// issues back to back and waits once.
 __device__ __forceinline__ uint64_t text_word(const uint64_t* __restrict__ text64,
-                                              uint64_t nwords, uint64_t w) {
-  const uint64_t v = text64[w < nwords ? w : nwords - 1];
-  return w < nwords ? v : 0ull;
+                                              uint64_t tlen, uint64_t w) {
+  const uint64_t nfull = tlen >> 3, last = (tlen - 1) >> 3;
+  const uint64_t v = text64[w < last ? w : last];
+  if (w < nfull) return v;
+  return w == nfull ? v & ((1ull << (8 * (tlen & 7))) - 1ull) : 0ull;
 }
 
 // Clears every byte after the first NUL of little-endian `v`; *ended tells
@@ -180,12 +184,13 @@ struct __attribute__((aligned(8))) TextPair {
   uint64_t lo, hi;
 };
 
-// Aligned text words [w, w+NW): 16-byte loads when all lie inside the text,
-// word by word (0 beyond the end) only in the last NW words of the text.
+// Aligned text words [w, w+NW) of a text of tlen bytes: 16-byte loads when
+// all are whole words inside the text, word by word (bytes at or beyond
+// tlen read as 0) only near the end of the text.
 template <int NW>
-__device__ __forceinline__ void text_words(const uint64_t* __restrict__ text64, uint64_t nwords,
+__device__ __forceinline__ void text_words(const uint64_t* __restrict__ text64, uint64_t tlen,
                                            uint64_t w, uint64_t (&raw)[NW]) {
-  if (w + NW <= nwords) {
+  if (w + NW <= (tlen >> 3)) {
 #pragma unroll
     for (int q = 0; q + 1 < NW; q += 2) {
       const TextPair pr = *reinterpret_cast<const TextPair*>(text64 + w + q);
@@ -195,7 +200,7 @@ __device__ __forceinline__ void text_words(const uint64_t* __restrict__ text64, 
     if (NW & 1) raw[NW - 1] = text64[w + NW - 1];
   } else {
 #pragma unroll
-    for (int q = 0; q < NW; ++q) raw[q] = text_word(text64, nwords, w + q);
+    for (int q = 0; q < NW; ++q) raw[q] = text_word(text64, tlen, w + q);
   }
 }
 
@@ -203,10 +208,10 @@ __device__ __forceinline__ void text_words(const uint64_t* __restrict__ text64, 
 // significant), zero after the first NUL; words beyond the text read as 0.
 // Same packing as fill_cache in src/multikey_cache.cpp:107-125.
 __device__ __forceinline__ uint64_t load_key(const uint64_t* __restrict__ text64,
-                                             uint64_t nwords, uint64_t pos) {
+                                             uint64_t tlen, uint64_t pos) {
   const uint32_t sh = (uint32_t)(pos & 7) * 8;
   uint64_t raw[2];
-  text_words<2>(text64, nwords, pos >> 3, raw);
+  text_words<2>(text64, tlen, pos >> 3, raw);
   bool ended;
   return __builtin_bswap64(cut_at_nul(funnel(raw[0], raw[1], sh), &ended));
 }
@@ -215,11 +220,11 @@ __device__ __forceinline__ uint64_t load_key(const uint64_t* __restrict__ text64
 // [pos+8q, pos+8q+8); every byte after the first NUL is zero, so the keys of
 // a string that ended are 0.
 template <int KW>
-__device__ __forceinline__ void load_keys(const uint64_t* __restrict__ text64, uint64_t nwords,
+__device__ __forceinline__ void load_keys(const uint64_t* __restrict__ text64, uint64_t tlen,
                                           uint64_t pos, uint64_t (&out)[KW]) {
   const uint32_t sh = (uint32_t)(pos & 7) * 8;
   uint64_t raw[KW + 1];
-  text_words<KW + 1>(text64, nwords, pos >> 3, raw);
+  text_words<KW + 1>(text64, tlen, pos >> 3, raw);
   bool done = false;
 #pragma unroll
   for (int q = 0; q < KW; ++q) {
@@ -416,6 +421,21 @@ hipError_t launch_publish(const void* a, size_t a_bytes, const void* b, const ui
 hipError_t launch_off_to_orig_ptr(const uint32_t* off, const uint32_t* starts,
                                   const uint64_t* orig, uint64_t* ptrs, uint64_t n,
                                   hipStream_t s);
+
+// 2-way LCP merge (pipeline.hip).  merge_splits(total) is the number of
+// merge-path split entries a merge of `total` strings uses; merge_launch
+// queues the partition and merge kernels on `st` (no sync), with `split` a
+// device array of at least merge_splits(na + nb) entries, each written
+// before it is read.  merge_check validates the arguments of a merge entry
+// point (GSS_OK or the GSS_E_* code, with the message set).
+uint64_t merge_splits(uint64_t total);
+hipError_t merge_launch(const uint64_t* t64, uint64_t tlen, const uint32_t* a,
+                        const uint64_t* alcp, uint64_t na, const uint32_t* b,
+                        const uint64_t* blcp, uint64_t nb, uint32_t* out, uint64_t* olcp,
+                        uint64_t* split, hipStream_t st);
+int merge_check(const unsigned char* d_text, uint64_t text_bytes, const uint32_t* d_a,
+                const uint64_t* d_a_lcp, uint64_t na, const uint32_t* d_b,
+                const uint64_t* d_b_lcp, uint64_t nb, const uint32_t* d_out);
 
 // One piece of a multi-array copy: `words` 32-bit words from src to dst.
 // The served loop gathers its requests' offsets into one batch array and

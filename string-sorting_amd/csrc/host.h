@@ -40,12 +40,17 @@ gss_tuning current_tuning();
 int autotune_choose(const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
                     uint64_t n, hipStream_t stream, gss_tuning* out, bool* use);
 
-// Device bytes of a context's workspace sized for n strings under the
-// current tuning.
-size_t workspace_bytes(uint64_t n);
+// Device bytes of a context's workspace sized for n strings under `tuning`.
+size_t workspace_bytes(uint64_t n, const gss_tuning& tuning);
 
-// Sizes a context's workspace for n strings and its initial request lists
-// for nreq requests (0: leave them).
-int ctx_reserve(gss_ctx* ctx, uint64_t n, uint32_t nreq);
+// Sizes a context's workspace for n strings under `tuning` and its initial
+// request lists for nreq requests (0: leave them).
+int ctx_reserve(gss_ctx* ctx, uint64_t n, uint32_t nreq, const gss_tuning& tuning);
+
+// gss_ctx_sort_batch_device32 with an explicit tuning instead of the current
+// one (the served loop keeps the tuning its slots were sized for).
+int ctx_sort_batch_tuned(gss_ctx* ctx, const unsigned char* text, uint64_t text_bytes,
+                         const uint32_t* in, uint32_t* out, uint64_t* lcp, const uint64_t* req,
+                         uint32_t nreq, hipStream_t stream, const gss_tuning& tuning);
 
 }  // namespace gss

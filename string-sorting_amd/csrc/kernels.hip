@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const Bi
 #pragma unroll
     for (int j = 0; j < kTileItems; ++j) {
       uint64_t kk[2];
-      load_keys<2>(p.text64, p.nwords, (uint64_t)o[j] + sg.depth, kk);
+      load_keys<2>(p.text64, p.text_bytes, (uint64_t)o[j] + sg.depth, kk);
       k[j] = kk[0];
       x[j] = kk[1];
     }
@@ -908,8 +908,8 @@ __global__ void k_boundary_lcp(Plan p, uint32_t first) {
   for (uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const Boundary b = p.bounds[i];
     const uint32_t* ix = p.idx[b.buf];
-    const uint64_t k1 = load_key(p.text64, p.nwords, (uint64_t)ix[b.pos] + b.depth);
-    const uint64_t k2 = load_key(p.text64, p.nwords, (uint64_t)ix[b.pos + 1] + b.depth);
+    const uint64_t k1 = load_key(p.text64, p.text_bytes, (uint64_t)ix[b.pos] + b.depth);
+    const uint64_t k2 = load_key(p.text64, p.text_bytes, (uint64_t)ix[b.pos + 1] + b.depth);
     p.lcp[b.pos] = (uint64_t)b.depth + key_lcp(k1, k2);
   }
 }

@@ -357,7 +357,7 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
         const uint32_t g = v ? s.sid[i] : 0u;
         const uint32_t o = v ? s.off[i] : 0u;
         gv[j] = v ? g : 0xFFFFu;
-        load_keys<KW>(p.text64, p.nwords, (uint64_t)o + s.ss_depth[g], kw[j]);
+        load_keys<KW>(p.text64, p.text_bytes, (uint64_t)o + s.ss_depth[g], kw[j]);
       }
     }
     if (lane == kWave - 1) {
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
       }
       for (uint32_t g = threadIdx.x; g < nss; g += BLOCK) s.ss_depth[g] += 8 * KW;
       __syncthreads();
-      if ((uint64_t)rounds > p.nwords + 2) {
+      if ((uint64_t)rounds > (p.text_bytes >> 3) + 3) {
         if (threadIdx.x == 0) atomicOr(&p.ctr->error, 32u);
         break;
       }
@@ -619,7 +619,7 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
     if (nact == 0) break;
     // Every round moves each group >= 8 bytes deeper and no string reaches
     // past the text, so the rounds are bounded; stop rather than spin.
-    if ((uint64_t)rounds > p.nwords + 2) {
+    if ((uint64_t)rounds > (p.text_bytes >> 3) + 3) {
       if (threadIdx.x == 0) atomicOr(&p.ctr->error, 32u);
       break;
     }

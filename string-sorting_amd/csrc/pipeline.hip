@@ -83,14 +83,13 @@ __device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* scra
 // bytes, loaded 16 bytes at a time); bytes at or beyond len never match.
 __device__ __forceinline__ void ingest_flags(const uint64_t* __restrict__ t64, uint64_t len,
                                              uint64_t pat, uint64_t w0, uint64_t (&f)[kIngWords]) {
-  const uint64_t nwords = (len + 7) / 8;
   uint64_t raw[kIngWords];
   if (w0 * 8 >= len) {
 #pragma unroll
     for (int q = 0; q < kIngWords; ++q) f[q] = 0;
     return;
   }
-  text_words<kIngWords>(t64, nwords, w0, raw);
+  text_words<kIngWords>(t64, len, w0, raw);
 #pragma unroll
   for (int q = 0; q < kIngWords; ++q) {
     const uint64_t b = (w0 + q) * 8;
@@ -243,7 +242,7 @@ __global__ __launch_bounds__(256) void k_strlen(const uint64_t* __restrict__ t64
     uint64_t end = len;
     while (w < nwords) {
       uint64_t raw[8];
-      text_words<8>(t64, nwords, w, raw);
+      text_words<8>(t64, len, w, raw);
       bool found = false;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
@@ -303,7 +302,6 @@ __global__ __launch_bounds__(256) void k_write(const uint64_t* __restrict__ t64,
                                               const uint64_t* __restrict__ pos, uint64_t n,
                                               const uint32_t* __restrict__ first,
                                               uint64_t total, unsigned char* __restrict__ out) {
-  const uint64_t nwords = (len + 7) / 8;
   const uint64_t nblk = (total + 15) / 16;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nblk; b += stride) {
@@ -318,7 +316,7 @@ __global__ __launch_bounds__(256) void k_write(const uint64_t* __restrict__ t64,
       // Up to 16 string bytes starting at byte (p - ps) of string s.
       const uint64_t src = (uint64_t)off[s] + (p - ps);
       uint64_t raw[3];
-      text_words<3>(t64, nwords, src >> 3, raw);
+      text_words<3>(t64, len, src >> 3, raw);
       const uint32_t sh = (uint32_t)(src & 7) * 8;
       const U128 v{funnel(raw[0], raw[1], sh), funnel(raw[1], raw[2], sh)};
       const uint32_t a = (uint32_t)(p - p0);  // first output byte of this string here
@@ -363,10 +361,10 @@ struct StrCmp {
 
 // Compares the strings at byte offsets x and y, known equal on their first
 // h bytes, 8 bytes at a time (the key packing of multikey_cache).
-__device__ __forceinline__ StrCmp str_compare(const uint64_t* __restrict__ t64, uint64_t nwords,
+__device__ __forceinline__ StrCmp str_compare(const uint64_t* __restrict__ t64, uint64_t tlen,
                                               uint64_t x, uint64_t y, uint64_t h) {
   for (uint64_t d = h;; d += 8) {
-    const uint64_t kx = load_key(t64, nwords, x + d), ky = load_key(t64, nwords, y + d);
+    const uint64_t kx = load_key(t64, tlen, x + d), ky = load_key(t64, tlen, y + d);
     if (kx != ky) return StrCmp{kx < ky ? -1 : 1, d + key_lcp(kx, ky)};
     if ((kx & 0xFFu) == 0) return StrCmp{0, d + key_len(kx)};
   }
@@ -375,13 +373,13 @@ __device__ __forceinline__ StrCmp str_compare(const uint64_t* __restrict__ t64, 
 // Smallest i in [lo, hi] with A[i] > B[d-1-i] (ties A first): the number of
 // A strings among the first d outputs.  Every string compared is known to
 // share its first `depth` bytes with the others.
-__device__ __forceinline__ uint64_t merge_path(const uint64_t* __restrict__ t64, uint64_t nwords,
+__device__ __forceinline__ uint64_t merge_path(const uint64_t* __restrict__ t64, uint64_t tlen,
                                                const uint32_t* __restrict__ a,
                                                const uint32_t* __restrict__ b, uint64_t d,
                                                uint64_t lo, uint64_t hi, uint64_t depth = 0) {
   while (lo < hi) {
     const uint64_t mid = (lo + hi) >> 1;
-    if (str_compare(t64, nwords, a[mid], b[d - 1 - mid], depth).sign <= 0)
+    if (str_compare(t64, tlen, a[mid], b[d - 1 - mid], depth).sign <= 0)
       lo = mid + 1;
     else
       hi = mid;
@@ -391,7 +389,7 @@ __device__ __forceinline__ uint64_t merge_path(const uint64_t* __restrict__ t64,
 
 // split[p] = A strings among the first min(p * kMergeTile, na + nb) outputs.
 __global__ __launch_bounds__(256) void k_merge_partition(const uint64_t* __restrict__ t64,
-                                                        uint64_t nwords,
+                                                        uint64_t tlen,
                                                         const uint32_t* __restrict__ a, uint64_t na,
                                                         const uint32_t* __restrict__ b, uint64_t nb,
                                                         uint64_t nparts, uint64_t* __restrict__ split) {
@@ -400,7 +398,7 @@ __global__ __launch_bounds__(256) void k_merge_partition(const uint64_t* __restr
   const uint64_t total = na + nb;
   const uint64_t d = p * kMergeTile < total ? p * kMergeTile : total;
   const uint64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
-  split[p] = merge_path(t64, nwords, a, b, d, lo, hi);
+  split[p] = merge_path(t64, tlen, a, b, d, lo, hi);
 }
 
 // Each thread merges kMergeItems outputs from its own merge-path split and
@@ -415,7 +413,7 @@ __global__ __launch_bounds__(256) void k_merge_partition(const uint64_t* __restr
 // and the merge's comparisons are decided by the keys; the text is read
 // only past depth D + 8, for strings equal on those 8 bytes.
 __global__ __launch_bounds__(kMergeThreads) void k_merge(
-    const uint64_t* __restrict__ t64, uint64_t nwords, const uint32_t* __restrict__ a,
+    const uint64_t* __restrict__ t64, uint64_t tlen, const uint32_t* __restrict__ a,
     const uint64_t* __restrict__ alcp, uint64_t na, const uint32_t* __restrict__ b,
     const uint64_t* __restrict__ blcp, uint64_t nb, const uint64_t* __restrict__ split,
     uint32_t* __restrict__ out, uint64_t* __restrict__ olcp) {
@@ -432,18 +430,18 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
     uint64_t first, last;
     if (i0 == i1) first = b[j0];
     else if (j0 == j1) first = a[i0];
-    else first = str_compare(t64, nwords, a[i0], b[j0], 0).sign <= 0 ? a[i0] : b[j0];
+    else first = str_compare(t64, tlen, a[i0], b[j0], 0).sign <= 0 ? a[i0] : b[j0];
     if (i0 == i1) last = b[j1 - 1];
     else if (j0 == j1) last = a[i1 - 1];
-    else last = str_compare(t64, nwords, a[i1 - 1], b[j1 - 1], 0).sign <= 0 ? b[j1 - 1] : a[i1 - 1];
-    block_depth = str_compare(t64, nwords, first, last, 0).lcp;
+    else last = str_compare(t64, tlen, a[i1 - 1], b[j1 - 1], 0).sign <= 0 ? b[j1 - 1] : a[i1 - 1];
+    block_depth = str_compare(t64, tlen, first, last, 0).lcp;
   }
   __syncthreads();
   const uint64_t depth = block_depth;
   for (uint32_t x = threadIdx.x; x < ia + jb; x += kMergeThreads) {
     const uint32_t o = x < ia ? a[i0 + x] : b[j0 + (x - ia)];
     s_off[x] = o;
-    s_key[x] = load_key(t64, nwords, (uint64_t)o + depth);
+    s_key[x] = load_key(t64, tlen, (uint64_t)o + depth);
   }
   __syncthreads();
   // A[i] against B[j] for strings of the block (sharing `depth` bytes), as
@@ -452,7 +450,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
     const uint64_t ka = s_key[i - i0], kb = s_key[ia + (j - j0)];
     if (ka != kb) return StrCmp{ka < kb ? -1 : 1, depth + key_lcp(ka, kb)};
     if ((ka & 0xFFu) == 0) return StrCmp{0, depth + key_len(ka)};
-    return str_compare(t64, nwords, s_off[i - i0], s_off[ia + (j - j0)], depth + 8);
+    return str_compare(t64, tlen, s_off[i - i0], s_off[ia + (j - j0)], depth + 8);
   };
   const uint64_t d = k0 + (uint64_t)threadIdx.x * kMergeItems;
   if (d >= k1) return;
@@ -485,7 +483,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
       take_a = true;
       lprev = ha;
     } else if (!have_s) {
-      const StrCmp c = inside ? cmp_block(i, j) : str_compare(t64, nwords, a[i], b[j], 0);
+      const StrCmp c = inside ? cmp_block(i, j) : str_compare(t64, tlen, a[i], b[j], 0);
       take_a = c.sign <= 0;
       if (take_a) hb = c.lcp; else ha = c.lcp;
     } else if (ha != hb) {
@@ -494,7 +492,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
     } else {
       // Both share ha >= depth bytes with the last output, hence with each
       // other: the keys at `depth` decide as a comparison from ha would.
-      const StrCmp c = inside ? cmp_block(i, j) : str_compare(t64, nwords, a[i], b[j], ha);
+      const StrCmp c = inside ? cmp_block(i, j) : str_compare(t64, tlen, a[i], b[j], ha);
       take_a = c.sign <= 0;
       lprev = ha;
       if (take_a) hb = c.lcp; else ha = c.lcp;
@@ -638,25 +636,36 @@ int gss_ingest_device32(unsigned char* d_text, uint64_t text_bytes, int delim, u
 }  // extern "C"
 
 namespace gss {
-namespace {
 
-// Queues one 2-way merge on `st` (no sync); `split` holds at least
-// merge_splits(na + nb) entries.
 uint64_t merge_splits(uint64_t total) { return (total + kMergeTile - 1) / kMergeTile + 1; }
 
-hipError_t merge_launch(const uint64_t* t64, uint64_t nwords, const uint32_t* a,
+hipError_t merge_launch(const uint64_t* t64, uint64_t tlen, const uint32_t* a,
                         const uint64_t* alcp, uint64_t na, const uint32_t* b,
                         const uint64_t* blcp, uint64_t nb, uint32_t* out, uint64_t* olcp,
                         uint64_t* split, hipStream_t st) {
   const uint64_t nparts = merge_splits(na + nb) - 1;
-  k_merge_partition<<<(uint32_t)((nparts + 1 + 255) / 256), 256, 0, st>>>(t64, nwords, a, na, b,
+  k_merge_partition<<<(uint32_t)((nparts + 1 + 255) / 256), 256, 0, st>>>(t64, tlen, a, na, b,
                                                                          nb, nparts, split);
-  k_merge<<<(uint32_t)nparts, kMergeThreads, 0, st>>>(t64, nwords, a, alcp, na, b, blcp, nb,
+  k_merge<<<(uint32_t)nparts, kMergeThreads, 0, st>>>(t64, tlen, a, alcp, na, b, blcp, nb,
                                                       split, out, olcp);
   return hipGetLastError();
 }
 
-}  // namespace
+int merge_check(const unsigned char* d_text, uint64_t text_bytes, const uint32_t* d_a,
+                const uint64_t* d_a_lcp, uint64_t na, const uint32_t* d_b,
+                const uint64_t* d_b_lcp, uint64_t nb, const uint32_t* d_out) {
+  const uint64_t total = na + nb;
+  if (!d_text || !d_out || (na && !d_a) || (nb && !d_b))
+    return fail(GSS_E_INVALID, "NULL text, run or output pointer");
+  if ((na > 1 && !d_a_lcp) || (nb > 1 && !d_b_lcp))
+    return fail(GSS_E_INVALID, "each run of two or more strings needs its LCP array");
+  if (text_bytes == 0) return fail(GSS_E_INVALID, "empty text: every string starts inside it");
+  if (reinterpret_cast<uintptr_t>(d_text) & 7)
+    return fail(GSS_E_INVALID, "the device text must be 8-byte aligned");
+  if (total >= (1ull << 32)) return fail(GSS_E_TOO_LARGE, "at most 2^32-1 strings");
+  return GSS_OK;
+}
+
 }  // namespace gss
 
 extern "C" {
@@ -667,13 +676,8 @@ int gss_merge_lcp_device32(const unsigned char* d_text, uint64_t text_bytes, con
                            uint64_t* d_out_lcp, void* stream) {
   const uint64_t total = na + nb;
   if (total == 0) return GSS_OK;
-  if (!d_text || !d_out || (na && !d_a) || (nb && !d_b))
-    return fail(GSS_E_INVALID, "NULL text, run or output pointer");
-  if ((na > 1 && !d_a_lcp) || (nb > 1 && !d_b_lcp))
-    return fail(GSS_E_INVALID, "each run of two or more strings needs its LCP array");
-  if (reinterpret_cast<uintptr_t>(d_text) & 7)
-    return fail(GSS_E_INVALID, "the device text must be 8-byte aligned");
-  if (total >= (1ull << 32)) return fail(GSS_E_TOO_LARGE, "at most 2^32-1 strings");
+  const int vrc = merge_check(d_text, text_bytes, d_a, d_a_lcp, na, d_b, d_b_lcp, nb, d_out);
+  if (vrc) return vrc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   Scratch* sc = nullptr;
   int rc = scratch_for(&sc);
@@ -681,7 +685,7 @@ int gss_merge_lcp_device32(const unsigned char* d_text, uint64_t text_bytes, con
   std::lock_guard<std::mutex> lk(sc->mu);
   rc = scratch_reserve(*sc, al256(8 * merge_splits(total)), st);
   if (rc) return rc;
-  PL_HIP(merge_launch(reinterpret_cast<const uint64_t*>(d_text), (text_bytes + 7) / 8, d_a,
+  PL_HIP(merge_launch(reinterpret_cast<const uint64_t*>(d_text), text_bytes, d_a,
                       d_a_lcp, na, d_b, d_b_lcp, nb, d_out, d_out_lcp,
                       reinterpret_cast<uint64_t*>(sc->dev), st));
   PL_HIP(hipStreamSynchronize(st));
@@ -703,6 +707,7 @@ int gss_merge_lcp_kway_device32(const unsigned char* d_text, uint64_t text_bytes
   }
   if (total == 0) return GSS_OK;
   if (!d_text || !d_out) return fail(GSS_E_INVALID, "NULL text or output pointer");
+  if (text_bytes == 0) return fail(GSS_E_INVALID, "empty text: every string starts inside it");
   if (reinterpret_cast<uintptr_t>(d_text) & 7)
     return fail(GSS_E_INVALID, "the device text must be 8-byte aligned");
   if (total >= (1ull << 32)) return fail(GSS_E_TOO_LARGE, "at most 2^32-1 strings");
@@ -741,7 +746,7 @@ int gss_merge_lcp_kway_device32(const unsigned char* d_text, uint64_t text_bytes
                         reinterpret_cast<uint64_t*>(base + 2 * b_off + b_lcp)};
   uint64_t* split = reinterpret_cast<uint64_t*>(base + 2 * b_off + 2 * b_lcp);
   const uint64_t* t64 = reinterpret_cast<const uint64_t*>(d_text);
-  const uint64_t nwords = (text_bytes + 7) / 8;
+  const uint64_t tlen = text_bytes;
   // Rounds of pairwise merges of adjacent runs (run 2i with 2i + 1), so
   // that equal strings keep the order of their runs; the last round writes
   // the output.  The rounds run in stream order, so one round's buffer and
@@ -755,7 +760,7 @@ int gss_merge_lcp_kway_device32(const unsigned char* d_text, uint64_t text_bytes
     for (size_t r = 0; r < cur.size(); r += 2) {
       if (r + 1 < cur.size()) {
         const uint64_t len = cur[r].len + cur[r + 1].len;
-        PL_HIP(merge_launch(t64, nwords, cur[r].off, cur[r].lcp, cur[r].len, cur[r + 1].off,
+        PL_HIP(merge_launch(t64, tlen, cur[r].off, cur[r].lcp, cur[r].len, cur[r + 1].off,
                             cur[r + 1].lcp, cur[r + 1].len, o_off + p,
                             o_lcp ? o_lcp + p : nullptr, split + sp, st));
         next.push_back(Run{o_off + p, o_lcp ? o_lcp + p : nullptr, len});
@@ -794,6 +799,7 @@ int gss_write_device32(const unsigned char* d_text, uint64_t text_bytes, const u
   if (!out_bytes) return fail(GSS_E_INVALID, "NULL out_bytes pointer");
   if (n == 0) return GSS_OK;
   if (!d_text || !d_off) return fail(GSS_E_INVALID, "NULL text or offsets");
+  if (text_bytes == 0) return fail(GSS_E_INVALID, "empty text: every string starts inside it");
   if (reinterpret_cast<uintptr_t>(d_text) & 7)
     return fail(GSS_E_INVALID, "the device text must be 8-byte aligned");
   if (d_out && (reinterpret_cast<uintptr_t>(d_out) & 15))

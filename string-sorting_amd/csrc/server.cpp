@@ -10,6 +10,12 @@
 // one kernel and completes the requests.  Several slots run at once, so
 // requests that arrive while batches are on the GPU are taken up as soon as a
 // slot finishes, and the slots' kernels overlap on the device.
+//
+// Stream order: a request may name a HIP event whose work must finish before
+// its offsets are read.  The event's state at submission is captured on the
+// server's own submit stream (wait on it, then record a pooled event), so a
+// later re-record of the caller's event does not move the dependency; the
+// slot stream waits for the pooled event before its gather.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,6 +47,7 @@ struct Request {
   uint32_t* out = nullptr;
   uint64_t* lcp = nullptr;
   uint64_t n = 0;
+  hipEvent_t ready = nullptr;  // pooled event captured at submission, or none
   Clock::time_point submitted;
   double latency_ms = 0;
   int status = GSS_OK;
@@ -76,8 +83,8 @@ size_t slot_buffer_bytes(uint64_t cap, uint32_t max_reqs) {
          align256(sizeof(CopyChunk) * chunk_cap_for(cap, max_reqs));
 }
 
-size_t slot_bytes(uint64_t cap, uint32_t max_reqs) {
-  return workspace_bytes(cap) + slot_buffer_bytes(cap, max_reqs);
+size_t slot_bytes(uint64_t cap, uint32_t max_reqs, const gss_tuning& tun) {
+  return workspace_bytes(cap, tun) + slot_buffer_bytes(cap, max_reqs);
 }
 
 double percentile(std::vector<double>& v, double p) {
@@ -97,7 +104,10 @@ struct gss_server {
   const unsigned char* text = nullptr;
   uint64_t text_bytes = 0;
   gss_server_config cfg{};
-  uint64_t cap = 0;  // strings one slot's batch may hold
+  gss_tuning tuning{};  // the tuning current at creation: every batch uses it
+  uint64_t cap = 0;     // strings one slot's batch may hold
+  hipStream_t submit_stream = nullptr;  // captures the requests' ready events
+  std::vector<hipEvent_t> free_events;  // pooled; guarded by mu
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<gss::Request*> queue;
@@ -119,6 +129,23 @@ namespace {
     if (e_ != hipSuccess)                                                          \
       return fail(GSS_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+// Queues the chunk-table upload, the gather, the sort and the scatter of a
+// batch whose chunk table (g entries, the first ngather of them the gather)
+// is in sl.h_chunks.
+int launch_batch(gss_server* s, Slot& sl, uint64_t g, uint64_t ngather, bool want_lcp) {
+  UploadBatch up;  // the chunk table, read by a kernel from coherent pinned memory
+  if (add_upload(up, sl.h_chunks, sl.d_chunks, 0, sizeof(CopyChunk) * g))
+    return fail(GSS_E_INTERNAL, "batch chunk table upload");
+  SRV_HIP(launch_upload(up, sl.stream));
+  SRV_HIP(launch_copy_chunks(sl.d_chunks, (uint32_t)ngather, sl.stream));
+  const int rc = ctx_sort_batch_tuned(sl.ctx, s->text, s->text_bytes, sl.d_in, sl.d_out,
+                                      want_lcp ? sl.d_lcp : nullptr, sl.req.data(),
+                                      (uint32_t)sl.batch.size(), sl.stream, s->tuning);
+  if (rc) return rc;
+  SRV_HIP(launch_copy_chunks(sl.d_chunks + ngather, (uint32_t)(g - ngather), sl.stream));
+  return GSS_OK;
+}
 
 // Gathers, sorts and scatters the slot's batch; returns its status.
 int run_batch(gss_server* s, Slot& sl) {
@@ -149,16 +176,15 @@ int run_batch(gss_server* s, Slot& sl) {
     }
   }
   if (g > sl.chunk_cap) return fail(GSS_E_INTERNAL, "batch chunk table overflow");
-  UploadBatch up;  // the chunk table, read by a kernel from coherent pinned memory
-  if (add_upload(up, sl.h_chunks, sl.d_chunks, 0, sizeof(CopyChunk) * g))
-    return fail(GSS_E_INTERNAL, "batch chunk table upload");
-  SRV_HIP(launch_upload(up, sl.stream));
-  SRV_HIP(launch_copy_chunks(sl.d_chunks, (uint32_t)ngather, sl.stream));
-  const int rc = gss_ctx_sort_batch_device32(sl.ctx, s->text, s->text_bytes, sl.d_in, sl.d_out,
-                                             want_lcp ? sl.d_lcp : nullptr, sl.req.data(),
-                                             (uint32_t)sl.batch.size(), sl.stream);
-  if (rc) return rc;
-  SRV_HIP(launch_copy_chunks(sl.d_chunks + ngather, (uint32_t)(g - ngather), sl.stream));
+  for (const Request* r : sl.batch)
+    if (r->ready) SRV_HIP(hipStreamWaitEvent(sl.stream, r->ready, 0));
+  // From the first launch on, an error still drains the stream before it is
+  // reported: queued copies read the chunk table and the requests' arrays.
+  const int rc = launch_batch(s, sl, g, ngather, want_lcp);
+  if (rc) {
+    (void)hipStreamSynchronize(sl.stream);
+    return rc;
+  }
   return wait_stream(sl.stream);
 }
 
@@ -168,6 +194,10 @@ void complete(gss_server* s, Slot& sl, int rc) {
   {
     std::lock_guard<std::mutex> lk(s->mu);
     for (Request* r : sl.batch) {
+      if (r->ready) {  // the batch's stream work is over: the event is free
+        s->free_events.push_back(r->ready);
+        r->ready = nullptr;
+      }
       r->status = rc;
       r->error = msg;
       r->latency_ms = std::chrono::duration<double, std::milli>(now - r->submitted).count();
@@ -218,11 +248,11 @@ void free_slot(Slot& sl) {
   sl.h_chunks = nullptr;
 }
 
-int init_slot(Slot& sl, uint64_t cap, uint32_t max_reqs) {
+int init_slot(Slot& sl, uint64_t cap, uint32_t max_reqs, const gss_tuning& tun) {
   sl.ctx = gss_ctx_create();
   if (!sl.ctx) return GSS_E_DEVICE;  // message set by gss_ctx_create
   sl.stream = reinterpret_cast<hipStream_t>(gss_ctx_stream(sl.ctx));
-  int rc = ctx_reserve(sl.ctx, cap, max_reqs);
+  int rc = ctx_reserve(sl.ctx, cap, max_reqs, tun);
   if (rc) return rc;
   sl.chunk_cap = chunk_cap_for(cap, max_reqs);
   const size_t bytes = slot_buffer_bytes(cap, max_reqs);
@@ -256,8 +286,15 @@ void shutdown(gss_server* s) {
   for (auto& sl : s->slots)
     if (sl->thread.joinable()) sl->thread.join();
   for (auto& sl : s->slots) free_slot(*sl);
-  for (auto& kv : s->reqs) delete kv.second;
+  for (auto& kv : s->reqs) {
+    if (kv.second->ready) s->free_events.push_back(kv.second->ready);
+    delete kv.second;
+  }
   s->reqs.clear();
+  for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
+  s->free_events.clear();
+  if (s->submit_stream) (void)hipStreamDestroy(s->submit_stream);
+  s->submit_stream = nullptr;
 }
 
 }  // namespace
@@ -285,16 +322,22 @@ gss_server* gss_server_create(const unsigned char* d_text, uint64_t text_bytes,
     fail(GSS_E_INVALID, "max_batch_requests must lie in [1, 2^20]");
     return nullptr;
   }
-  // Largest per-slot capacity whose buffers fit the budget.
+  if (text_bytes == 0) {
+    fail(GSS_E_INVALID, "empty server text");
+    return nullptr;
+  }
+  // Largest per-slot capacity whose buffers fit the budget, under the tuning
+  // current now (kept for the server's lifetime).
+  const gss_tuning tun = current_tuning();
   const uint64_t per_slot = cfg->hbm_budget / cfg->slots;
   uint64_t lo = 0, hi = (1ull << 31) - 1;
   while (lo < hi) {
     const uint64_t mid = lo + (hi - lo + 1) / 2;
-    if (slot_bytes(mid, cfg->max_batch_requests) <= per_slot) lo = mid; else hi = mid - 1;
+    if (slot_bytes(mid, cfg->max_batch_requests, tun) <= per_slot) lo = mid; else hi = mid - 1;
   }
   if (lo == 0) {
     fail(GSS_E_NOMEM, "hbm_budget is too small for one slot's workspace (" +
-                          std::to_string(slot_bytes(1, cfg->max_batch_requests)) +
+                          std::to_string(slot_bytes(1, cfg->max_batch_requests, tun)) +
                           " bytes per slot needed)");
     return nullptr;
   }
@@ -302,15 +345,18 @@ gss_server* gss_server_create(const unsigned char* d_text, uint64_t text_bytes,
   s->text = d_text;
   s->text_bytes = text_bytes;
   s->cfg = *cfg;
+  s->tuning = tun;
   s->cap = lo;
-  if (hipGetDevice(&s->device) != hipSuccess) {
-    fail(GSS_E_DEVICE, "hipGetDevice failed");
+  if (hipGetDevice(&s->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->submit_stream, hipStreamNonBlocking) != hipSuccess) {
+    fail(GSS_E_DEVICE, "hipGetDevice / hipStreamCreate failed");
+    s->submit_stream = nullptr;
     delete s;
     return nullptr;
   }
   for (uint32_t i = 0; i < cfg->slots; ++i) {
     s->slots.emplace_back(new Slot);
-    if (init_slot(*s->slots.back(), s->cap, cfg->max_batch_requests)) {
+    if (init_slot(*s->slots.back(), s->cap, cfg->max_batch_requests, s->tuning)) {
       const std::string msg = gss_last_error();
       shutdown(s);
       delete s;
@@ -323,7 +369,7 @@ gss_server* gss_server_create(const unsigned char* d_text, uint64_t text_bytes,
 }
 
 int64_t gss_server_submit(gss_server* s, const uint32_t* d_off, uint32_t* d_out, uint64_t* d_lcp,
-                          uint64_t n) {
+                          uint64_t n, void* ready_event) {
   if (!s) return fail(GSS_E_INVALID, "NULL server");
   if (n && (!d_off || !d_out)) return fail(GSS_E_INVALID, "NULL offsets or output");
   if (n > s->cap)
@@ -342,6 +388,28 @@ int64_t gss_server_submit(gss_server* s, const uint32_t* d_off, uint32_t* d_out,
     if (s->stopping) {
       delete r;
       return fail(GSS_E_INVALID, "server is shutting down");
+    }
+    if (ready_event && n) {
+      // The caller's event as recorded now: the submit stream waits for it
+      // and a pooled event records that point.
+      hipEvent_t ev = nullptr;
+      if (!s->free_events.empty()) {
+        ev = s->free_events.back();
+        s->free_events.pop_back();
+      } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        delete r;
+        return fail(GSS_E_DEVICE, "hipEventCreate failed");
+      }
+      const hipError_t e1 =
+          hipStreamWaitEvent(s->submit_stream, reinterpret_cast<hipEvent_t>(ready_event), 0);
+      const hipError_t e2 = e1 == hipSuccess ? hipEventRecord(ev, s->submit_stream) : e1;
+      if (e2 != hipSuccess) {
+        s->free_events.push_back(ev);
+        delete r;
+        return fail(GSS_E_DEVICE, std::string("capturing the ready event: ") +
+                                      hipGetErrorString(e2));
+      }
+      r->ready = ev;
     }
     id = s->next_id++;
     s->reqs.emplace(id, r);

@@ -389,6 +389,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   if (!text || !in || !out) return fail(GSS_E_INVALID, "NULL text, input or output pointer");
   if (reinterpret_cast<uintptr_t>(text) & 7)
     return fail(GSS_E_INVALID, "the device text must be 8-byte aligned");
+  if (text_bytes == 0) return fail(GSS_E_INVALID, "empty text: every string starts inside it");
   if (n >= (1ull << 31))
     return fail(GSS_E_TOO_LARGE, "at most 2^31-1 strings per sort on the 32-bit path");
   int dev = -1;
@@ -420,7 +421,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   Workspace& w = c.ws;
   Plan p = w.plan;
   p.text64 = reinterpret_cast<const uint64_t*>(text);
-  p.nwords = (text_bytes + 7) / 8;
+  p.text_bytes = text_bytes;
   p.out = out;
   p.lcp = lcp;
   p.local_threshold = T;
@@ -516,7 +517,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     uint32_t cur = 0, bound_first = 0, pass = 0;
     // Each pass either splits a segment or moves it >= 1 byte deeper, and no
     // string reaches past the text, so the pass count is bounded.
-    const uint64_t max_passes = p.nwords * 8 + 64;
+    const uint64_t max_passes = text_bytes + 64;
     while (nbig) {
       if (pass > max_passes) return fail(GSS_E_INTERNAL, "global passes do not terminate");
       uint32_t ntiles = 0;
@@ -671,6 +672,13 @@ int run_locked(Context& c, const unsigned char* text, uint64_t text_bytes, const
                hipStream_t stream, const gss_tuning* tuning = nullptr) {
   std::lock_guard<std::mutex> lk(c.mu);
   const int rc = run_sort(c, text, text_bytes, in, out, lcp, n, req, nreq, stream, tuning);
+  if (rc) {
+    // A failed sort may leave launches queued on both streams; they read the
+    // caller's arrays and the workspace, so they finish before the error is
+    // returned (and before the workspace can be reallocated).
+    (void)hipStreamSynchronize(stream ? stream : c.stream);
+    (void)hipStreamSynchronize(c.side);
+  }
   g_last_stats = c.stats;
   return rc;
 }
@@ -701,6 +709,7 @@ int library_stream(hipStream_t* out) {
 
 int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
                   uint32_t* out, uint64_t* lcp, uint64_t n, hipStream_t stream) {
+  if (n && text_bytes == 0) return fail(GSS_E_INVALID, "empty text: every string starts inside it");
   Context* c = nullptr;
   int rc = default_context(&c);
   if (rc) return rc;
@@ -720,6 +729,14 @@ int sort_device32_tuned(const unsigned char* text, uint64_t text_bytes, const ui
   return run_locked(*c, text, text_bytes, in, out, nullptr, n, nullptr, 0, stream, &tuning);
 }
 
+int ctx_sort_batch_tuned(gss_ctx* h, const unsigned char* text, uint64_t text_bytes,
+                         const uint32_t* in, uint32_t* out, uint64_t* lcp, const uint64_t* req,
+                         uint32_t nreq, hipStream_t stream, const gss_tuning& tuning) {
+  if (!h) return fail(GSS_E_INVALID, "NULL context");
+  if (!req) return fail(GSS_E_INVALID, "NULL request table");
+  return run_locked(h->c, text, text_bytes, in, out, lcp, req[nreq], req, nreq, stream, &tuning);
+}
+
 int check_tuning(const gss_tuning& t) {
   if (t.local_threshold < 64 || t.local_threshold > kLocalCapMax)
     return fail(GSS_E_INVALID, "local_threshold must lie in [64, " + std::to_string(kLocalCapMax) + "]");
@@ -734,30 +751,20 @@ gss_tuning current_tuning() {
   return g_tuning;
 }
 
-size_t workspace_bytes(uint64_t n) {
-  gss_tuning tun;
-  {
-    std::lock_guard<std::mutex> lk(g_mu);
-    tun = g_tuning;
-  }
+size_t workspace_bytes(uint64_t n, const gss_tuning& tun) {
   Plan p{};
   uint32_t* idx_copy = nullptr;
   return items_bytes(workspace_items(p, &idx_copy, caps_for(n, tun.local_threshold),
                                      tun.radix_bits));
 }
 
-int ctx_reserve(gss_ctx* h, uint64_t n, uint32_t nreq) {
+int ctx_reserve(gss_ctx* h, uint64_t n, uint32_t nreq, const gss_tuning& tun) {
   if (!h) return fail(GSS_E_INVALID, "NULL context");
   Context& c = h->c;
   std::lock_guard<std::mutex> lk(c.mu);
   int dev = -1;
   GSS_HIP(hipGetDevice(&dev));
   if (dev != c.device) return fail(GSS_E_INVALID, "the context belongs to another device");
-  gss_tuning tun;
-  {
-    std::lock_guard<std::mutex> lk2(g_mu);
-    tun = g_tuning;
-  }
   int rc = ensure_workspace(c, n, tun.radix_bits, tun.local_threshold);
   if (rc) return rc;
   return nreq ? ensure_init_lists(c.ws, nreq) : GSS_OK;
@@ -823,7 +830,7 @@ int gss_ctx_sort_batch_device32(gss_ctx* h, const unsigned char* d_text, uint64_
                     req_begin, nreq, reinterpret_cast<hipStream_t>(stream));
 }
 
-int gss_ctx_reserve(gss_ctx* h, uint64_t n) { return ctx_reserve(h, n, 0); }
+int gss_ctx_reserve(gss_ctx* h, uint64_t n) { return ctx_reserve(h, n, 0, current_tuning()); }
 
 int gss_ctx_get_stats(gss_ctx* h, gss_stats* out) {
   if (!h || !out) return fail(GSS_E_INVALID, "NULL context or stats pointer");

@@ -91,7 +91,7 @@ _SIGNATURES = {
     "gss_ctx_reserve": (C.c_int, [C.c_void_p, C.c_uint64]),
     "gss_server_create": (C.c_void_p, [C.c_void_p, C.c_uint64, C.POINTER(ServerConfig)]),
     "gss_server_submit": (C.c_int64, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                      C.c_uint64]),
+                                      C.c_uint64, C.c_void_p]),
     "gss_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_double)]),
     "gss_server_done": (C.c_int, [C.c_void_p, C.c_int64]),
     "gss_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
@@ -192,6 +192,40 @@ def sort_strings(strs, lcp: bool = False):
 # ---------------------------------------------------------------------------
 # Device-resident path
 
+def _check_text(text, text_bytes):
+    """The text must be a contiguous uint8 device tensor holding text_bytes
+    bytes (default: all of it); returns text_bytes."""
+    import torch
+
+    if not text.is_cuda or text.dtype != torch.uint8 or not text.is_contiguous():
+        raise GssError("text must be a contiguous uint8 device tensor")
+    tb = text.numel() if text_bytes is None else int(text_bytes)
+    if tb < 0 or tb > text.numel():
+        raise GssError(f"text_bytes={tb} does not fit the {text.numel()}-byte text tensor")
+    return tb
+
+
+def _check_offsets(t, device, what, n=None):
+    """A contiguous 32-bit tensor on `device` with at least n entries."""
+    import torch
+
+    if not t.is_cuda or t.device != device:
+        raise GssError(f"{what} must be a device tensor on {device}")
+    if t.dtype not in (torch.int32, torch.uint32) or not t.is_contiguous():
+        raise GssError(f"{what} must be a contiguous 32-bit tensor")
+    if n is not None and t.numel() < n:
+        raise GssError(f"{what} must hold at least {n} entries (has {t.numel()})")
+
+
+def _check_lcp(lcp, device, n):
+    """A contiguous 64-bit tensor on `device` with at least n-1 entries."""
+    if (not lcp.is_cuda or lcp.device != device or lcp.element_size() != 8
+            or lcp.is_floating_point() or not lcp.is_contiguous()
+            or lcp.numel() < max(0, n - 1)):
+        raise GssError(f"lcp must be a contiguous 64-bit integer tensor of at least "
+                       f"{max(0, n - 1)} entries on {device}")
+
+
 def sort_device(text, offsets, out=None, lcp=None, text_bytes=None, stream=None):
     """Sorts the strings at uint32 `offsets` into uint8 `text`, both torch
     tensors on the GPU.  Writes the sorted offsets to `out` (allocated when
@@ -199,21 +233,17 @@ def sort_device(text, offsets, out=None, lcp=None, text_bytes=None, stream=None)
     Returns `out`."""
     import torch
 
-    if not (text.is_cuda and offsets.is_cuda):
-        raise GssError("text and offsets must be device tensors")
-    if offsets.dtype not in (torch.int32, torch.uint32):
-        raise GssError("offsets must be 32-bit")
+    text_bytes = _check_text(text, text_bytes)
+    _check_offsets(offsets, text.device, "offsets")
     n = offsets.numel()
     if out is None:
         out = torch.empty_like(offsets)
-    if text_bytes is None:
-        text_bytes = text.numel()
+    _check_offsets(out, text.device, "out", n)
     if stream is None:
         stream = torch.cuda.current_stream(text.device)
     lp = 0
     if lcp is not None:
-        if lcp.numel() < max(0, n - 1) or lcp.element_size() != 8:
-            raise GssError("lcp must hold n-1 64-bit entries")
+        _check_lcp(lcp, text.device, n)
         lp = lcp.data_ptr()
     rc = lib().gss_sort_device32(text.data_ptr(), int(text_bytes), offsets.data_ptr(),
                                  out.data_ptr(), lp if lp else None, n, stream.cuda_stream)
@@ -234,16 +264,17 @@ def sort_batch(text, offsets, req_begin, out=None, lcp=None, text_bytes=None, st
     `out`.  `ctx` is a Context (None = the default context)."""
     import torch
 
-    if not (text.is_cuda and offsets.is_cuda):
-        raise GssError("text and offsets must be device tensors")
+    text_bytes = _check_text(text, text_bytes)
+    _check_offsets(offsets, text.device, "offsets")
     n = offsets.numel()
     nreq = len(req_begin) - 1
     if nreq < 0 or int(req_begin[0]) != 0 or int(req_begin[-1]) != n:
         raise GssError("req_begin must run from 0 to the number of offsets")
     if out is None:
         out = torch.empty_like(offsets)
-    if text_bytes is None:
-        text_bytes = text.numel()
+    _check_offsets(out, text.device, "out", n)
+    if lcp is not None:
+        _check_lcp(lcp, text.device, n)
     lp = lcp.data_ptr() if lcp is not None else None
     table = _req_table(req_begin)
     if ctx is None:
@@ -400,11 +431,13 @@ class Server:
     """The served request loop (gss_server_* in gss.h) over one device text.
 
     ``submit`` queues a sort of a uint32 offset tensor (a contiguous device
-    tensor, e.g. a slice of a pool) into an output tensor of the same size
-    (the server's streams do not wait for torch's: work that writes these
-    tensors must have finished, e.g. after ``torch.cuda.synchronize()``);
-    slots take queued requests in order, fold as many as fit into one batch
-    and complete them together.  ``wait`` returns the request's latency in ms.
+    tensor, e.g. a slice of a pool) into an output tensor of the same size.
+    With ``ready`` (a recorded ``torch.cuda.Event``) the request's device work
+    waits for the work that event marks, e.g. the kernel that wrote the
+    offsets on another stream; without it that work must have finished
+    before the call (e.g. ``torch.cuda.synchronize()``).  Slots take queued
+    requests in order, fold as many as fit into one batch and complete them
+    together.  ``wait`` returns the request's latency in ms.
     """
 
     def __init__(self, text, text_bytes=None, hbm_budget=8 << 30, slots=2,
@@ -419,16 +452,19 @@ class Server:
             raise GssError(f"gss_server_create failed: {last_error()}")
         self._keep = {}
 
-    def submit(self, offsets, out, lcp=None) -> int:
+    def submit(self, offsets, out, lcp=None, ready=None) -> int:
         n = offsets.numel()
-        if out.numel() < n:
-            raise GssError("output tensor too small")
-        if not (offsets.is_contiguous() and out.is_contiguous()):
-            raise GssError("offsets and out must be contiguous")
-        if lcp is not None and (lcp.numel() < max(0, n - 1) or lcp.element_size() != 8):
-            raise GssError("lcp must hold n-1 64-bit entries")
+        _check_offsets(offsets, self.text.device, "offsets")
+        _check_offsets(out, self.text.device, "out", n)
+        if lcp is not None:
+            _check_lcp(lcp, self.text.device, n)
+        ev = None
+        if ready is not None:
+            ev = getattr(ready, "cuda_event", 0) or None
+            if ev is None:
+                raise GssError("ready must be a recorded torch.cuda.Event")
         rid = lib().gss_server_submit(self.handle, offsets.data_ptr(), out.data_ptr(),
-                                      lcp.data_ptr() if lcp is not None else None, n)
+                                      lcp.data_ptr() if lcp is not None else None, n, ev)
         if rid < 0:
             raise GssError(f"gss_server_submit failed ({rid}): {last_error()}")
         self._keep[rid] = (offsets, out, lcp)

@@ -117,7 +117,7 @@ def test_server_create_validates_before_touching_a_device():
     cfg = gss.ServerConfig(1 << 30, 1, 16)
     assert not lib.gss_server_create(C.c_void_p(1 << 20 | 4), 4096, C.byref(cfg))
     assert "aligned" in gss.last_error()
-    assert lib.gss_server_submit(None, None, None, None, 5) < 0
+    assert lib.gss_server_submit(None, None, None, None, 5, None) < 0
     assert lib.gss_server_wait(None, 0, None) < 0
     assert lib.gss_server_done(None, 0) < 0
     lib.gss_server_destroy(None)

@@ -60,12 +60,71 @@ EDGE_CASES = {
 }
 
 
+def _descending(n, seed, dup_every=0):
+    """n strings in non-increasing order: the input on which msd_CE7 and
+    msd_CE8 take a descending bucket of 65,536 or more strings for sorted and
+    skip its distribution (src/msd_ce.cpp:444-463, 520-542)."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(3, 13, size=n)
+    strs = [bytes(rng.integers(97, 123, size=int(k)).astype(np.uint8)) for k in lens]
+    if dup_every:
+        strs = [strs[i - i % dup_every] for i in range(n)]
+    return sorted(strs, reverse=True)
+
+
+EDGE_CASES["reverse_sorted_70000"] = _descending(70_000, 1)
+EDGE_CASES["non_increasing_dups_66000"] = _descending(66_000, 2, dup_every=3)
+# Every string of the trap sharing one long prefix: a single descending
+# bucket of 70,000 strings below the first byte.
+EDGE_CASES["reverse_sorted_shared_prefix_70000"] = [b"prefix/" + s
+                                                     for s in _descending(70_000, 3)]
+
+
 @pytest.mark.parametrize("case", sorted(EDGE_CASES))
 def test_edge_cases(case, oracle_mod):
     ss = _set(EDGE_CASES[case], case)
     log(f"edge case {case} n={ss.n}")
     off, lcp = gpu_sort(ss)
     assert_matches_oracle(ss, off, lcp, oracle_mod)
+
+
+@pytest.mark.parametrize("n", [5, 3000, 60000])
+def test_unterminated_last_string(n, oracle_mod):
+    """The text ends inside the last string (no NUL) at a length that is not
+    a multiple of 8, and the bytes after text_bytes are 0xFF: the last
+    string is cut at text_bytes, as if a NUL followed, so it orders before
+    every string it is a proper prefix of.  Offsets and LCPs against the
+    oracle on a NUL-terminated copy."""
+    import torch
+
+    rng = np.random.default_rng(n)
+    strs = [b"abc" + bytes(rng.integers(97, 100, size=int(rng.integers(0, 9))).astype(np.uint8))
+            for _ in range(n - 1)]
+    last = b"abc" + b"a" * int(rng.integers(0, 3))
+    head = b"".join(s + b"\0" for s in strs)
+    if (len(head) + len(last)) % 8 == 0:  # the text must end inside a word
+        last += b"a"
+    body = head + last
+    starts = np.zeros(n, dtype=np.uint32)
+    pos = 0
+    for i, s in enumerate(strs + [last]):
+        starts[i] = pos
+        pos += len(s) + 1
+    tb = len(body)
+    assert tb % 8 != 0 and body[starts[-1]:] == last
+    dev = torch.device("cuda:0")
+    garbage = np.frombuffer(body + b"\xff" * 24, np.uint8).copy()
+    text = torch.from_numpy(garbage).to(dev)
+    off = torch.from_numpy(starts.view(np.int32)).to(dev)
+    out = torch.empty_like(off)
+    lc = torch.empty(max(1, n - 1), dtype=torch.int64, device=dev)
+    log(f"unterminated last string, n={n}, text_bytes={tb}")
+    gss.sort_device(text, off, out, lc, text_bytes=tb)
+    torch.cuda.synchronize()
+    clean = np.frombuffer(body + b"\0" * 24, np.uint8).copy()
+    want, want_lcp = oracle_mod.sort_offsets(clean, starts, "msd_ce2", lcp=True)
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), want)
+    assert np.array_equal(lc.cpu().numpy().view(np.uint64)[: n - 1], want_lcp)
 
 
 def test_empty_and_single():

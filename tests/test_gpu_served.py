@@ -132,3 +132,92 @@ def test_served_close_finishes_queued_requests(stream, pools_dev):
     for (a, n), o in zip(reqs, outs):
         alone, _ = _alone(text, off, a, n, tb)
         assert np.array_equal(o.cpu().numpy().view(np.uint32), alone)
+
+
+def _delay(stream_, ms):
+    """Queues about `ms` milliseconds of device work on `stream_` (a bounded
+    spin kernel where torch has one, else elementwise passes)."""
+    import torch
+
+    with torch.cuda.stream(stream_):
+        if hasattr(torch.cuda, "_sleep"):
+            torch.cuda._sleep(int(ms * 2.0e6))  # cycles at about 2 GHz
+        else:
+            x = torch.ones(1 << 26, device=device())
+            for _ in range(max(1, int(ms))):
+                x.mul_(1.0000001)
+
+
+def test_served_waits_for_ready_event(stream, pools_dev, oracle_mod):
+    """Stream order through the ready event: each request's offsets are
+    written on another torch stream behind a delay, the event is recorded
+    there and the request is submitted at once, with no host
+    synchronization.  The results must equal the oracle's.  The offset
+    buffers first hold valid offsets (all 0), so a server that ignored the
+    event would sort the wrong strings, never read out of bounds."""
+    import torch
+
+    text, off = pools_dev
+    tb = stream.pools.text_bytes
+    reqs = [(stream.first(i), int(stream.size[i])) for i in range(10)]
+    side = torch.cuda.Stream(device())
+    bufs = [torch.zeros(n, dtype=torch.int32, device=text.device) for _, n in reqs]
+    outs = [torch.full((n,), -1, dtype=torch.int32, device=text.device) for _, n in reqs]
+    torch.cuda.synchronize()  # the zero fills above are done; nothing below waits
+    with gss.Server(text, tb, hbm_budget=_budget_for(2, 300_000), slots=2,
+                    max_batch_requests=4) as srv:
+        ids, events = [], []
+        for k, (a, n) in enumerate(reqs):
+            _delay(side, 20 if k < 3 else 2)
+            with torch.cuda.stream(side):
+                bufs[k].copy_(off[a:a + n])
+            ev = torch.cuda.Event()
+            ev.record(side)
+            events.append(ev)
+            ids.append(srv.submit(bufs[k], outs[k], ready=ev))
+        log(f"ready events: {len(ids)} requests submitted without a host sync")
+        for i in ids:
+            srv.wait(i)
+    torch.cuda.synchronize()
+    for k, (a, n) in enumerate(reqs):
+        want = oracle_mod.sort_offsets(stream.pools.text, stream.pools.offsets[a:a + n], "msd_ce2")
+        got = outs[k].cpu().numpy().view(np.uint32)
+        assert np.array_equal(got, want), f"request {k} (n={n}) ignored its ready event"
+
+
+def test_request_stream_sample_at_scale(oracle_mod):
+    """The served request stream at its stated scale: pools of 4,000,000
+    strings per generator and requests of 1e3..4e6 strings.  A seeded sample
+    of the first 20,000 requests, always including the largest, is served
+    (several slots, batches folded) and every request is checked against the
+    oracle's msd_CE2, LCP arrays included."""
+    import torch
+
+    rs = gen.request_stream()
+    big = int(np.argmax(rs.size))
+    rng = np.random.default_rng(99)
+    pick = [big] + [int(i) for i in rng.choice(rs.nreq, size=23, replace=False) if i != big]
+    sizes = [int(rs.size[i]) for i in pick]
+    log(f"stream sample: {len(pick)} requests, largest {sizes[0]} strings, total {sum(sizes)}")
+    dev = device()
+    text = torch.from_numpy(rs.pools.text).to(dev)
+    off = torch.from_numpy(rs.pools.offsets.view(np.int32)).to(dev)
+    tb = rs.pools.text_bytes
+    outs = [torch.full((n,), -1, dtype=torch.int32, device=dev) for n in sizes]
+    lcps = [torch.full((max(1, n - 1),), -1, dtype=torch.int64, device=dev) for n in sizes]
+    torch.cuda.synchronize()
+    with gss.Server(text, tb, hbm_budget=16 << 30, slots=4, max_batch_requests=1024) as srv:
+        ids = [srv.submit(off[rs.first(i):rs.first(i) + n], outs[k], lcps[k])
+               for k, (i, n) in enumerate(zip(pick, sizes))]
+        for r in ids:
+            srv.wait(r)
+        st = srv.stats()
+        log(f"served {st.completed} requests in {st.batches} batches")
+    for k, (i, n) in enumerate(zip(pick, sizes)):
+        a = rs.first(i)
+        want, want_lcp = oracle_mod.sort_offsets(rs.pools.text, rs.pools.offsets[a:a + n],
+                                                 "msd_ce2", lcp=True)
+        assert np.array_equal(outs[k].cpu().numpy().view(np.uint32), want), f"request {i}"
+        assert np.array_equal(lcps[k].cpu().numpy().view(np.uint64)[: max(0, n - 1)],
+                              want_lcp), f"request {i} LCP"
+        log(f"request {i} (generator {int(rs.gen_id[i])}, {n} strings) equals msd_CE2")
