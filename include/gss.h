@@ -213,6 +213,23 @@ int gss_merge_lcp_device32(const unsigned char *d_text, uint64_t text_bytes,
 			   const uint32_t *d_b, const uint64_t *d_b_lcp, uint64_t nb,
 			   uint32_t *d_out, uint64_t *d_out_lcp, void *stream);
 
+/* Allocation-free variant of gss_merge_lcp_device32 (no counterpart in the
+ * reference: it is for callers that cannot allocate or block, such as code
+ * under HIP graph capture or a tight serving loop).  Same arguments, same
+ * output, bit for bit; but it allocates nothing (its one scratch array, the
+ * merge-path splits, is a fixed 1 MiB device array of the library, one per
+ * GPU, held until the process ends) and does not synchronize: it queues the
+ * merge on `stream` and returns, and the result is complete when the
+ * stream's work is.  It takes at most 536,866,816 strings (na + nb), beyond
+ * that it returns GSS_E_TOO_LARGE.  Calls on one GPU share that GPU's scratch
+ * array: calls queued in order on one stream are safe, calls whose work
+ * overlaps (other streams, or host threads racing on one stream) race. */
+int gss_merge_lcp_device32_noalloc(const unsigned char *d_text, uint64_t text_bytes,
+				   const uint32_t *d_a, const uint64_t *d_a_lcp,
+				   uint64_t na, const uint32_t *d_b,
+				   const uint64_t *d_b_lcp, uint64_t nb,
+				   uint32_t *d_out, uint64_t *d_out_lcp, void *stream);
+
 /* K-way LCP merge of sorted runs of one text, replacing merge_lcp_3way
  * (src/mergesort_lcp.cpp:399-709) for any number of runs: run r is
  * d_runs[r] (run_len[r] device offsets) with its LCP array d_run_lcps[r]

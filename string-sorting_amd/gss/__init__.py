@@ -104,6 +104,9 @@ _SIGNATURES = {
     "gss_merge_lcp_device32": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
                                          C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint64,
                                          C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gss_merge_lcp_device32_noalloc": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p,
+                                                 C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                                 C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gss_merge_lcp_kway_device32": (C.c_int, [C.c_void_p, C.c_uint64,
                                               C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                               C.POINTER(C.c_uint64), C.c_uint32, C.c_void_p,

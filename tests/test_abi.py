@@ -160,3 +160,27 @@ def test_autotune_switches_without_device():
     assert lib.gss_autotune_device32(C.c_void_p(1 << 20), 64, C.c_void_p(1 << 21), 1,
                                      C.byref(r), None) < 0
     assert "fewer than two" in gss.last_error()
+
+
+def test_noalloc_merge_validates_before_touching_a_device():
+    """The allocation-free merge checks its arguments and its size limit on
+    the host: more strings than its fixed scratch covers is GSS_E_TOO_LARGE
+    with a message naming the limit, an empty text or a missing LCP array is
+    GSS_E_INVALID, and no strings at all is a no-op.  None of these reach a
+    device (the pointers are never dereferenced)."""
+    import ctypes as C
+
+    lib = gss.lib()
+    fake = C.c_void_p(1 << 20)
+    na = nb = 300_000_000  # 6e8 strings > 536,866,816
+    assert lib.gss_merge_lcp_device32_noalloc(fake, 4096, fake, fake, na, fake, fake, nb, fake,
+                                              None, None) == -4
+    assert "at most 536866816 strings" in gss.last_error()
+    assert lib.gss_merge_lcp_device32_noalloc(fake, 0, fake, fake, 5, fake, fake, 5, fake,
+                                              None, None) == -1
+    assert "empty text" in gss.last_error()
+    assert lib.gss_merge_lcp_device32_noalloc(fake, 4096, fake, None, 5, fake, fake, 5, fake,
+                                              None, None) == -1
+    assert "LCP" in gss.last_error()
+    assert lib.gss_merge_lcp_device32_noalloc(None, 0, None, None, 0, None, None, 0, None,
+                                              None, None) == 0
