@@ -295,6 +295,12 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
   return v;
 }
 
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o, kWave));
+  return v;
+}
+
 __device__ __forceinline__ uint64_t wave_min64(uint64_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {

@@ -36,6 +36,15 @@ namespace gss {
 
 constexpr int kLocalWords = 4;
 
+// Words the largest class's first round reads per string (see k_local): 64
+// bytes, so a group whose strings share up to 56 more bytes (URL-like prefix
+// groups) is split in that round instead of after one or more rounds that
+// only find the next 32 bytes equal.  The smaller classes keep the ordinary
+// first round: with 32-byte windows the probe saves them no bytes, and its
+// registers cost them a wave per SIMD.
+constexpr int kProbeWords = 8;
+constexpr bool probes_first_round(uint32_t cap) { return cap > kLocalClassCap[1]; }
+
 // Largest group a round still sorts without the LSD passes over all items.
 // The 1024-thread class and the one-wave class rank every item inside its
 // group by counting (independent LDS reads, broadcast across the lanes of a
@@ -68,6 +77,7 @@ struct LocalSmem {
   uint8_t ss_stop[(SS + 3) & ~3];  // bit q: key q varies or holds a NUL in the group
   alignas(8) uint16_t whist[256][W];  // per-wave digit counters, digit-major
   uint64_t edge[W][kLocalWords];  // keys of each wave's last item
+  uint64_t refw[kProbeWords];  // first round: the reference string's window
   uint32_t wvm[W];   // per wave: varying key bytes | 0x100 if a group is > insertion_max
   uint32_t wtot[W];  // per wave: survivors | their group heads << 16
   uint32_t scan[W + 1];
@@ -297,8 +307,14 @@ __device__ __forceinline__ bool local_two_byte_count(LocalSmem<BLOCK, CAP>& s, u
 
 // COUNT: also count the 8-byte key levels the strings advance through (for
 // profiling only: the count costs registers).
+// Waves per SIMD each class is compiled for (the second launch bound): the
+// small classes run many workgroups per CU and keep their occupancy.
+constexpr int local_min_waves(uint32_t cap) {
+  return cap <= kLocalClassCap[0] ? 7 : (cap <= kLocalClassCap[1] ? 6 : 4);
+}
+
 template <int BLOCK, int CAP, bool COUNT>
-__global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
+__global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, const LocalSeg* segs) {
   constexpr int W = BLOCK / kWave;
   constexpr int IPT = CAP / BLOCK;
   constexpr int KW = kLocalWords;
@@ -343,9 +359,105 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
     const uint32_t jn = ipw / kWave;
     const uint32_t base = wid * ipw + lane;
 
+    uint32_t gv[IPT];  // sub-segment of the item (0xFFFF when inactive)
+    bool wave_big;     // some sub-segment holds more than insertion_max strings
+    if (probes_first_round(CAP) && rounds == 1) {
+      // 0. First round: one group, every string active.  Each string's PW
+      //    words at the group depth are compared with those of the
+      //    segment's first string (the reference): l_i = lcp(s_i, ref)
+      //    within the window, the equal bytes before the first mismatch or
+      //    NUL.  The group's LCP is G = min l_i (the LCP of a set is the
+      //    minimum over its members' LCPs with any one of them, and a NUL
+      //    ends every comparison), so the group's first word that differs
+      //    or holds a NUL is e = G / 8 -- the word the later rounds' stop
+      //    rule picks.  A string keeps only l_i and its own word at l_i / 8:
+      //    its word e is that one when l_i / 8 == e, else it equals the
+      //    reference's word e.  A window with no mismatch and no NUL moves
+      //    the group PW words deeper.
+      constexpr int PW = kProbeWords;
+#pragma unroll
+      for (int j = 0; j < IPT; ++j)
+        gv[j] = (uint32_t)j < jn && base + j * kWave < nact ? 0u : 0xFFFFu;
+      wave_big = n > insertion_max(CAP);
+      // l_i and the string's word at l_i / 8 wait in s.sid and s.key (sid
+      // is 0 for every string of a first round and is restored below).
+      // The reference window passes through LDS (registers are this class's
+      // limit).
+      uint32_t depth = sg.depth;
+      uint32_t G;
+      for (;;) {
+        if (threadIdx.x == 0) {
+          uint64_t rw[PW];
+          load_keys<PW>(p.text64, p.text_bytes, (uint64_t)s.off[0] + depth, rw);
+#pragma unroll
+          for (int q = 0; q < PW; ++q) s.refw[q] = rw[q];
+        }
+        __syncthreads();
+        uint32_t lmin = 8 * PW;
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+          if (gv[j] != 0xFFFFu) {
+            const uint32_t i = base + j * kWave;
+            uint64_t w[PW];
+            load_keys<PW>(p.text64, p.text_bytes, (uint64_t)s.off[i] + depth, w);
+            uint32_t l = 8 * PW;
+            uint64_t o = 0;
+#pragma unroll
+            for (int q = PW - 1; q >= 0; --q) {  // the first stop wins
+              const uint64_t r = s.refw[q];  // the same word for every lane
+              if (w[q] != r || (w[q] & 0xFFu) == 0) {
+                l = 8 * q + (w[q] != r ? key_lcp(w[q], r) : key_len(w[q]));
+                o = w[q];
+              }
+            }
+            s.sid[i] = (uint16_t)l;
+            s.key[i] = o;
+            lmin = min(lmin, l);
+          }
+        }
+        lmin = wave_min(lmin);
+        if (lane == 0) s.wtot[wid] = lmin;
+        __syncthreads();
+        G = 8 * PW;
+#pragma unroll
+        for (int w = 0; w < W; ++w) G = min(G, s.wtot[w]);
+        if (G < 8u * PW) break;
+        // Every string equals the reference on all PW words, with no NUL.
+        if (COUNT) {
+#pragma unroll
+          for (int j = 0; j < IPT; ++j)
+            if ((uint32_t)j < jn) wave_levels += PW * (uint64_t)__popcll(__ballot(gv[j] != 0xFFFFu));
+        }
+        depth += 8 * PW;
+        ++rounds;
+        round_strings += nact;
+        __syncthreads();  // s.refw and s.wtot are rewritten
+        if ((uint64_t)rounds > (p.text_bytes >> 3) + 3) break;
+      }
+      if (G >= 8u * PW) {  // the text ended without a NUL: cannot happen
+        if (threadIdx.x == 0) atomicOr(&p.ctr->error, 32u);
+        break;
+      }
+      const uint32_t e = G / 8;
+      const uint64_t ref_e = s.refw[e];
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        if (gv[j] != 0xFFFFu) {
+          const uint32_t i = base + j * kWave;
+          if (s.sid[i] / 8u > e) s.key[i] = ref_e;
+          s.sid[i] = 0;
+        }
+        if (COUNT && (uint32_t)j < jn)
+          wave_levels += (e + 1) * (uint64_t)__popcll(__ballot(gv[j] != 0xFFFFu));
+      }
+      if (threadIdx.x == 0) {
+        s.ss_depth[0] = depth + 8 * e;  // the chosen word is key 0 at this depth
+        s.ss_stop[0] = 1;
+      }
+      __syncthreads();
+    } else {
     // 1. Gather the next KW keys of every active string.
     uint64_t kw[IPT][KW];
-    uint32_t gv[IPT];  // sub-segment of the item (0xFFFF when inactive)
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
 #pragma unroll
@@ -400,7 +512,7 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
         }
       }
     }
-    const bool wave_big = __ballot(big) != 0;
+    wave_big = __ballot(big) != 0;
     __syncthreads();
 
     // A round in which no group found a differing key or a NUL only moves
@@ -449,6 +561,7 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
       }
     }
     __syncthreads();
+    }  // rounds after the first
 
     // 4. Bytes of the chosen keys that vary inside some sub-segment.
     uint32_t vm = 0;
@@ -468,7 +581,7 @@ __global__ __launch_bounds__(BLOCK) void k_local(Plan p, const LocalSeg* segs) {
     __syncthreads();
     vm = 0;
     for (int w = 0; w < W; ++w) vm |= s.wvm[w];
-    big = (vm & 0x100u) != 0;
+    const bool big = (vm & 0x100u) != 0;
     vm &= 0xFFu;
 
     // 5. Sort every sub-segment by its key (stable).
