@@ -50,7 +50,7 @@ HBM_PEAK_GBPS = 8000.0
 # Algorithmic bytes per active string per MSD round: read its 32-bit offset
 # and 8 text bytes, write the partitioned (8-byte key, offset) pair.
 BYTES_PER_ROUND_STRING = 2 * (4 + 8)  # per string and 8-byte key level
-KERNELS = ["gather", "segreduce", "layout", "histogram", "flat_scan", "segscan", "scatter",
+KERNELS = ["plan", "gather", "segreduce", "layout", "histogram", "flat_scan", "segscan", "scatter",
            "boundary", "retire", "local128", "local1024", "local6144", "local128_early",
            "local1024_early", "local6144_early"]
 PASSES = [f"pass{r}" for r in range(9)] + ["pass9+", "local_phase"]

@@ -83,18 +83,29 @@ struct Boundary {
   uint32_t pad;
 };
 
-// Device-side counters, zeroed at the start of every sort.
+// Device-side counters, set by the host at the start of every sort.  A
+// later pass's set-up kernel takes its segment count from big_next and sets
+// big_cur, ntiles and active_cur, which the pass's other kernels (and the
+// host, when it reads them) use; the first pass's come from the host.
 struct Counters {
-  uint32_t big_next;     // BigSegs emitted for the next pass
+  uint32_t big_next;     // BigSegs emitted for the next pass (initially: the first list)
   uint32_t local[3];     // LocalSegs per size class
   uint32_t retire;       // Retire chunks
   uint32_t boundary;     // Boundaries emitted so far
   uint32_t flat_len;     // entries of the current pass's flat count array
   uint32_t error;        // nonzero: a list overflowed or a loop bound was hit
   uint32_t local_strings[3];  // strings handed to each local size class
-  uint32_t pad;               // whole 8-byte words: copied in 8-byte words
+  uint32_t big_cur;      // BigSegs of the current pass
+  uint32_t ntiles;       // tiles of the current pass
+  uint32_t bound_done;   // boundaries whose LCPs earlier passes wrote
+  uint32_t active_cur;   // strings of the current pass's segments
+  uint32_t big_next_strings;  // strings of the BigSegs emitted for the next pass
 };
 static_assert(sizeof(Counters) % 8 == 0, "Counters is copied in 8-byte words");
+
+// Strings active in each global pass (the statistics of a sort), written by
+// the pass's set-up kernel.
+constexpr int kMaxPasses = 64;
 static_assert(sizeof(BigSeg) % 8 == 0, "BigSeg is copied in 8-byte words");
 
 // Statistics of the local kernels, spread over kStatSlots slots (block b
@@ -140,7 +151,8 @@ struct Plan {
   Retire* retire;
   Boundary* bounds;
   Counters* ctr;
-  LocalStats* lstats;
+  LocalStats* lstats;    // [0]: main stream, [1]: side stream
+  unsigned long long* pass_active;  // kMaxPasses entries
   uint64_t* tile_mm;     // per tile: min, max key
   uint32_t* tile_alpha;  // per tile: 256-bit byte-presence set
   uint32_t* seg_alpha;   // per BigSeg: 256-bit byte-presence set
@@ -379,23 +391,34 @@ __device__ __forceinline__ uint32_t block_max(uint32_t v, uint32_t* scratch) {
 
 // ---------------------------------------------------------------------------
 // Launch wrappers (kernels.hip).  All return hipError_t of the launch.
+//
+// A global pass needs no host hand-over: launch_plan reads the pass's
+// segment count from the device and writes the tile table, and the other
+// kernels are launched with `grid` blocks, an upper bound of their work
+// (tiles or segments); blocks beyond the device count exit at once.
 
-hipError_t launch_gather_minmax(const Plan& p, const BigSeg* segs, uint32_t ntiles,
+// grid_tiles / grid_segs: the blocks the pass's tile and segment kernels are
+// launched with; a pass that needs more is flagged as an error.
+hipError_t launch_plan(const Plan& p, const BigSeg* segs, uint32_t pass, uint32_t grid_tiles,
+                       uint32_t grid_segs, hipStream_t s);
+hipError_t launch_gather_minmax(const Plan& p, const BigSeg* segs, uint32_t grid,
                                 hipStream_t s);
-hipError_t launch_segreduce(const Plan& p, uint32_t nsegs, hipStream_t s);
-hipError_t launch_layout(const Plan& p, uint32_t nsegs, uint32_t radix_bits, hipStream_t s);
-hipError_t launch_histogram(const Plan& p, uint32_t ntiles, uint32_t radix_bits,
-                            hipStream_t s);
+hipError_t launch_segreduce(const Plan& p, uint32_t grid, hipStream_t s);
+hipError_t launch_layout(const Plan& p, uint32_t radix_bits, hipStream_t s);
+hipError_t launch_histogram(const Plan& p, uint32_t grid, uint32_t radix_bits, hipStream_t s);
 // Scans flat[0, ctr->flat_len) in place; max_len bounds the launch size.
 hipError_t launch_flat_scan(const Plan& p, uint64_t max_len, hipStream_t s);
-hipError_t launch_segscan(const Plan& p, const BigSeg* segs, BigSeg* next,
-                          uint32_t nsegs, uint32_t radix_bits, hipStream_t s);
-hipError_t launch_scatter(const Plan& p, const BigSeg* segs, uint32_t ntiles,
+hipError_t launch_segscan(const Plan& p, const BigSeg* segs, BigSeg* next, uint32_t grid,
                           uint32_t radix_bits, hipStream_t s);
-hipError_t launch_boundary_lcp(const Plan& p, uint32_t first, hipStream_t s);
-hipError_t launch_retire(const Plan& p, uint32_t nretire, hipStream_t s);
-// Finishes local segments [first, first + nsegs) of class cls; count_levels
-// also counts the 8-byte key levels into the statistics (slower: profiling).
+hipError_t launch_scatter(const Plan& p, const BigSeg* segs, uint32_t grid,
+                          uint32_t radix_bits, hipStream_t s);
+// LCPs of the boundaries the current pass emitted.
+hipError_t launch_boundary_lcp(const Plan& p, hipStream_t s);
+// Every retire chunk of the counters (grid: workgroups, each looping).
+hipError_t launch_retire(const Plan& p, uint32_t grid, hipStream_t s);
+// Finishes local segments [first, first + nsegs) of class cls;
+// count_levels also counts the 8-byte key levels into the statistics
+// (slower: profiling).
 hipError_t launch_local(const Plan& p, int cls, uint32_t first, uint32_t nsegs, hipStream_t s,
                         bool count_levels);
 // Host-to-device uploads from fine-grained pinned host memory, launched

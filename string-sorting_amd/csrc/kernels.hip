@@ -29,10 +29,14 @@
 //   boundary_lcp   LCP of each new adjacent pair of child buckets.
 //
 // Blocks of the tile kernels take tiles in XCD-aware order (xcd_tile).
-// Between passes the host and the device hand over through two small
-// kernels instead of copies: upload reads the host-built tile table from
-// fine-grained pinned memory, publish writes the counters and the next
-// segment list back and then a sequence word that the host polls.
+// A later pass needs nothing from the host: a one-workgroup set-up kernel
+// (plan) takes the segment list as the previous pass left it, numbers the
+// tiles and writes the tile table (the host builds the first pass's); the
+// pass's other kernels are launched with an upper bound of blocks and read
+// the real tile or segment count from the device counters.  The host reads
+// the counters when it needs them (publish writes them to pinned memory and
+// then a sequence word that the host polls): after the queued passes, to
+// size the local phase, and at the end.
 //
 // A segment whose keys are all equal costs the gather only: it keeps its
 // place and buffer and moves 8 bytes deeper (or retires if the key holds the
@@ -122,6 +126,77 @@ __global__ __launch_bounds__(256) void k_copy_chunks(const CopyChunk* __restrict
   const CopyChunk c = chunks[blockIdx.x];
   for (uint32_t i = threadIdx.x; i < c.words; i += 256) c.dst[i] = c.src[i];
 }
+
+// ---------------------------------------------------------------------------
+// Global pass, step 0 (one workgroup): the pass's segments are the list the
+// previous pass (or the start of the sort) left, big_next of them.  Segment
+// i gets ceil(size / kTile) tiles, numbered in segment order (tile0 = the
+// exclusive scan of the counts); tile t finds its segment by a binary search
+// over tile0.  The counts go to big_cur / ntiles for the pass's kernels, the
+// boundaries emitted so far are marked done (this pass's boundary kernel
+// takes the ones it adds), and the pass's active strings go to the
+// statistics.
+constexpr int kPlanThreads = 1024;
+constexpr uint32_t kPlanLdsSegs = 8192;  // segments whose first tiles the search finds in LDS
+
+__global__ __launch_bounds__(kPlanThreads) void k_plan(Plan p, const BigSeg* segs, uint32_t pass,
+                                                      uint32_t grid_tiles, uint32_t grid_segs) {
+  __shared__ uint32_t s_scan[kPlanThreads / kWave + 1];
+  __shared__ unsigned long long s_act[kPlanThreads / kWave];
+  __shared__ uint32_t s_tile0[kPlanLdsSegs];
+  const uint32_t nsegs = min(p.ctr->big_next, p.cap_big);
+  const bool in_lds = nsegs <= kPlanLdsSegs;
+  uint32_t carry = 0;
+  unsigned long long act = 0;
+  for (uint32_t s0 = 0; s0 < nsegs; s0 += kPlanThreads) {
+    const uint32_t i = s0 + threadIdx.x;
+    uint32_t nt = 0;
+    if (i < nsegs) {
+      const uint32_t sz = segs[i].size;
+      nt = (sz + kTile - 1) / kTile;
+      act += sz;
+    }
+    uint32_t total;
+    const uint32_t pre = block_excl_scan<kPlanThreads>(nt, s_scan, &total);
+    if (i < nsegs) {
+      p.aux[i] = SegAux{0, nt, 0, carry + pre};
+      if (in_lds) s_tile0[i] = carry + pre;
+    }
+    carry += total;
+  }
+  const uint32_t ntiles = min(carry, p.cap_tiles);
+  __syncthreads();  // the first tiles are visible to the whole workgroup
+  for (uint32_t t = threadIdx.x; t < ntiles; t += kPlanThreads) {
+    uint32_t lo = 0, hi = nsegs - 1;  // the last segment whose first tile is <= t
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if ((in_lds ? s_tile0[mid] : p.aux[mid].tile0) <= t) lo = mid; else hi = mid - 1;
+    }
+    const BigSeg sg = segs[lo];
+    const uint32_t k = t - (in_lds ? s_tile0[lo] : p.aux[lo].tile0);
+    p.tiles[t] = Tile{lo, sg.begin + k * kTile, min(kTile, sg.size - k * kTile), k};
+  }
+  // Active strings of the pass: a block sum of the 64-bit partial sums.
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) act += __shfl_xor(act, o, kWave);
+  if (lane_id() == 0) s_act[threadIdx.x / kWave] = act;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tot = 0;
+    for (int w = 0; w < kPlanThreads / kWave; ++w) tot += s_act[w];
+    // The pass's kernels are launched with grid_tiles / grid_segs blocks:
+    // counts beyond them would leave work undone.
+    if (carry > p.cap_tiles || carry > grid_tiles || nsegs > grid_segs)
+      atomicOr(&p.ctr->error, 128u);
+    p.ctr->big_cur = nsegs;
+    p.ctr->ntiles = ntiles;
+    p.ctr->active_cur = (uint32_t)tot;
+    p.ctr->bound_done = p.ctr->boundary;
+    if (pass < (uint32_t)kMaxPasses) p.pass_active[pass] = tot;
+  }
+}
+
+
 
 // ---------------------------------------------------------------------------
 // Digits of a global pass.
@@ -215,11 +290,12 @@ __device__ __forceinline__ uint64_t match_digit_n(uint32_t d, uint64_t live, uin
 // byte values seen (no atomics: thousands of tiles would serialise on one
 // segment's words).
 
-__global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const BigSeg* segs) {
+__device__ __forceinline__ void gather_tile(Plan& p, const BigSeg* segs, uint32_t b,
+                                            uint32_t nt) {
   __shared__ uint64_t s_mn[kTileThreads / kWave], s_mx[kTileThreads / kWave];
   __shared__ uint8_t s_present[256];
   if (threadIdx.x < 256) s_present[threadIdx.x] = 0;
-  const uint32_t ti = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t ti = xcd_tile(b, nt);
   const Tile t = p.tiles[ti];
   const BigSeg sg = segs[t.seg];
   const uint32_t* __restrict__ src = p.idx[sg.buf];
@@ -297,16 +373,25 @@ __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const Bi
   }
 }
 
+// One tile per block; a launch may hold more blocks than the pass has tiles
+// (an upper bound), the rest exit at once.  (A loop over tiles would let one
+// launch take any count, but the compiler then keeps the per-thread indices
+// of every item live across it: far more registers, lower occupancy.)
+__global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const BigSeg* segs) {
+  const uint32_t nt = p.ctr->ntiles;
+  if (blockIdx.x < nt) gather_tile(p, segs, blockIdx.x, nt);
+}
+
 // One workgroup per segment: min/max key and byte set over its tiles.  A
 // first pass's single segment has thousands of tiles: 1024 threads with
 // unrolled loads keep its latency down.
 constexpr int kSegReduceThreads = 1024;
 
-__global__ __launch_bounds__(kSegReduceThreads) void k_segreduce(Plan p) {
+__device__ __forceinline__ void segreduce_seg(Plan& p, uint32_t si) {
   constexpr int T = kSegReduceThreads;
   __shared__ uint64_t s_mn[T / kWave], s_mx[T / kWave];
   __shared__ uint32_t s_al[8];
-  const SegAux ax = p.aux[blockIdx.x];
+  const SegAux ax = p.aux[si];
   if (threadIdx.x < 8) s_al[threadIdx.x] = 0;
   uint64_t mn = ~0ull, mx = 0ull;
   uint32_t al = 0;
@@ -332,24 +417,29 @@ __global__ __launch_bounds__(kSegReduceThreads) void k_segreduce(Plan p) {
   static_assert(T % 8 == 0, "word index of i is i & 7");
   if (al) atomicOr(&s_al[threadIdx.x & 7], al);
   __syncthreads();
-  if (threadIdx.x < 8) p.seg_alpha[8ull * blockIdx.x + threadIdx.x] = s_al[threadIdx.x];
+  if (threadIdx.x < 8) p.seg_alpha[8ull * si + threadIdx.x] = s_al[threadIdx.x];
   if (threadIdx.x == 0) {
     for (int w = 1; w < T / kWave; ++w) {
       mn = s_mn[w] < mn ? s_mn[w] : mn;
       mx = s_mx[w] > mx ? s_mx[w] : mx;
     }
-    p.segmm[2ull * blockIdx.x] = mn;
-    p.segmm[2ull * blockIdx.x + 1] = mx;
+    p.segmm[2ull * si] = mn;
+    p.segmm[2ull * si + 1] = mx;
   }
+}
+
+__global__ __launch_bounds__(kSegReduceThreads) void k_segreduce(Plan p) {
+  if (blockIdx.x < p.ctr->big_cur) segreduce_seg(p, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
 // Global pass, step 2: digit ranges and the flat layout of the counts.
 
 template <int RB>
-__global__ __launch_bounds__(1024) void k_layout(Plan p, uint32_t nsegs) {
+__global__ __launch_bounds__(1024) void k_layout(Plan p) {
   __shared__ uint32_t s_scan[1024 / kWave + 1];
   __shared__ uint32_t s_al[8];
+  const uint32_t nsegs = p.ctr->big_cur;
   // The pass alphabet: union of the segments' byte sets.
   if (threadIdx.x < 8) s_al[threadIdx.x] = 0;
   __syncthreads();
@@ -379,6 +469,7 @@ __global__ __launch_bounds__(1024) void k_layout(Plan p, uint32_t nsegs) {
   if (threadIdx.x == 0) {
     p.ctr->flat_len = carry;
     p.ctr->big_next = 0;  // the coming segscan counts this pass's big children
+    p.ctr->big_next_strings = 0;
   }
 }
 
@@ -386,11 +477,11 @@ __global__ __launch_bounds__(1024) void k_layout(Plan p, uint32_t nsegs) {
 // Global pass, step 3: per-tile digit counts, digit-major per segment.
 
 template <int RB>
-__global__ __launch_bounds__(kTileThreads) void k_histogram(Plan p) {
+__device__ __forceinline__ void histogram_tile(Plan& p, uint32_t b, uint32_t nt) {
   constexpr uint32_t NB = 1u << RB;
   __shared__ uint32_t h[NB];
   __shared__ uint8_t rank[256];
-  const Tile t = p.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const Tile t = p.tiles[xcd_tile(b, nt)];
   // Everything that needs only the tile is loaded before the segment's
   // min/max can end the block (a load after that return cannot be hoisted
   // above it): the keys (slots past the tile read its last key and are not
@@ -420,6 +511,12 @@ __global__ __launch_bounds__(kTileThreads) void k_histogram(Plan p) {
   for (uint32_t d = threadIdx.x; d < ds.nbu; d += kTileThreads) col[(uint64_t)d * ax.ntiles] = h[d];
 }
 
+template <int RB>
+__global__ __launch_bounds__(kTileThreads) void k_histogram(Plan p) {
+  const uint32_t nt = p.ctr->ntiles;
+  if (blockIdx.x < nt) histogram_tile<RB>(p, blockIdx.x, nt);
+}
+
 // ---------------------------------------------------------------------------
 // Global pass, step 4: exclusive scan of the flat count array in place
 // (block sums, scan of the block sums, block-local scans plus offsets).
@@ -431,7 +528,8 @@ constexpr int kScanItems = kScanBlock / kScanThreads;
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(Plan p) {
   __shared__ uint32_t s_scan[kScanThreads / kWave + 1];
   const uint32_t len = p.ctr->flat_len;
-  const uint64_t b0 = (uint64_t)blockIdx.x * kScanBlock;
+  const uint32_t blk = blockIdx.x;
+  const uint64_t b0 = (uint64_t)blk * kScanBlock;
   if (b0 >= len) return;
   uint32_t sum = 0;
 #pragma unroll
@@ -441,7 +539,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(Plan p) {
   }
   uint32_t total;
   (void)block_excl_scan<kScanThreads>(sum, s_scan, &total);
-  if (threadIdx.x == 0) p.flat_blk[blockIdx.x] = total;
+  if (threadIdx.x == 0) p.flat_blk[blk] = total;
 }
 
 __global__ __launch_bounds__(1024) void k_scan_blocks(Plan p) {
@@ -466,7 +564,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(Plan p) {
   __shared__ uint32_t s_scan[kScanThreads / kWave + 1];
   __shared__ __attribute__((aligned(16))) uint32_t s_v[kScanBlock];
   const uint32_t len = p.ctr->flat_len;
-  const uint64_t b0 = (uint64_t)blockIdx.x * kScanBlock;
+  const uint32_t blk = blockIdx.x;
+  const uint64_t b0 = (uint64_t)blk * kScanBlock;
   if (b0 >= len) return;
   // The block's entries go through LDS so that global loads and stores are
   // 16-byte and coalesced while each thread scans consecutive entries.
@@ -490,7 +589,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(Plan p) {
     sum += q[r].x + q[r].y + q[r].z + q[r].w;
   }
   uint32_t total;
-  uint32_t run = block_excl_scan<kScanThreads>(sum, s_scan, &total) + p.flat_blk[blockIdx.x];
+  uint32_t run = block_excl_scan<kScanThreads>(sum, s_scan, &total) + p.flat_blk[blk];
 #pragma unroll
   for (int r = 0; r < Q; ++r) {
     uint4 o;
@@ -534,6 +633,7 @@ __device__ __forceinline__ void push_child(Plan& p, BigSeg* next, uint32_t begin
     return;
   }
   const uint32_t slot = atomicAdd(&p.ctr->big_next, 1u);
+  atomicAdd(&p.ctr->big_next_strings, size);
   if (slot >= p.cap_big) {
     atomicOr(&p.ctr->error, 4u);
     return;
@@ -557,11 +657,11 @@ __device__ __forceinline__ void push_retire(Plan& p, uint32_t begin, uint32_t si
 }
 
 template <int RB>
-__global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, BigSeg* next) {
+__device__ __forceinline__ void segscan_seg(Plan& p, const BigSeg* segs, BigSeg* next,
+                                            uint32_t si) {
   constexpr uint32_t NB = 1u << RB;
   constexpr uint32_t DPT = NB / 256;
   __shared__ uint32_t s_first;
-  const uint32_t si = blockIdx.x;
   // The segment's count-matrix place and the alphabet are loaded with its
   // bounds, before its min/max can end the block.
   const SegAux ax = p.aux[si];
@@ -617,7 +717,7 @@ __global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, Big
   // Classify every bucket.  kind: 0 final or empty, 1..3 local class 0..2,
   // 4 next pass, 5 run of identical strings (nul = where they end).
   uint32_t kind[DPT], nulp[DPT];
-  uint32_t nb = 0, nr = 0, ng = 0, nl[3] = {0, 0, 0}, sl[3] = {0, 0, 0};
+  uint32_t nb = 0, nr = 0, ng = 0, sg_next = 0, nl[3] = {0, 0, 0}, sl[3] = {0, 0, 0};
 #pragma unroll
   for (uint32_t q = 0; q < DPT; ++q) {
     const uint32_t d = threadIdx.x * DPT + q;
@@ -657,9 +757,12 @@ __global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, Big
       } else {
         kind[q] = 4;
         ++ng;
+        sg_next += c;
       }
     }
   }
+
+  if (sg_next) atomicAdd(&p.ctr->big_next_strings, sg_next);
 
   // One reservation per list for the whole segment: same-address atomics
   // from every bucket would serialise.
@@ -727,13 +830,19 @@ __global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, Big
   }
 }
 
+template <int RB>
+__global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, BigSeg* next) {
+  if (blockIdx.x < p.ctr->big_cur) segscan_seg<RB>(p, segs, next, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Global pass, step 6: stable scatter.  The in-tile rank of each string
 // among the tile's strings of its digit comes from wave match-ranking; the
 // scanned flat array gives where the tile's strings of each digit start.
 
 template <int RB>
-__global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* segs) {
+__device__ __forceinline__ void scatter_tile(Plan& p, const BigSeg* segs, uint32_t b,
+                                             uint32_t nt) {
   constexpr uint32_t NB = 1u << RB;
   constexpr int W = kTileThreads / kWave;
   constexpr int IPW = kTile / W;  // items per wave
@@ -747,7 +856,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
   __shared__ uint32_t s_scan[W + 1];
   __shared__ uint8_t rank[256];
 
-  const Tile t = p.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const Tile t = p.tiles[xcd_tile(b, nt)];
   const bool carry = !p.keys_carried;  // hand the next pass its keys
   const int wid = threadIdx.x / kWave;
   const int lane = lane_id();
@@ -899,10 +1008,17 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* 
   }
 }
 
+template <int RB>
+__global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* segs) {
+  const uint32_t nt = p.ctr->ntiles;
+  if (blockIdx.x < nt) scatter_tile<RB>(p, segs, blockIdx.x, nt);
+}
+
 // ---------------------------------------------------------------------------
 // Global pass, step 7: LCP of pairs split by this pass.
 
-__global__ void k_boundary_lcp(Plan p, uint32_t first) {
+__global__ void k_boundary_lcp(Plan p) {
+  const uint32_t first = p.ctr->bound_done;
   const uint32_t n = min(p.ctr->boundary, p.cap_bounds);
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -917,7 +1033,8 @@ __global__ void k_boundary_lcp(Plan p, uint32_t first) {
 // ---------------------------------------------------------------------------
 // Runs of identical strings: copy offsets (when needed) and fill their LCPs.
 
-__global__ void k_retire(Plan p, uint32_t count) {
+__global__ void k_retire(Plan p) {
+  const uint32_t count = min(p.ctr->retire, p.cap_retire);
   for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
     const Retire r = p.retire[e];
     const uint32_t* ix = p.idx[r.buf];
@@ -938,32 +1055,41 @@ static inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 4096)
   return (uint32_t)g;
 }
 
-hipError_t launch_segreduce(const Plan& p, uint32_t nsegs, hipStream_t s) {
-  k_segreduce<<<nsegs, kSegReduceThreads, 0, s>>>(p);
+hipError_t launch_plan(const Plan& p, const BigSeg* segs, uint32_t pass, uint32_t grid_tiles,
+                       uint32_t grid_segs, hipStream_t s) {
+  k_plan<<<1, kPlanThreads, 0, s>>>(p, segs, pass, grid_tiles, grid_segs);
   return hipGetLastError();
 }
 
-hipError_t launch_gather_minmax(const Plan& p, const BigSeg* segs, uint32_t ntiles,
+hipError_t launch_segreduce(const Plan& p, uint32_t grid, hipStream_t s) {
+  if (!grid) return hipSuccess;
+  k_segreduce<<<grid, kSegReduceThreads, 0, s>>>(p);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_minmax(const Plan& p, const BigSeg* segs, uint32_t grid,
                                 hipStream_t s) {
-  k_gather_minmax<<<ntiles, kTileThreads, 0, s>>>(p, segs);
+  if (!grid) return hipSuccess;
+  k_gather_minmax<<<grid, kTileThreads, 0, s>>>(p, segs);
   return hipGetLastError();
 }
 
-hipError_t launch_layout(const Plan& p, uint32_t nsegs, uint32_t radix_bits, hipStream_t s) {
+hipError_t launch_layout(const Plan& p, uint32_t radix_bits, hipStream_t s) {
   if (radix_bits == 8)
-    k_layout<8><<<1, 1024, 0, s>>>(p, nsegs);
+    k_layout<8><<<1, 1024, 0, s>>>(p);
   else if (radix_bits == 11)
-    k_layout<11><<<1, 1024, 0, s>>>(p, nsegs);
+    k_layout<11><<<1, 1024, 0, s>>>(p);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-hipError_t launch_histogram(const Plan& p, uint32_t ntiles, uint32_t radix_bits, hipStream_t s) {
+hipError_t launch_histogram(const Plan& p, uint32_t grid, uint32_t radix_bits, hipStream_t s) {
+  if (!grid) return hipSuccess;
   if (radix_bits == 8)
-    k_histogram<8><<<ntiles, kTileThreads, 0, s>>>(p);
+    k_histogram<8><<<grid, kTileThreads, 0, s>>>(p);
   else if (radix_bits == 11)
-    k_histogram<11><<<ntiles, kTileThreads, 0, s>>>(p);
+    k_histogram<11><<<grid, kTileThreads, 0, s>>>(p);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -977,36 +1103,38 @@ hipError_t launch_flat_scan(const Plan& p, uint64_t max_len, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_segscan(const Plan& p, const BigSeg* segs, BigSeg* next, uint32_t nsegs,
+hipError_t launch_segscan(const Plan& p, const BigSeg* segs, BigSeg* next, uint32_t grid,
                           uint32_t radix_bits, hipStream_t s) {
+  if (!grid) return hipSuccess;
   if (radix_bits == 8)
-    k_segscan<8><<<nsegs, 256, 0, s>>>(p, segs, next);
+    k_segscan<8><<<grid, 256, 0, s>>>(p, segs, next);
   else if (radix_bits == 11)
-    k_segscan<11><<<nsegs, 256, 0, s>>>(p, segs, next);
+    k_segscan<11><<<grid, 256, 0, s>>>(p, segs, next);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-hipError_t launch_scatter(const Plan& p, const BigSeg* segs, uint32_t ntiles,
+hipError_t launch_scatter(const Plan& p, const BigSeg* segs, uint32_t grid,
                           uint32_t radix_bits, hipStream_t s) {
+  if (!grid) return hipSuccess;
   if (radix_bits == 8)
-    k_scatter<8><<<ntiles, kTileThreads, 0, s>>>(p, segs);
+    k_scatter<8><<<grid, kTileThreads, 0, s>>>(p, segs);
   else if (radix_bits == 11)
-    k_scatter<11><<<ntiles, kTileThreads, 0, s>>>(p, segs);
+    k_scatter<11><<<grid, kTileThreads, 0, s>>>(p, segs);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-hipError_t launch_boundary_lcp(const Plan& p, uint32_t first, hipStream_t s) {
-  k_boundary_lcp<<<1024, 256, 0, s>>>(p, first);
+hipError_t launch_boundary_lcp(const Plan& p, hipStream_t s) {
+  k_boundary_lcp<<<1024, 256, 0, s>>>(p);
   return hipGetLastError();
 }
 
-hipError_t launch_retire(const Plan& p, uint32_t nretire, hipStream_t s) {
-  if (!nretire) return hipSuccess;
-  k_retire<<<grid_for(nretire, 1, 8192), 256, 0, s>>>(p, nretire);
+hipError_t launch_retire(const Plan& p, uint32_t grid, hipStream_t s) {
+  if (!grid) return hipSuccess;
+  k_retire<<<grid, 256, 0, s>>>(p);
   return hipGetLastError();
 }
 

@@ -313,8 +313,9 @@ constexpr int local_min_waves(uint32_t cap) {
   return cap <= kLocalClassCap[0] ? 7 : (cap <= kLocalClassCap[1] ? 6 : 4);
 }
 
+// Block b finishes segment first + b of the class's list.
 template <int BLOCK, int CAP, bool COUNT>
-__global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, const LocalSeg* segs) {
+__global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, uint32_t first) {
   constexpr int W = BLOCK / kWave;
   constexpr int IPT = CAP / BLOCK;
   constexpr int KW = kLocalWords;
@@ -322,15 +323,20 @@ __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, c
   constexpr uint64_t kSkipKey = ~0ull;  // key of a group that only moves deeper
   static_assert(CAP % BLOCK == 0 && (CAP / W) % kWave == 0, "layout");
   __shared__ LocalSmem<BLOCK, CAP> s;
-  const LocalSeg sg = segs[blockIdx.x];
-  const uint32_t n = sg.size;
-  const uint32_t gb = sg.begin;
-  const uint32_t* __restrict__ src = p.idx[sg.buf];
   uint64_t* __restrict__ lcp = p.lcp;
   uint32_t* __restrict__ out = p.out;
   const uint32_t wid = threadIdx.x / kWave;
   const uint32_t lane = lane_id();
   const uint64_t lt = lanemask_lt();
+  unsigned long long round_strings = 0;
+  // 8-byte key levels this wave's strings advanced through: a round's group
+  // that stops at key e consumed e + 1 of them, a skipping group KW.  Built
+  // from ballots, so the count is wave-uniform (scalar registers).
+  unsigned long long wave_levels = 0;
+  const LocalSeg sg = p.local[kClass][first + blockIdx.x];
+  const uint32_t n = sg.size;
+  const uint32_t gb = sg.begin;
+  const uint32_t* __restrict__ src = p.idx[sg.buf];
 
   for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
     s.off[i] = src[gb + i];
@@ -344,11 +350,6 @@ __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, c
     s.ss_stop[0] = 0;
   }
   uint32_t nact = n, nss = 1, rounds = 0;
-  unsigned long long round_strings = 0;
-  // 8-byte key levels this wave's strings advanced through: a round's group
-  // that stops at key e consumed e + 1 of them, a skipping group KW.  Built
-  // from ballots, so the count is wave-uniform (scalar registers).
-  unsigned long long wave_levels = 0;
   __syncthreads();
 
   while (true) {
@@ -749,13 +750,13 @@ template <bool COUNT>
 hipError_t launch_local_t(const Plan& p, int cls, uint32_t first, uint32_t nsegs, hipStream_t s) {
   switch (cls) {
     case 0:
-      k_local<64, 128, COUNT><<<nsegs, 64, 0, s>>>(p, p.local[0] + first);
+      k_local<64, 128, COUNT><<<nsegs, 64, 0, s>>>(p, first);
       break;
     case 1:
-      k_local<256, 1024, COUNT><<<nsegs, 256, 0, s>>>(p, p.local[1] + first);
+      k_local<256, 1024, COUNT><<<nsegs, 256, 0, s>>>(p, first);
       break;
     case 2:
-      k_local<1024, 6144, COUNT><<<nsegs, 1024, 0, s>>>(p, p.local[2] + first);
+      k_local<1024, 6144, COUNT><<<nsegs, 1024, 0, s>>>(p, first);
       break;
     default:
       return hipErrorInvalidValue;

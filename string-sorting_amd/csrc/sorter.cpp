@@ -33,26 +33,23 @@ struct Workspace {
   uint32_t* idx_copy = nullptr;
   // Pinned host mirrors.
   Counters* h_ctr = nullptr;
-  BigSeg* h_big = nullptr;
-  Tile* h_tiles = nullptr;
-  SegAux* h_aux = nullptr;
   LocalSeg* h_local = nullptr;  // initial local segments, 3 classes of cap_init
   uint32_t cap_init = 0;
-  // Fine-grained (coherent) pinned blocks, one per direction of a pass.  Up:
-  // the segment aux table and the tile table, read by one upload launch
-  // (they are adjacent on the device too); h_init and h_local go up the same
-  // way.  Down: the hand-over block, a sequence word (bytes [0, 8)) and from
-  // kPubHead on the counters and the next segment list after a pass, or the
-  // local statistics and the counters at the end, written by the publish
-  // kernel; h_lstats points into it.
-  char* h_up = nullptr;
-  size_t up_tiles_off = 0;  // byte offset of the tiles behind the aux table
+  // Fine-grained (coherent) pinned blocks, one per direction.  Up (h_init,
+  // h_local): the start of a sort, read by one upload launch.  Down (h_pub):
+  // a sequence word (bytes [0, 8)) and from kPubHead on the statistics (local
+  // statistics, per-pass active strings) and the counters, written by the
+  // publish kernel after a local phase; h_lstats and h_pass point into it.
   char* h_pub = nullptr;
   uint64_t pub_seq = 0;
+  // The first pass's segment aux table and tile table, built by the host
+  // (adjacent on the device, uploaded with the start of the sort).
+  char* h_up = nullptr;
+  size_t up_tiles_off = 0;
   const LocalStats* h_lstats = nullptr;
+  const unsigned long long* h_pass = nullptr;
   // The start of a sort: the counters and the initial segment list go up in
-  // one upload that also clears the local statistics (adjacent on the
-  // device).
+  // one upload that also clears the statistics (adjacent on the device).
   char* h_init = nullptr;
   size_t init_ctr_off = 0, init_big_off = 0;
 };
@@ -90,11 +87,16 @@ int local_class(uint64_t size) {
 struct Context {
   int device = -1;
   hipStream_t stream = nullptr;
-  // Local segments that a global pass finishes early are sorted on this
-  // stream while the next passes run on `stream`.
+  // The requests of a batch that start as local segments are sorted on
+  // this stream while the large ones take their global passes on `stream`.
   hipStream_t side = nullptr;
   hipEvent_t side_done = nullptr;
   Workspace ws;
+  // Global passes the last sort on this context took, and the strings its
+  // first pass held: a sort of a similar size queues as many passes before
+  // it first looks at the counters.
+  uint32_t pass_hint = 1;
+  uint64_t hint_strings = 0;
   std::mutex mu;
   gss_stats stats{};
   std::vector<Pending> pending;
@@ -222,18 +224,15 @@ void prof_collect(Context& c) {
 
 void free_workspace(Workspace& w) {
   if (w.base) (void)hipFree(w.base);
-  if (w.h_big) (void)hipHostFree(w.h_big);
-  if (w.h_up) (void)hipHostFree(w.h_up);
   if (w.h_pub) (void)hipHostFree(w.h_pub);
   if (w.h_init) (void)hipHostFree(w.h_init);
+  if (w.h_up) (void)hipHostFree(w.h_up);
+  w.h_up = nullptr;
   w.h_init = nullptr;
   w.base = nullptr;
-  w.h_big = nullptr;
-  w.h_up = nullptr;
   w.h_pub = nullptr;
-  w.h_tiles = nullptr;
-  w.h_aux = nullptr;
   w.h_lstats = nullptr;
+  w.h_pass = nullptr;
   w.bytes = 0;
   w.n_cap = 0;
 }
@@ -272,16 +271,16 @@ std::vector<Item> workspace_items(Plan& p, uint32_t** idx_copy, const Caps& k, u
       {(void**)&p.keys, 8 * k.ncap},
       {(void**)&p.keys_next, 8 * k.ncap},
       {(void**)&p.ext, 8 * k.ncap},
-      // lstats, ctr, big[0]: uploaded together at the start of a sort;
-      // ctr, big[0], big[1]: read back together after each pass.
-      {(void**)&p.lstats, 2 * sizeof(LocalStats)},  // [0]: main stream, [1]: side stream
+      // lstats (+ the per-pass active strings), ctr, big[0]: uploaded
+      // together at the start of a sort; lstats and ctr read back together.
+      {(void**)&p.lstats, 2 * sizeof(LocalStats) + 8 * kMaxPasses},
       {(void**)&p.ctr, sizeof(Counters)},
       {(void**)&p.big[0], sizeof(BigSeg) * k.big},
       {(void**)&p.big[1], sizeof(BigSeg) * k.big},
       {(void**)&p.segmm, 16 * k.big},
       {(void**)&p.bstart, 4 * k.big * nb},
       {(void**)&p.alpha, 32},
-      {(void**)&p.aux, sizeof(SegAux) * k.big},  // aux, tiles: uploaded together
+      {(void**)&p.aux, sizeof(SegAux) * k.big},
       {(void**)&p.tiles, sizeof(Tile) * k.tiles},
       {(void**)&p.flat, 4 * (k.tiles * nb + 1)},
       {(void**)&p.flat_blk, 4 * ((k.tiles * nb) / kScanBlock + 2)},
@@ -328,6 +327,7 @@ int ensure_workspace(Context& c, uint64_t n, uint32_t rb, uint32_t t) {
     *it.dst = base + off;
     off += align_up(it.bytes, 256);
   }
+  p.pass_active = reinterpret_cast<unsigned long long*>(p.lstats + 2);
   w.base = base;
   w.bytes = total;
   w.n_cap = ncap;
@@ -338,18 +338,15 @@ int ensure_workspace(Context& c, uint64_t n, uint32_t rb, uint32_t t) {
   p.cap_retire = (uint32_t)cap_retire;
   p.cap_bounds = (uint32_t)cap_bounds;
   for (int i = 0; i < 3; ++i) p.cap_local[i] = (uint32_t)cap_local[i];
-  GSS_HIP(hipHostMalloc((void**)&w.h_big, sizeof(BigSeg) * cap_big, hipHostMallocDefault));
   w.up_tiles_off = (size_t)(reinterpret_cast<char*>(p.tiles) - reinterpret_cast<char*>(p.aux));
   GSS_HIP(hipHostMalloc((void**)&w.h_up, w.up_tiles_off + sizeof(Tile) * cap_tiles,
                         hipHostMallocCoherent));
-  w.h_aux = reinterpret_cast<SegAux*>(w.h_up);
-  w.h_tiles = reinterpret_cast<Tile*>(w.h_up + w.up_tiles_off);
   w.init_ctr_off = (size_t)(reinterpret_cast<char*>(p.ctr) - reinterpret_cast<char*>(p.lstats));
-  const size_t pub_bytes = kPubHead + std::max(sizeof(Counters) + sizeof(BigSeg) * cap_big,
-                                               w.init_ctr_off + sizeof(Counters));
+  const size_t pub_bytes = kPubHead + w.init_ctr_off + sizeof(Counters);
   GSS_HIP(hipHostMalloc((void**)&w.h_pub, pub_bytes, hipHostMallocCoherent));
   memset(w.h_pub, 0, pub_bytes);
   w.h_lstats = reinterpret_cast<const LocalStats*>(w.h_pub + kPubHead);
+  w.h_pass = reinterpret_cast<const unsigned long long*>(w.h_lstats + 2);
   w.init_big_off = (size_t)(reinterpret_cast<char*>(p.big[0]) - reinterpret_cast<char*>(p.lstats));
   const size_t init_bytes = w.init_big_off + sizeof(BigSeg) * cap_big;
   GSS_HIP(hipHostMalloc((void**)&w.h_init, init_bytes, hipHostMallocCoherent));
@@ -438,7 +435,12 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   memset(&h0, 0, sizeof(h0));
   LocalSeg* init[3] = {w.h_local, w.h_local + w.cap_init, w.h_local + 2 * w.cap_init};
   Retire* init_retire = reinterpret_cast<Retire*>(w.h_local + 3 * w.cap_init);
+  BigSeg* init_big = reinterpret_cast<BigSeg*>(w.h_init + w.init_big_off);
   uint32_t nbig = 0;
+  uint32_t tiles0 = 0;  // tiles of the first pass
+  uint64_t pass0_strings = 0;
+  SegAux* aux0 = reinterpret_cast<SegAux*>(w.h_up);
+  Tile* tiles = reinterpret_cast<Tile*>(w.h_up + w.up_tiles_off);
   for (uint32_t r = 0; r < nreq; ++r) {
     const uint32_t b = (uint32_t)req[r], size = (uint32_t)(req[r + 1] - req[r]);
     if (size == 0) continue;
@@ -450,13 +452,29 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
       h0.local_strings[cls] += size;
     } else {
       if (nbig >= p.cap_big) return fail(GSS_E_INTERNAL, "segment table overflow");
-      w.h_big[nbig++] = BigSeg{b, size, 0, 0};
+      const uint32_t first = tiles0;
+      for (uint32_t o = 0; o < size; o += kTile) {
+        if (tiles0 >= p.cap_tiles) return fail(GSS_E_INTERNAL, "tile table overflow");
+        tiles[tiles0] = Tile{nbig, b + o, std::min<uint32_t>(kTile, size - o), tiles0 - first};
+        ++tiles0;
+      }
+      aux0[nbig] = SegAux{0, tiles0 - first, 0, first};
+      init_big[nbig++] = BigSeg{b, size, 0, 0};
+      pass0_strings += size;
     }
   }
-  // The start of the sort goes up in one launch, together with the first
-  // pass's tiles when no early local launch needs the lists before them:
-  // the initial local lists, the retire runs, and (one range) cleared
-  // statistics, the counters and the big segments.
+  // The first pass's counts, as its set-up kernel would leave them (the
+  // host built its tile table).
+  h0.big_cur = nbig;
+  h0.ntiles = tiles0;
+  h0.active_cur = (uint32_t)pass0_strings;
+  // With global passes to run, the initial local segments are sorted on the
+  // side stream meanwhile; the main stream's local phase starts after them.
+  // h0 lives in the pinned counter mirror, which the read-backs overwrite.
+  const Counters start = h0;
+  // The start of the sort goes up in one launch: the initial local lists,
+  // the retire runs, and (one range) cleared statistics, the counters and
+  // the first segment list.
   UploadBatch up;
   auto add = [&](const void* src, void* dst, size_t zero_bytes, size_t bytes) -> int {
     if (add_upload(up, src, dst, zero_bytes, bytes))
@@ -467,158 +485,193 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     if ((rc = add(init[cls], p.local[cls], 0, sizeof(LocalSeg) * h0.local[cls]))) return rc;
   if ((rc = add(init_retire, p.retire, 0, sizeof(Retire) * h0.retire))) return rc;
   memcpy(w.h_init + w.init_ctr_off, &h0, sizeof(Counters));
-  memcpy(w.h_init + w.init_big_off, w.h_big, sizeof(BigSeg) * nbig);
   if ((rc = add(w.h_init, p.lstats, w.init_ctr_off, w.init_big_off + sizeof(BigSeg) * nbig)))
     return rc;
-  if (!nbig) GSS_HIP(launch_upload(up, st));
-  // Local segments launched early run on the side stream with their own
-  // statistics block; the side stream first waits for the uploads above.
-  Plan p_side = p;
-  p_side.lstats = p.lstats + 1;
+  if (nbig && (rc = add(w.h_up, p.aux, 0, w.up_tiles_off + sizeof(Tile) * tiles0))) return rc;
+  GSS_HIP(launch_upload(up, st));
   // Key levels are counted only while kernels are profiled.
   const bool count_levels = g_levels.load(std::memory_order_relaxed);
-  uint32_t launched[3] = {0, 0, 0};
-  uint64_t early_strings = 0;  // strings of the local segments launched early
+  // Local segments sorted on the side stream while global passes run: the
+  // small requests of a batch from the start, and (when the host reads each
+  // pass's counts) those a pass left when they hold at least half as many
+  // strings as the next pass.  Against a heavier pass they would hold its
+  // workgroups off the CUs (one local workgroup fills a CU's LDS).  The side
+  // stream has its own statistics block.
   bool side_used = false;
   long side_tok[3] = {-1, -1, -1};
+  uint32_t launched[3] = {0, 0, 0};  // local segments handed to a launch so far
+  uint64_t early_strings = 0;        // their strings
+  // Sorts local segments [launched, upto) of each class on the side stream,
+  // after the main stream's work so far (which wrote them).
   auto launch_early = [&](const uint32_t* upto) -> int {
-    bool any = false;
-    for (int cls = 0; cls < 3; ++cls) any = any || upto[cls] > launched[cls];
-    if (!any) return GSS_OK;
-    // The side stream waits for the work queued on the main stream so far,
-    // the uploads still held back included.
-    GSS_HIP(launch_upload(up, st));
+    Plan p_side = p;
+    p_side.lstats = p.lstats + 1;
     GSS_HIP(hipEventRecord(c.side_done, st));
     GSS_HIP(hipStreamWaitEvent(c.side, c.side_done, 0));
     side_used = true;
     static const char* kEarlyNames[3] = {"local128_early", "local1024_early", "local6144_early"};
     for (int cls = 2; cls >= 0; --cls) {
       if (upto[cls] <= launched[cls]) continue;
-      if (upto[cls] > p.cap_local[cls]) return fail(GSS_E_INTERNAL, "local segment list overflow");
       const size_t mark = c.pending.size();
       ProfToken tok = prof_begin(c, kEarlyNames[cls], 0, c.side);
-      GSS_HIP(launch_local(p_side, cls, launched[cls], upto[cls] - launched[cls], c.side, count_levels));
+      GSS_HIP(launch_local(p_side, cls, launched[cls], upto[cls] - launched[cls], c.side,
+                           count_levels));
       prof_end(c, tok, c.side);
       if (c.pending.size() > mark) side_tok[cls] = (long)mark;
       launched[cls] = upto[cls];
     }
     return GSS_OK;
   };
+  if (nbig && (start.local[0] || start.local[1] || start.local[2])) {
+    if ((rc = launch_early(start.local))) return rc;
+    for (int cls = 0; cls < 3; ++cls) early_strings += start.local_strings[cls];
+  }
 
-  uint32_t nlocal[3] = {h0.local[0], h0.local[1], h0.local[2]};
-  if (nbig) {  // small requests of a batch sort while the big ones take global passes
-    const int erc = launch_early(nlocal);
-    if (erc) return erc;
-    for (int cls = 0; cls < 3; ++cls) early_strings += h0.local_strings[cls];
-  }
-  uint32_t local_strings[3] = {h0.local_strings[0], h0.local_strings[1], h0.local_strings[2]};
-  uint32_t nretire = h0.retire;
-  if (nbig) {
-    uint32_t cur = 0, bound_first = 0, pass = 0;
-    // Each pass either splits a segment or moves it >= 1 byte deeper, and no
-    // string reaches past the text, so the pass count is bounded.
-    const uint64_t max_passes = text_bytes + 64;
-    while (nbig) {
-      if (pass > max_passes) return fail(GSS_E_INTERNAL, "global passes do not terminate");
-      uint32_t ntiles = 0;
-      uint64_t active = 0;
-      for (uint32_t i = 0; i < nbig; ++i) {
-        const BigSeg& sg = w.h_big[i];
-        active += sg.size;
-        const uint32_t first = ntiles;
-        for (uint32_t o = 0; o < sg.size; o += kTile) {
-          if (ntiles >= p.cap_tiles) return fail(GSS_E_INTERNAL, "tile table overflow");
-          w.h_tiles[ntiles] = Tile{i, sg.begin + o, std::min<uint32_t>(kTile, sg.size - o),
-                                   ntiles - first};
-          ++ntiles;
-        }
-        w.h_aux[i] = SegAux{0, ntiles - first, 0, first};
-      }
-      if (pass < GSS_MAX_PASSES) stats.active[pass] = active;
-      if ((rc = add(w.h_up, p.aux, 0, w.up_tiles_off + sizeof(Tile) * ntiles))) return rc;
-      GSS_HIP(launch_upload(up, st));
-      const BigSeg* segs = p.big[cur];
-      BigSeg* next = p.big[cur ^ 1];
-      static const char* kPassNames[10] = {"pass0", "pass1", "pass2", "pass3", "pass4",
-                                           "pass5", "pass6", "pass7", "pass8", "pass9+"};
-      ProfToken pass_tok = prof_begin(c, kPassNames[pass < 9 ? pass : 9], active, st);
-      // Passes alternate: one reads its keys (and the next 8 bytes) from the
-      // text and hands the following pass its keys, which that pass takes
-      // as they lie.
-      p.keys_carried = pass % 2;
-      if (p.keys_carried) std::swap(p.keys, p.keys_next);
-      GSS_LAUNCH("gather", active, launch_gather_minmax(p, segs, ntiles, st));
-      GSS_LAUNCH("segreduce", nbig, launch_segreduce(p, nbig, st));
-      GSS_LAUNCH("layout", nbig, launch_layout(p, nbig, rb, st));
-      GSS_LAUNCH("histogram", active, launch_histogram(p, ntiles, rb, st));
-      GSS_LAUNCH("flat_scan", (uint64_t)ntiles * nb,
-                 launch_flat_scan(p, (uint64_t)ntiles * nb, st));
-      GSS_LAUNCH("segscan", nbig, launch_segscan(p, segs, next, nbig, rb, st));
-      GSS_LAUNCH("scatter", active, launch_scatter(p, segs, ntiles, rb, st));
-      if (lcp) GSS_LAUNCH("boundary", 0, launch_boundary_lcp(p, bound_first, st));
-      prof_end(c, pass_tok, st);
-      // The counters and the used part of the next segment list come back
-      // through the hand-over block.
-      const uint64_t seq = ++w.pub_seq;
-      GSS_HIP(launch_publish(p.ctr, sizeof(Counters), next, &p.ctr->big_next, p.cap_big,
-                             sizeof(BigSeg), w.h_pub + kPubHead,
-                             reinterpret_cast<uint64_t*>(w.h_pub), seq, st));
-      rc = wait_published(w, st, seq);
-      if (rc) return rc;
-      memcpy(w.h_ctr, w.h_pub + kPubHead, sizeof(Counters));
-      if (w.h_ctr->big_next <= p.cap_big)
-        memcpy(w.h_big, w.h_pub + kPubHead + sizeof(Counters),
-               sizeof(BigSeg) * w.h_ctr->big_next);
-      if (w.h_ctr->error)
-        return fail(GSS_E_INTERNAL, "device pass failed (code " +
-                                        std::to_string(w.h_ctr->error) + ")");
-      nbig = w.h_ctr->big_next;
-      bound_first = w.h_ctr->boundary;
-      cur ^= 1;
-      ++pass;
-      if (nbig) {
-        // More passes follow.  The local segments found so far are sorted on
-        // the side stream meanwhile when they hold at least half as many
-        // strings as the next pass.  Launched against a heavier pass they
-        // hold its workgroups off the CUs (one local workgroup fills a CU's
-        // LDS): on 10M URL strings, launching 2.9M local strings against a
-        // 7.1M-string pass made the sort 19% slower, while 100M Zipf words
-        // gain 3% from the half rule over an equal-size rule.
-        uint64_t next_active = 0, waiting = 0;
-        for (uint32_t i = 0; i < nbig; ++i) next_active += w.h_big[i].size;
-        for (int cls = 0; cls < 3; ++cls) waiting += w.h_ctr->local_strings[cls];
-        waiting -= early_strings;
-        if (2 * waiting >= next_active) {
-          const uint32_t upto[3] = {w.h_ctr->local[0], w.h_ctr->local[1], w.h_ctr->local[2]};
-          const int erc = launch_early(upto);
-          if (erc) return erc;
-          early_strings += waiting;
-        }
-      }
-    }
-    stats.global_passes = pass;
-    for (int i = 0; i < 3; ++i) {
-      nlocal[i] = w.h_ctr->local[i];
-      local_strings[i] = w.h_ctr->local_strings[i];
-    }
-    nretire = w.h_ctr->retire;
-  }
-  // The local phase as one bracket ("local_phase"; its items, the strings
-  // summed over the local key rounds, are filled in after the read-back).
-  ProfToken phase_tok = prof_begin(c, "local_phase", 0, st);
-  if (nretire) GSS_LAUNCH("retire", nretire, launch_retire(p, nretire, st));
+  // Launch bounds of the passes.  The first pass's segments and tiles are
+  // known here; a later pass has at most one segment per T + 1 of the first
+  // pass's strings and at most (those strings) / kTile tiles beyond one per
+  // segment.  A pass's kernels are launched with those bounds and their
+  // blocks past the device counts exit at once, so passes follow each other
+  // with no host hand-over.  An empty block costs about 0.4 ns to dispatch
+  // (five such launches per pass), a hand-over about 20 us: past
+  // kBoundedTilesMax tiles (first passes of some 50M strings, e.g. 100M
+  // Zipf words) the host reads each later pass's counts after its set-up
+  // kernel instead and launches exact grids.
+  const uint64_t ub_segs = std::min<uint64_t>(p.cap_big, pass0_strings / ((uint64_t)T + 1) + 1);
+  const uint64_t ub_tiles = std::min<uint64_t>(p.cap_tiles, pass0_strings / kTile + ub_segs + 1);
+  constexpr uint64_t kBoundedTilesMax = 20480;
+  const bool read_counts = ub_tiles > kBoundedTilesMax;
+  static const char* kPassNames[10] = {"pass0", "pass1", "pass2", "pass3", "pass4",
+                                       "pass5", "pass6", "pass7", "pass8", "pass9+"};
   static const char* kLocalNames[3] = {"local128", "local1024", "local6144"};
-  long local_tok[3] = {-1, -1, -1};  // pending profile entries of the local launches
-  for (int cls = 2; cls >= 0; --cls) {
-    if (nlocal[cls] > p.cap_local[cls]) return fail(GSS_E_INTERNAL, "local segment list overflow");
-    if (nlocal[cls] > launched[cls]) {
-      const size_t mark = c.pending.size();
-      GSS_LAUNCH(kLocalNames[cls], 0,
-                 launch_local(p, cls, launched[cls], nlocal[cls] - launched[cls], st, count_levels));
-      if (c.pending.size() > mark) local_tok[cls] = (long)mark;
+  // Each pass either splits a segment or moves it >= 1 byte deeper, and no
+  // string reaches past the text, so the pass count is bounded.
+  const uint64_t max_passes = text_bytes + 64;
+  uint32_t pass = 0, cur = 0;
+  // Publishes `bytes` from the start of the statistics block (all of it and
+  // the counters behind it, or from the counters on) and waits for them.
+  auto read_back = [&](bool counters_only) -> int {
+    const uint64_t seq = ++w.pub_seq;
+    const size_t skip = counters_only ? w.init_ctr_off : 0;
+    GSS_HIP(launch_publish(reinterpret_cast<const char*>(p.lstats) + skip,
+                           w.init_ctr_off + sizeof(Counters) - skip, nullptr, nullptr, 0, 8,
+                           w.h_pub + kPubHead + skip, reinterpret_cast<uint64_t*>(w.h_pub), seq,
+                           st));
+    const int r = wait_published(w, st, seq);
+    if (r) return r;
+    memcpy(w.h_ctr, w.h_pub + kPubHead + w.init_ctr_off, sizeof(Counters));
+    if (w.h_ctr->error)
+      return fail(GSS_E_INTERNAL,
+                  "device sort failed (code " + std::to_string(w.h_ctr->error) + ")");
+    return GSS_OK;
+  };
+  // When the host has read the counters between two passes: the local
+  // segments the passes so far left are sorted on the side stream while the
+  // next pass runs if they hold at least half as many strings as it does.
+  // On 100M Zipf words that gains 3% over waiting for the end, while on 10M
+  // URL-like strings launching 2.9M local strings against a 7.1M-string pass
+  // made the sort 19% slower (one local workgroup fills a CU's LDS).
+  auto maybe_launch_early = [&](uint64_t next_strings) -> int {
+    uint64_t waiting = 0;
+    for (int cls = 0; cls < 3; ++cls) waiting += w.h_ctr->local_strings[cls];
+    waiting -= early_strings;
+    if (!waiting || 2 * waiting < next_strings) return GSS_OK;
+    uint32_t upto[3];
+    for (int cls = 0; cls < 3; ++cls) upto[cls] = std::min(w.h_ctr->local[cls], p.cap_local[cls]);
+    const int e = launch_early(upto);
+    if (e) return e;
+    early_strings += waiting;
+    return GSS_OK;
+  };
+  // Queues one pass; false when its set-up kernel found no segment (only
+  // known, and only checked, when the host reads the counts).
+  auto run_pass = [&](bool* ran) -> int {
+    *ran = true;
+    if (pass > max_passes) return fail(GSS_E_INTERNAL, "global passes do not terminate");
+    const uint64_t bound_tiles = pass == 0 ? tiles0 : ub_tiles;
+    const uint64_t bound_segs = pass == 0 ? nbig : ub_segs;
+    const BigSeg* segs = p.big[cur];
+    BigSeg* next = p.big[cur ^ 1];
+    ProfToken pass_tok = prof_begin(c, kPassNames[pass < 9 ? pass : 9], 0, st);
+    if (pass > 0)  // the first pass's tables came with the start of the sort
+      GSS_LAUNCH("plan", 0,
+                 launch_plan(p, segs, pass, (uint32_t)bound_tiles, (uint32_t)bound_segs, st));
+    uint32_t grid_tiles = (uint32_t)bound_tiles, grid_segs = (uint32_t)bound_segs;
+    if (pass > 0 && read_counts) {
+      const int r = read_back(true);
+      if (r) return r;
+      grid_tiles = w.h_ctr->ntiles;
+      grid_segs = w.h_ctr->big_cur;
+      if (!grid_segs) {
+        prof_end(c, pass_tok, st);
+        *ran = false;
+        return GSS_OK;
+      }
+      const int e = maybe_launch_early(w.h_ctr->active_cur);
+      if (e) return e;
     }
-    stats.local_segments += nlocal[cls];
-    stats.local_strings += local_strings[cls];
+    // Passes alternate: one reads its keys (and the next 8 bytes) from the
+    // text and hands the following pass its keys, which that pass takes as
+    // they lie.
+    p.keys_carried = pass % 2;
+    if (p.keys_carried) std::swap(p.keys, p.keys_next);
+    GSS_LAUNCH("gather", 0, launch_gather_minmax(p, segs, grid_tiles, st));
+    GSS_LAUNCH("segreduce", 0, launch_segreduce(p, grid_segs, st));
+    GSS_LAUNCH("layout", 0, launch_layout(p, rb, st));
+    GSS_LAUNCH("histogram", 0, launch_histogram(p, grid_tiles, rb, st));
+    GSS_LAUNCH("flat_scan", 0, launch_flat_scan(p, (uint64_t)grid_tiles * nb, st));
+    GSS_LAUNCH("segscan", 0, launch_segscan(p, segs, next, grid_segs, rb, st));
+    GSS_LAUNCH("scatter", 0, launch_scatter(p, segs, grid_tiles, rb, st));
+    if (lcp) GSS_LAUNCH("boundary", 0, launch_boundary_lcp(p, st));
+    prof_end(c, pass_tok, st);
+    cur ^= 1;
+    ++pass;
+    return GSS_OK;
+  };
+  if (nbig) {
+    // Passes are queued back to back, as many as the last sort on this
+    // context took, before the host reads the counters; while they show
+    // segments left for another pass, one more is queued.  With the host
+    // reading each pass's counts anyway, it runs passes until one finds no
+    // segment.
+    bool ran = true;
+    if (read_counts) {
+      while (ran)
+        if ((rc = run_pass(&ran))) return rc;
+      if ((rc = read_back(true))) return rc;
+    } else {
+      // A hint from a sort of another size would queue empty passes (or
+      // too few): then one pass at a time.
+      const bool similar = c.hint_strings && pass0_strings <= 2 * c.hint_strings &&
+                           c.hint_strings <= 2 * pass0_strings;
+      uint32_t queue = similar ? std::max<uint32_t>(1, c.pass_hint) : 1u;
+      while (queue) {
+        for (uint32_t k = 0; k < queue; ++k)
+          if ((rc = run_pass(&ran))) return rc;
+        if ((rc = read_back(true))) return rc;
+        queue = w.h_ctr->big_next ? 1u : 0u;
+        if (queue && (rc = maybe_launch_early(w.h_ctr->big_next_strings))) return rc;
+      }
+    }
+  }
+  // (With no global pass the counters are still the ones this call set.)
+  // The local phase, sized by the counters: every retire run and every
+  // local segment the passes produced beyond those the side stream took
+  // (one bracket, "local_phase"; its items are filled in after the
+  // read-back).
+  const uint32_t nretire = std::min(w.h_ctr->retire, p.cap_retire);
+  uint32_t nlocal[3];
+  for (int cls = 0; cls < 3; ++cls)
+    nlocal[cls] = std::min(w.h_ctr->local[cls], p.cap_local[cls]) - launched[cls];
+  ProfToken phase_tok = prof_begin(c, "local_phase", 0, st);
+  if (nretire) GSS_LAUNCH("retire", nretire, launch_retire(p, std::min<uint32_t>(nretire, 8192), st));
+  long local_tok[3] = {-1, -1, -1};
+  for (int cls = 2; cls >= 0; --cls) {
+    if (!nlocal[cls]) continue;
+    const size_t mark = c.pending.size();
+    GSS_LAUNCH(kLocalNames[cls], 0,
+               launch_local(p, cls, launched[cls], nlocal[cls], st, count_levels));
+    if (c.pending.size() > mark) local_tok[cls] = (long)mark;
   }
   const size_t before_end = c.pending.size();
   prof_end(c, phase_tok, st);
@@ -627,19 +680,29 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     GSS_HIP(hipEventRecord(c.side_done, c.side));
     GSS_HIP(hipStreamWaitEvent(st, c.side_done, 0));
   }
-  // The local statistics and the counters behind them come back together.
-  const uint64_t seq = ++w.pub_seq;
-  GSS_HIP(launch_publish(p.lstats, w.init_ctr_off + sizeof(Counters), nullptr, nullptr, 0, 8,
-                         w.h_pub + kPubHead, reinterpret_cast<uint64_t*>(w.h_pub), seq, st));
-  rc = wait_published(w, st, seq);
-  if (rc) return rc;
+  if ((rc = read_back(false))) return rc;
   // Event brackets are read once their stream has passed them.
   if (!c.pending.empty()) GSS_HIP(hipStreamSynchronize(st));
-  memcpy(w.h_ctr, w.h_pub + kPubHead + w.init_ctr_off, sizeof(Counters));
+  uint32_t passes = 0;
+  for (uint32_t i = 0; i < pass && i < (uint32_t)kMaxPasses; ++i) {
+    const uint64_t act = i == 0 ? pass0_strings : w.h_pass[i];
+    if (i < GSS_MAX_PASSES) stats.active[i] = act;
+    if (act) passes = i + 1;
+  }
+  stats.global_passes = passes;
+  if (nbig) {
+    c.pass_hint = std::max<uint32_t>(1, passes);
+    c.hint_strings = pass0_strings;
+  }
+  for (int cls = 0; cls < 3; ++cls) {
+    stats.local_segments += w.h_ctr->local[cls];
+    stats.local_strings += w.h_ctr->local_strings[cls];
+  }
   // A local launch's unit of work is one active string in one key round;
-  // block 0 of the statistics holds the final launches, block 1 the early ones.
-  // The profiled unit of a local launch is one string through one 8-byte key
-  // level (the algorithmic unit of an MSD round with 8-byte keys).
+  // block 0 of the statistics holds the main stream's launches, block 1 the
+  // side stream's.  The profiled unit of a local launch is one string through
+  // one 8-byte key level (the algorithmic unit of an MSD round with 8-byte
+  // keys).
   uint64_t round_strings[2][3] = {{0, 0, 0}, {0, 0, 0}}, rounds_max = 0;
   uint64_t level_strings[2][3] = {{0, 0, 0}, {0, 0, 0}};
   for (int b = 0; b < 2; ++b)
@@ -655,15 +718,19 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     stats.local_round_strings += round_strings[0][cls] + round_strings[1][cls];
     stats.local_level_strings += level_strings[0][cls] + level_strings[1][cls];
     final_level_strings += level_strings[0][cls];
-    if (local_tok[cls] >= 0) c.pending[local_tok[cls]].items = level_strings[0][cls];
     if (side_tok[cls] >= 0) c.pending[side_tok[cls]].items = level_strings[1][cls];
   }
+  for (int cls = 0; cls < 3; ++cls)
+    if (local_tok[cls] >= 0) c.pending[local_tok[cls]].items = level_strings[0][cls];
   stats.local_rounds_max = (uint32_t)rounds_max;
   if (phase_idx >= 0) c.pending[phase_idx].items = final_level_strings;
+  // Pass brackets: the units are the pass's active strings.
+  for (auto& pe : c.pending)
+    if (pe.name.compare(0, 4, "pass") == 0) {
+      const int r = pe.name == "pass9+" ? 9 : pe.name[4] - '0';
+      if (r < 9 && r < (int)pass) pe.items = r == 0 ? pass0_strings : w.h_pass[r];
+    }
   prof_collect(c);
-  if (w.h_ctr->error)
-    return fail(GSS_E_INTERNAL,
-                "device sort failed (code " + std::to_string(w.h_ctr->error) + ")");
   return GSS_OK;
 }
 
