@@ -256,9 +256,12 @@ int gss_merge_lcp_kway_device32(const unsigned char *d_text, uint64_t text_bytes
 typedef struct gss_tuning {
 	uint32_t local_threshold; /* segments of at most this many strings finish
 				     inside one workgroup (64..6144)          */
-	uint32_t radix_bits;      /* digit width of a global MSD pass (8..11)  */
+	uint32_t radix_bits;      /* digit width of a global MSD pass (8 or 11) */
 	uint32_t tile;            /* strings per workgroup in a global pass
-				     (1024..16384, multiple of 1024)           */
+				     (2048 or 4096)                            */
+	uint32_t key_bytes;       /* bytes of every string the largest local
+				     class reads and compares at once in a
+				     segment's first round (32 or 64)         */
 } gss_tuning;
 
 int gss_get_tuning(gss_tuning *out);
@@ -267,10 +270,13 @@ int gss_set_tuning(const gss_tuning *in);
 /* Autotuner (no reference counterpart).  An input's signature is its mean
  * string length, alphabet size and string count, each as a power-of-two
  * bucket, read from 512 evenly spaced strings.  gss_autotune_device32 sorts
- * a strided sample of the input (at most 2M strings) with every candidate
- * tuning -- local_threshold 1024/2048/4096/6144 x radix_bits 11/8 -- and
- * returns the fastest (best_ms; default_ms is the current tuning's time, 0
- * when it is not a candidate), caching it for the signature.  It does not
+ * a strided sample of the input (at most 2M strings) with the candidate
+ * tunings -- local_threshold 1024/2048/4096/6144 x radix_bits 11/8 with the
+ * current tile and key_bytes, then the best of those with the other tile
+ * size (2048/4096), then with the other key_bytes (32/64): 10 trials when
+ * the current tuning is one of them -- and returns the fastest (best_ms;
+ * default_ms is the current tuning's time, 0 when it is not a candidate),
+ * caching it for the signature.  It does not
  * change the current tuning.  With gss_autotune_enable(1), gss_sort_device32
  * (and the pointer entry points) sort inputs of 65536 strings or more with
  * the cached tuning of their signature, tuning a new signature first. */

@@ -1,14 +1,19 @@
-// Autotuner: picks the local threshold (strings a workgroup finishes) and
-// the digit width of the global passes per input signature (no reference
+// Autotuner: picks the local threshold (strings a workgroup finishes), the
+// digit width and the tile size of the global passes, and the first-round
+// key window of the largest local class per input signature (no reference
 // counterpart: the reference fixes its thresholds at compile time, e.g.
-// src/msd_ce.cpp:79 and :216).
+// src/msd_ce.cpp:79 and :216, and its cached key width, src/msd_a.cpp:39,
+// src/multikey_cache.cpp:39-55).
 //
 // The signature of an input is (mean string length, alphabet size, count),
 // each as a power-of-two bucket, estimated from 512 strings at evenly spaced
 // positions of the offset array.  Tuning an input sorts a strided sample of
 // it (at most 2M strings, which keeps its length distribution and duplicate
-// ratio) with every candidate and keeps the fastest; the result is cached
-// per signature.  When enabled, gss_sort_device32 uses the cached tuning of
+// ratio) with the candidates and keeps the fastest: every threshold x digit
+// width with the current tile and key window, then the best of those with
+// each other tile, then with each other key window (the later knobs are
+// tuned around the earlier ones rather than in all 32 combinations); the
+// result is cached per signature.  When enabled, gss_sort_device32 uses the cached tuning of
 // its input's signature and tunes signatures it has not seen yet.
 
 #include <chrono>
@@ -31,6 +36,8 @@ constexpr int kTimedTrials = 2;
 
 constexpr uint32_t kThresholds[] = {1024, 2048, 4096, 6144};
 constexpr uint32_t kRadixBits[] = {11, 8};
+constexpr uint32_t kTiles[] = {4096, 2048};
+constexpr uint32_t kKeyBytes[] = {64, 32};
 
 struct SigOut {
   uint32_t alpha[8];            // byte values seen in the first 32 bytes
@@ -162,29 +169,38 @@ int run_trials(const unsigned char* text, uint64_t text_bytes, const uint32_t* i
   if (hipGetLastError() != hipSuccess) rc = fail(GSS_E_DEVICE, "autotuner sample launch");
   // The trial sorts may run on another stream: the sample must be complete.
   if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(GSS_E_DEVICE, "autotuner sample");
-  for (uint32_t rb : kRadixBits) {
-    for (uint32_t th : kThresholds) {
-      if (rc) break;
-      const gss_tuning cand{th, rb, def.tile};
-      double best = 1e30;
-      for (int t = 0; t <= kTimedTrials && !rc; ++t) {  // first run is a warm-up
-        const auto t0 = std::chrono::steady_clock::now();
-        rc = sort_device32_tuned(text, text_bytes, sample, out, m, st, cand);
-        const double ms =
-            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
-                .count();
-        if (t > 0 && ms < best) best = ms;
-      }
-      if (rc) break;
-      ++res->candidates;
-      if (cand.local_threshold == def.local_threshold && cand.radix_bits == def.radix_bits)
-        res->default_ms = best;
-      if (best < res->best_ms) {
-        res->best_ms = best;
-        res->best = cand;
-      }
+  // Times one candidate (a warm-up, then the best of kTimedTrials sorts).
+  auto trial = [&](const gss_tuning& cand) {
+    double best = 1e30;
+    for (int t = 0; t <= kTimedTrials && !rc; ++t) {  // first run is a warm-up
+      const auto t0 = std::chrono::steady_clock::now();
+      rc = sort_device32_tuned(text, text_bytes, sample, out, m, st, cand);
+      const double ms =
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+              .count();
+      if (t > 0 && ms < best) best = ms;
     }
-  }
+    if (rc) return;
+    ++res->candidates;
+    if (cand.local_threshold == def.local_threshold && cand.radix_bits == def.radix_bits &&
+        cand.tile == def.tile && cand.key_bytes == def.key_bytes)
+      res->default_ms = best;
+    if (best < res->best_ms) {
+      res->best_ms = best;
+      res->best = cand;
+    }
+  };
+  for (uint32_t rb : kRadixBits)
+    for (uint32_t th : kThresholds)
+      if (!rc) trial(gss_tuning{th, rb, def.tile, def.key_bytes});
+  for (uint32_t tile : kTiles)
+    if (!rc && tile != res->best.tile) trial(gss_tuning{res->best.local_threshold,
+                                                        res->best.radix_bits, tile,
+                                                        res->best.key_bytes});
+  for (uint32_t kb : kKeyBytes)
+    if (!rc && kb != res->best.key_bytes) trial(gss_tuning{res->best.local_threshold,
+                                                           res->best.radix_bits, res->best.tile,
+                                                           kb});
   (void)hipStreamSynchronize(st);
   (void)hipFree(buf);
   return rc;

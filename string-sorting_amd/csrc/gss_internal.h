@@ -13,10 +13,12 @@ constexpr int kWave = 64;
 // Largest segment a single workgroup finishes (the local kernel's capacity).
 constexpr uint32_t kLocalCapMax = 6144;
 
-// Global MSD pass geometry: strings per tile = kTileThreads * kTileItems.
+// Global MSD pass geometry: a tile is kTileThreads threads with 4 or 8
+// strings each, 2048 or 4096 strings (the tuning's tile); kTile is the
+// default.
 constexpr int kTileThreads = 512;
-constexpr int kTileItems = 8;
-constexpr uint32_t kTile = kTileThreads * kTileItems;
+constexpr uint32_t kTile = 4096;
+constexpr uint32_t kTileMin = 2048;
 
 // Retire ranges are split into chunks of at most this many strings so that a
 // huge run of identical strings spreads over many workgroups.
@@ -35,7 +37,7 @@ struct BigSeg {
 struct Tile {
   uint32_t seg;    // index into the pass's BigSeg list
   uint32_t begin;  // absolute position of the first string
-  uint32_t len;    // strings in this tile (<= kTile)
+  uint32_t len;    // strings in this tile (<= the pass's tile size)
   uint32_t tidx;   // index of this tile inside its segment
 };
 
@@ -157,6 +159,8 @@ struct Plan {
   uint32_t* tile_alpha;  // per tile: 256-bit byte-presence set
   uint32_t* seg_alpha;   // per BigSeg: 256-bit byte-presence set
   uint32_t local_threshold;
+  uint32_t tile;         // strings per tile of a global pass (2048 or 4096)
+  uint32_t key_bytes;    // first-round window of the largest local class (32 or 64)
   uint32_t cap_big, cap_retire, cap_bounds, cap_tiles;
   uint32_t cap_local[3];
 };
@@ -357,6 +361,41 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* scratc
   *total = scratch[W];
   __syncthreads();
   return r;
+}
+
+// K block-wide exclusive scans at once (one pair of barriers for all of
+// them): v[k] becomes its exclusive prefix, total[k] the block total.
+// scratch holds K * (BLOCK/64 + 1) words.  Every thread must call it.
+template <int BLOCK, int K>
+__device__ __forceinline__ void block_excl_scan_multi(uint32_t (&v)[K], uint32_t* scratch,
+                                                      uint32_t (&total)[K]) {
+  constexpr int W = BLOCK / kWave;
+  const int wid = threadIdx.x / kWave;
+  const int lane = (int)lane_id();
+  uint32_t inc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) inc[k] = wave_incl_scan(v[k]);
+  if (lane == kWave - 1) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) scratch[k * (W + 1) + wid] = inc[k];
+  }
+  __syncthreads();
+  if (wid == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t t = lane < W ? scratch[k * (W + 1) + lane] : 0u;
+      const uint32_t ti = wave_incl_scan(t);
+      if (lane < W) scratch[k * (W + 1) + lane] = ti - t;
+      if (lane == W - 1) scratch[k * (W + 1) + W] = ti;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    v[k] = scratch[k * (W + 1) + wid] + inc[k] - v[k];
+    total[k] = scratch[k * (W + 1) + W];
+  }
+  __syncthreads();
 }
 
 template <int BLOCK>

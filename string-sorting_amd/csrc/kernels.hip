@@ -3,7 +3,7 @@
 //
 // A sort runs global MSD passes over segments too large for one workgroup,
 // then finishes every remaining segment inside one workgroup
-// (local_sort.hip).  One global pass over the BigSeg list, tiles of kTile
+// (local_sort.hip).  One global pass over the BigSeg list, tiles of p.tile
 // strings that never span two segments:
 //
 //   gather_minmax  key = the 8 bytes at each string's depth (big-endian, NUL
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_copy_chunks(const CopyChunk* __restrict
 // ---------------------------------------------------------------------------
 // Global pass, step 0 (one workgroup): the pass's segments are the list the
 // previous pass (or the start of the sort) left, big_next of them.  Segment
-// i gets ceil(size / kTile) tiles, numbered in segment order (tile0 = the
+// i gets ceil(size / tile) tiles, numbered in segment order (tile0 = the
 // exclusive scan of the counts); tile t finds its segment by a binary search
 // over tile0.  The counts go to big_cur / ntiles for the pass's kernels, the
 // boundaries emitted so far are marked done (this pass's boundary kernel
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(Plan p, const BigSeg* seg
     uint32_t nt = 0;
     if (i < nsegs) {
       const uint32_t sz = segs[i].size;
-      nt = (sz + kTile - 1) / kTile;
+      nt = (sz + p.tile - 1) / p.tile;
       act += sz;
     }
     uint32_t total;
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(Plan p, const BigSeg* seg
     }
     const BigSeg sg = segs[lo];
     const uint32_t k = t - (in_lds ? s_tile0[lo] : p.aux[lo].tile0);
-    p.tiles[t] = Tile{lo, sg.begin + k * kTile, min(kTile, sg.size - k * kTile), k};
+    p.tiles[t] = Tile{lo, sg.begin + k * p.tile, min(p.tile, sg.size - k * p.tile), k};
   }
   // Active strings of the pass: a block sum of the 64-bit partial sums.
 #pragma unroll
@@ -290,6 +290,7 @@ __device__ __forceinline__ uint64_t match_digit_n(uint32_t d, uint64_t live, uin
 // byte values seen (no atomics: thousands of tiles would serialise on one
 // segment's words).
 
+template <int TI>
 __device__ __forceinline__ void gather_tile(Plan& p, const BigSeg* segs, uint32_t b,
                                             uint32_t nt) {
   __shared__ uint64_t s_mn[kTileThreads / kWave], s_mx[kTileThreads / kWave];
@@ -299,35 +300,35 @@ __device__ __forceinline__ void gather_tile(Plan& p, const BigSeg* segs, uint32_
   const Tile t = p.tiles[ti];
   const BigSeg sg = segs[t.seg];
   const uint32_t* __restrict__ src = p.idx[sg.buf];
-  uint32_t o[kTileItems];
-  uint64_t k[kTileItems];
+  uint32_t o[TI];
+  uint64_t k[TI];
   uint64_t mn = ~0ull, mx = 0ull;
   if (p.keys_carried) {
     // The previous pass left these keys at the strings' positions.
 #pragma unroll
-    for (int j = 0; j < kTileItems; ++j) {
+    for (int j = 0; j < TI; ++j) {
       const uint32_t i = j * kTileThreads + threadIdx.x;
       k[j] = i < t.len ? p.keys[t.begin + i] : 0ull;
     }
   } else {
 #pragma unroll
-    for (int j = 0; j < kTileItems; ++j) {
+    for (int j = 0; j < TI; ++j) {
       const uint32_t i = j * kTileThreads + threadIdx.x;
       o[j] = i < t.len ? src[t.begin + i] : 0u;
     }
     // Unconditional (out-of-tile slots read offset 0) so all loads issue
     // before the first wait.  16 bytes: the key and the 8 bytes after it,
     // from which the scatter forms the next pass's keys.
-    uint64_t x[kTileItems];
+    uint64_t x[TI];
 #pragma unroll
-    for (int j = 0; j < kTileItems; ++j) {
+    for (int j = 0; j < TI; ++j) {
       uint64_t kk[2];
       load_keys<2>(p.text64, p.text_bytes, (uint64_t)o[j] + sg.depth, kk);
       k[j] = kk[0];
       x[j] = kk[1];
     }
 #pragma unroll
-    for (int j = 0; j < kTileItems; ++j) {
+    for (int j = 0; j < TI; ++j) {
       const uint32_t i = j * kTileThreads + threadIdx.x;
       if (i < t.len) {
         p.keys[t.begin + i] = k[j];
@@ -337,7 +338,7 @@ __device__ __forceinline__ void gather_tile(Plan& p, const BigSeg* segs, uint32_
   }
   __syncthreads();  // s_present cleared
 #pragma unroll
-  for (int j = 0; j < kTileItems; ++j) {
+  for (int j = 0; j < TI; ++j) {
     const uint32_t i = j * kTileThreads + threadIdx.x;
     if (i < t.len) {
       mn = k[j] < mn ? k[j] : mn;
@@ -377,9 +378,10 @@ __device__ __forceinline__ void gather_tile(Plan& p, const BigSeg* segs, uint32_
 // (an upper bound), the rest exit at once.  (A loop over tiles would let one
 // launch take any count, but the compiler then keeps the per-thread indices
 // of every item live across it: far more registers, lower occupancy.)
+template <int TI>
 __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const BigSeg* segs) {
   const uint32_t nt = p.ctr->ntiles;
-  if (blockIdx.x < nt) gather_tile(p, segs, blockIdx.x, nt);
+  if (blockIdx.x < nt) gather_tile<TI>(p, segs, blockIdx.x, nt);
 }
 
 // One workgroup per segment: min/max key and byte set over its tiles.  A
@@ -476,7 +478,7 @@ __global__ __launch_bounds__(1024) void k_layout(Plan p) {
 // ---------------------------------------------------------------------------
 // Global pass, step 3: per-tile digit counts, digit-major per segment.
 
-template <int RB>
+template <int RB, int TI>
 __device__ __forceinline__ void histogram_tile(Plan& p, uint32_t b, uint32_t nt) {
   constexpr uint32_t NB = 1u << RB;
   __shared__ uint32_t h[NB];
@@ -486,9 +488,9 @@ __device__ __forceinline__ void histogram_tile(Plan& p, uint32_t b, uint32_t nt)
   // min/max can end the block (a load after that return cannot be hoisted
   // above it): the keys (slots past the tile read its last key and are not
   // counted), the alphabet and the segment's count-matrix place.
-  uint64_t key[kTileItems];
+  uint64_t key[TI];
 #pragma unroll
-  for (int j = 0; j < kTileItems; ++j) {
+  for (int j = 0; j < TI; ++j) {
     const uint32_t i = j * kTileThreads + threadIdx.x;
     key[j] = p.keys[t.begin + (i < t.len ? i : t.len - 1)];
   }
@@ -502,7 +504,7 @@ __device__ __forceinline__ void histogram_tile(Plan& p, uint32_t b, uint32_t nt)
   for (uint32_t d = threadIdx.x; d < ds.nbu; d += kTileThreads) h[d] = 0;
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < kTileItems; ++j) {
+  for (int j = 0; j < TI; ++j) {
     const uint32_t i = j * kTileThreads + threadIdx.x;
     if (i < t.len) atomicAdd(&h[digit_of(key[j], ds, rank)], 1u);
   }
@@ -511,10 +513,10 @@ __device__ __forceinline__ void histogram_tile(Plan& p, uint32_t b, uint32_t nt)
   for (uint32_t d = threadIdx.x; d < ds.nbu; d += kTileThreads) col[(uint64_t)d * ax.ntiles] = h[d];
 }
 
-template <int RB>
+template <int RB, int TI>
 __global__ __launch_bounds__(kTileThreads) void k_histogram(Plan p) {
   const uint32_t nt = p.ctr->ntiles;
-  if (blockIdx.x < nt) histogram_tile<RB>(p, blockIdx.x, nt);
+  if (blockIdx.x < nt) histogram_tile<RB, TI>(p, blockIdx.x, nt);
 }
 
 // ---------------------------------------------------------------------------
@@ -766,20 +768,17 @@ __device__ __forceinline__ void segscan_seg(Plan& p, const BigSeg* segs, BigSeg*
 
   // One reservation per list for the whole segment: same-address atomics
   // from every bucket would serialise.
-  __shared__ uint32_t s_scan[256 / kWave + 1];
+  __shared__ uint32_t s_scan[9 * (256 / kWave + 1)];
   __shared__ uint32_t s_base[6];
-  uint32_t tb, tr, tg, t0, t1, t2;
-  uint32_t ob = block_excl_scan<256>(nb, s_scan, &tb);
-  uint32_t orr = block_excl_scan<256>(nr, s_scan, &tr);
-  uint32_t og = block_excl_scan<256>(ng, s_scan, &tg);
-  uint32_t ol[3];
-  ol[0] = block_excl_scan<256>(nl[0], s_scan, &t0);
-  ol[1] = block_excl_scan<256>(nl[1], s_scan, &t1);
-  ol[2] = block_excl_scan<256>(nl[2], s_scan, &t2);
-  uint32_t ts0, ts1, ts2;
-  (void)block_excl_scan<256>(sl[0], s_scan, &ts0);
-  (void)block_excl_scan<256>(sl[1], s_scan, &ts1);
-  (void)block_excl_scan<256>(sl[2], s_scan, &ts2);
+  // Nine counts per thread, scanned together: boundaries, retire chunks,
+  // next-pass segments, local segments and their strings per class.
+  uint32_t sc[9] = {nb, nr, ng, nl[0], nl[1], nl[2], sl[0], sl[1], sl[2]};
+  uint32_t tot[9];
+  block_excl_scan_multi<256, 9>(sc, s_scan, tot);
+  uint32_t ob = sc[0], orr = sc[1], og = sc[2];
+  uint32_t ol[3] = {sc[3], sc[4], sc[5]};
+  const uint32_t tb = tot[0], tr = tot[1], tg = tot[2], t0 = tot[3], t1 = tot[4], t2 = tot[5];
+  const uint32_t ts0 = tot[6], ts1 = tot[7], ts2 = tot[8];
   if (threadIdx.x == 0) {
     const uint32_t tl[3] = {t0, t1, t2}, ts[3] = {ts0, ts1, ts2};
     s_base[0] = tb ? atomicAdd(&p.ctr->boundary, tb) : 0u;
@@ -840,19 +839,19 @@ __global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, Big
 // among the tile's strings of its digit comes from wave match-ranking; the
 // scanned flat array gives where the tile's strings of each digit start.
 
-template <int RB>
+template <int RB, int TI>
 __device__ __forceinline__ void scatter_tile(Plan& p, const BigSeg* segs, uint32_t b,
                                              uint32_t nt) {
   constexpr uint32_t NB = 1u << RB;
   constexpr int W = kTileThreads / kWave;
-  constexpr int IPW = kTile / W;  // items per wave
+  constexpr int IPW = kTileThreads * TI / W;  // items per wave
   constexpr uint32_t DPT = NB >= kTileThreads ? NB / kTileThreads : 1;
   __shared__ __attribute__((aligned(16))) uint16_t s_whist[NB][W];  // digit-major
   __shared__ uint32_t s_tstart[NB];
   __shared__ uint32_t s_gbase[NB];
   __shared__ uint8_t s_kind[NB];  // kBucket* kind of each digit's bucket
-  __shared__ uint32_t s_off[kTile];
-  __shared__ uint16_t s_dig[kTile];
+  __shared__ uint32_t s_off[kTileThreads * TI];
+  __shared__ uint16_t s_dig[kTileThreads * TI];
   __shared__ uint32_t s_scan[W + 1];
   __shared__ uint8_t rank[256];
 
@@ -866,10 +865,10 @@ __device__ __forceinline__ void scatter_tile(Plan& p, const BigSeg* segs, uint32
   // its last string and are masked out), the alphabet and the segment's
   // count-matrix place.  The offsets follow once the segment names the
   // buffer that holds them.
-  uint64_t key[kTileItems];
-  uint64_t nk[kTileItems];  // key of the string at its child's depth (carry)
+  uint64_t key[TI];
+  uint64_t nk[TI];  // key of the string at its child's depth (carry)
 #pragma unroll
-  for (int j = 0; j < kTileItems; ++j) {
+  for (int j = 0; j < TI; ++j) {
     const uint32_t i = wid * IPW + j * kWave + lane;
     const uint32_t at = t.begin + (i < t.len ? i : t.len - 1);
     key[j] = p.keys[at];
@@ -894,9 +893,9 @@ __device__ __forceinline__ void scatter_tile(Plan& p, const BigSeg* segs, uint32
   const uint32_t fb = ds.fb0 + ds.k;  // bytes the children share: their keys start there
   const uint32_t* __restrict__ src = p.idx[sg.buf];
   uint32_t* __restrict__ dst = p.idx[sg.buf == 1 ? 2 : 1];
-  uint32_t off[kTileItems];
+  uint32_t off[TI];
 #pragma unroll
-  for (int j = 0; j < kTileItems; ++j) {
+  for (int j = 0; j < TI; ++j) {
     const uint32_t i = wid * IPW + j * kWave + lane;
     off[j] = src[t.begin + (i < t.len ? i : t.len - 1)];
   }
@@ -907,14 +906,14 @@ __device__ __forceinline__ void scatter_tile(Plan& p, const BigSeg* segs, uint32
   __syncthreads();
 
   const uint64_t lt = lanemask_lt();
-  uint32_t dig[kTileItems], rnk[kTileItems];
+  uint32_t dig[TI], rnk[TI];
 #pragma unroll
-  for (int j = 0; j < kTileItems; ++j) {
+  for (int j = 0; j < TI; ++j) {
     dig[j] = digit_of(key[j], ds, rank);
     if (carry) nk[j] = fb >= 8 ? nk[j] : (key[j] << (8 * fb)) | (nk[j] >> (64 - 8 * fb));
   }
 #pragma unroll
-  for (int j = 0; j < kTileItems; ++j) {
+  for (int j = 0; j < TI; ++j) {
     const uint32_t i = wid * IPW + j * kWave + lane;
     const bool v = i < t.len;
     const uint32_t d = v ? dig[j] : 0u;
@@ -980,7 +979,7 @@ __device__ __forceinline__ void scatter_tile(Plan& p, const BigSeg* segs, uint32
   }
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < kTileItems; ++j) {
+  for (int j = 0; j < TI; ++j) {
     const uint32_t i = wid * IPW + j * kWave + lane;
     if (i < t.len) {
       const uint32_t d = dig[j];
@@ -1008,10 +1007,10 @@ __device__ __forceinline__ void scatter_tile(Plan& p, const BigSeg* segs, uint32
   }
 }
 
-template <int RB>
+template <int RB, int TI>
 __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* segs) {
   const uint32_t nt = p.ctr->ntiles;
-  if (blockIdx.x < nt) scatter_tile<RB>(p, segs, blockIdx.x, nt);
+  if (blockIdx.x < nt) scatter_tile<RB, TI>(p, segs, blockIdx.x, nt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1070,7 +1069,11 @@ hipError_t launch_segreduce(const Plan& p, uint32_t grid, hipStream_t s) {
 hipError_t launch_gather_minmax(const Plan& p, const BigSeg* segs, uint32_t grid,
                                 hipStream_t s) {
   if (!grid) return hipSuccess;
-  k_gather_minmax<<<grid, kTileThreads, 0, s>>>(p, segs);
+  switch (p.tile) {
+    case 2048: k_gather_minmax<4><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+    case 4096: k_gather_minmax<8><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -1086,12 +1089,15 @@ hipError_t launch_layout(const Plan& p, uint32_t radix_bits, hipStream_t s) {
 
 hipError_t launch_histogram(const Plan& p, uint32_t grid, uint32_t radix_bits, hipStream_t s) {
   if (!grid) return hipSuccess;
-  if (radix_bits == 8)
-    k_histogram<8><<<grid, kTileThreads, 0, s>>>(p);
-  else if (radix_bits == 11)
-    k_histogram<11><<<grid, kTileThreads, 0, s>>>(p);
-  else
-    return hipErrorInvalidValue;
+  const int v = (radix_bits == 11 ? 2 : (radix_bits == 8 ? 0 : 4)) + (p.tile == 4096 ? 1 : 0);
+  if (p.tile != 2048 && p.tile != 4096) return hipErrorInvalidValue;
+  switch (v) {
+    case 0: k_histogram<8, 4><<<grid, kTileThreads, 0, s>>>(p); break;
+    case 1: k_histogram<8, 8><<<grid, kTileThreads, 0, s>>>(p); break;
+    case 2: k_histogram<11, 4><<<grid, kTileThreads, 0, s>>>(p); break;
+    case 3: k_histogram<11, 8><<<grid, kTileThreads, 0, s>>>(p); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -1118,12 +1124,15 @@ hipError_t launch_segscan(const Plan& p, const BigSeg* segs, BigSeg* next, uint3
 hipError_t launch_scatter(const Plan& p, const BigSeg* segs, uint32_t grid,
                           uint32_t radix_bits, hipStream_t s) {
   if (!grid) return hipSuccess;
-  if (radix_bits == 8)
-    k_scatter<8><<<grid, kTileThreads, 0, s>>>(p, segs);
-  else if (radix_bits == 11)
-    k_scatter<11><<<grid, kTileThreads, 0, s>>>(p, segs);
-  else
-    return hipErrorInvalidValue;
+  const int v = (radix_bits == 11 ? 2 : (radix_bits == 8 ? 0 : 4)) + (p.tile == 4096 ? 1 : 0);
+  if (p.tile != 2048 && p.tile != 4096) return hipErrorInvalidValue;
+  switch (v) {
+    case 0: k_scatter<8, 4><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+    case 1: k_scatter<8, 8><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+    case 2: k_scatter<11, 4><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+    case 3: k_scatter<11, 8><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

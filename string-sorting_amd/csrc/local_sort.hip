@@ -36,13 +36,14 @@ namespace gss {
 
 constexpr int kLocalWords = 4;
 
-// Words the largest class's first round reads per string (see k_local): 64
-// bytes, so a group whose strings share up to 56 more bytes (URL-like prefix
-// groups) is split in that round instead of after one or more rounds that
-// only find the next 32 bytes equal.  The smaller classes keep the ordinary
+// The largest class's first round (see k_local) reads PW words per string,
+// the tuning's key_bytes: 64 by default, so a group whose strings share up to
+// 56 more bytes (URL-like prefix groups) is split in that round instead of
+// after one or more rounds that only find the next 32 bytes equal; 32 reads
+// no more than an ordinary round.  The smaller classes keep the ordinary
 // first round: with 32-byte windows the probe saves them no bytes, and its
 // registers cost them a wave per SIMD.
-constexpr int kProbeWords = 8;
+constexpr int kProbeWordsMax = 8;
 constexpr bool probes_first_round(uint32_t cap) { return cap > kLocalClassCap[1]; }
 
 // Largest group a round still sorts without the LSD passes over all items.
@@ -77,7 +78,7 @@ struct LocalSmem {
   uint8_t ss_stop[(SS + 3) & ~3];  // bit q: key q varies or holds a NUL in the group
   alignas(8) uint16_t whist[256][W];  // per-wave digit counters, digit-major
   uint64_t edge[W][kLocalWords];  // keys of each wave's last item
-  uint64_t refw[kProbeWords];  // first round: the reference string's window
+  uint64_t refw[kProbeWordsMax];  // first round: the reference string's window
   uint32_t wvm[W];   // per wave: varying key bytes | 0x100 if a group is > insertion_max
   uint32_t wtot[W];  // per wave: survivors | their group heads << 16
   uint32_t scan[W + 1];
@@ -313,8 +314,9 @@ constexpr int local_min_waves(uint32_t cap) {
   return cap <= kLocalClassCap[0] ? 7 : (cap <= kLocalClassCap[1] ? 6 : 4);
 }
 
-// Block b finishes segment first + b of the class's list.
-template <int BLOCK, int CAP, bool COUNT>
+// Block b finishes segment first + b of the class's list; PW: the first
+// round's words per string (largest class only).
+template <int BLOCK, int CAP, bool COUNT, int PW = kProbeWordsMax>
 __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, uint32_t first) {
   constexpr int W = BLOCK / kWave;
   constexpr int IPT = CAP / BLOCK;
@@ -375,7 +377,6 @@ __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, u
       //    its word e is that one when l_i / 8 == e, else it equals the
       //    reference's word e.  A window with no mismatch and no NUL moves
       //    the group PW words deeper.
-      constexpr int PW = kProbeWords;
 #pragma unroll
       for (int j = 0; j < IPT; ++j)
         gv[j] = (uint32_t)j < jn && base + j * kWave < nact ? 0u : 0xFFFFu;
@@ -756,7 +757,10 @@ hipError_t launch_local_t(const Plan& p, int cls, uint32_t first, uint32_t nsegs
       k_local<256, 1024, COUNT><<<nsegs, 256, 0, s>>>(p, first);
       break;
     case 2:
-      k_local<1024, 6144, COUNT><<<nsegs, 1024, 0, s>>>(p, first);
+      if (p.key_bytes == 32)
+        k_local<1024, 6144, COUNT, 4><<<nsegs, 1024, 0, s>>>(p, first);
+      else
+        k_local<1024, 6144, COUNT, 8><<<nsegs, 1024, 0, s>>>(p, first);
       break;
     default:
       return hipErrorInvalidValue;

@@ -29,6 +29,7 @@ struct Workspace {
   uint64_t n_cap = 0;
   uint32_t rb = 0;
   uint32_t threshold = 0;
+  uint32_t tile = 0;
   Plan plan{};
   uint32_t* idx_copy = nullptr;
   // Pinned host mirrors.
@@ -67,7 +68,7 @@ struct Pending {
 };
 
 std::mutex g_mu;  // tuning, profile totals, default contexts
-gss_tuning g_tuning = {6144u, 11u, kTile};
+gss_tuning g_tuning = {6144u, 11u, kTile, 64u};
 std::atomic<bool> g_prof{false};    // HIP event brackets
 std::atomic<bool> g_levels{false};  // counting variant of the local kernels
 std::map<std::string, ProfTotal> g_prof_totals;
@@ -243,11 +244,11 @@ struct Caps {
   uint64_t ncap, big, tiles, local[3], retire, bounds;
 };
 
-Caps caps_for(uint64_t ncap, uint32_t t) {
+Caps caps_for(uint64_t ncap, uint32_t t, uint32_t tile) {
   Caps k;
   k.ncap = std::max<uint64_t>(ncap, 4096);
   k.big = k.ncap / (t + 1) + 2;
-  k.tiles = k.ncap / kTile + k.big + 2;
+  k.tiles = k.ncap / tile + k.big + 2;
   k.local[0] = k.ncap / 2 + 2;
   k.local[1] = k.ncap / (kLocalClassCap[0] + 1) + 2;
   k.local[2] = k.ncap / (kLocalClassCap[1] + 1) + 2;
@@ -301,13 +302,14 @@ size_t items_bytes(const std::vector<Item>& items) {
   return total;
 }
 
-// Sizes every device array for up to n strings, 2^rb digits and local
-// threshold t, and carves them out of one allocation.
-int ensure_workspace(Context& c, uint64_t n, uint32_t rb, uint32_t t) {
+// Sizes every device array for up to n strings, 2^rb digits, local
+// threshold t and tiles of `tile` strings, and carves them out of one
+// allocation.
+int ensure_workspace(Context& c, uint64_t n, uint32_t rb, uint32_t t, uint32_t tile) {
   Workspace& w = c.ws;
-  if (w.base && n <= w.n_cap && rb == w.rb && t == w.threshold) return GSS_OK;
+  if (w.base && n <= w.n_cap && rb == w.rb && t == w.threshold && tile == w.tile) return GSS_OK;
   GSS_HIP(hipStreamSynchronize(c.stream));
-  const Caps k = caps_for(std::max<uint64_t>(n, w.n_cap), t);
+  const Caps k = caps_for(std::max<uint64_t>(n, w.n_cap), t, tile);
   free_workspace(w);
   const uint64_t ncap = k.ncap, cap_big = k.big, cap_tiles = k.tiles, cap_retire = k.retire,
                  cap_bounds = k.bounds;
@@ -333,6 +335,7 @@ int ensure_workspace(Context& c, uint64_t n, uint32_t rb, uint32_t t) {
   w.n_cap = ncap;
   w.rb = rb;
   w.threshold = t;
+  w.tile = tile;
   p.cap_big = (uint32_t)cap_big;
   p.cap_tiles = (uint32_t)cap_tiles;
   p.cap_retire = (uint32_t)cap_retire;
@@ -411,7 +414,8 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   }
   const uint32_t rb = tun.radix_bits;
   const uint32_t T = tun.local_threshold;
-  int rc = ensure_workspace(c, n, rb, T);
+  const uint32_t tile = tun.tile;
+  int rc = ensure_workspace(c, n, rb, T, tile);
   if (rc) return rc;
   rc = ensure_init_lists(c.ws, nreq);
   if (rc) return rc;
@@ -422,6 +426,8 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   p.out = out;
   p.lcp = lcp;
   p.local_threshold = T;
+  p.tile = tile;
+  p.key_bytes = tun.key_bytes;
   p.idx[0] = const_cast<uint32_t*>(in);
   if (out == in) {
     GSS_HIP(hipMemcpyAsync(w.idx_copy, in, 4 * n, hipMemcpyDeviceToDevice, st));
@@ -453,9 +459,9 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     } else {
       if (nbig >= p.cap_big) return fail(GSS_E_INTERNAL, "segment table overflow");
       const uint32_t first = tiles0;
-      for (uint32_t o = 0; o < size; o += kTile) {
+      for (uint32_t o = 0; o < size; o += tile) {
         if (tiles0 >= p.cap_tiles) return fail(GSS_E_INTERNAL, "tile table overflow");
-        tiles[tiles0] = Tile{nbig, b + o, std::min<uint32_t>(kTile, size - o), tiles0 - first};
+        tiles[tiles0] = Tile{nbig, b + o, std::min<uint32_t>(tile, size - o), tiles0 - first};
         ++tiles0;
       }
       aux0[nbig] = SegAux{0, tiles0 - first, 0, first};
@@ -529,7 +535,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
 
   // Launch bounds of the passes.  The first pass's segments and tiles are
   // known here; a later pass has at most one segment per T + 1 of the first
-  // pass's strings and at most (those strings) / kTile tiles beyond one per
+  // pass's strings and at most (those strings) / tile tiles beyond one per
   // segment.  A pass's kernels are launched with those bounds and their
   // blocks past the device counts exit at once, so passes follow each other
   // with no host hand-over.  An empty block costs about 0.4 ns to dispatch
@@ -538,7 +544,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   // Zipf words) the host reads each later pass's counts after its set-up
   // kernel instead and launches exact grids.
   const uint64_t ub_segs = std::min<uint64_t>(p.cap_big, pass0_strings / ((uint64_t)T + 1) + 1);
-  const uint64_t ub_tiles = std::min<uint64_t>(p.cap_tiles, pass0_strings / kTile + ub_segs + 1);
+  const uint64_t ub_tiles = std::min<uint64_t>(p.cap_tiles, pass0_strings / tile + ub_segs + 1);
   constexpr uint64_t kBoundedTilesMax = 20480;
   const bool read_counts = ub_tiles > kBoundedTilesMax;
   static const char* kPassNames[10] = {"pass0", "pass1", "pass2", "pass3", "pass4",
@@ -809,7 +815,9 @@ int check_tuning(const gss_tuning& t) {
     return fail(GSS_E_INVALID, "local_threshold must lie in [64, " + std::to_string(kLocalCapMax) + "]");
   if (t.radix_bits != 8 && t.radix_bits != 11)
     return fail(GSS_E_INVALID, "radix_bits must be 8 or 11");
-  if (t.tile != kTile) return fail(GSS_E_INVALID, "tile must be 4096");
+  if (t.tile != 2048 && t.tile != 4096) return fail(GSS_E_INVALID, "tile must be 2048 or 4096");
+  if (t.key_bytes != 32 && t.key_bytes != 64)
+    return fail(GSS_E_INVALID, "key_bytes must be 32 or 64");
   return GSS_OK;
 }
 
@@ -821,7 +829,7 @@ gss_tuning current_tuning() {
 size_t workspace_bytes(uint64_t n, const gss_tuning& tun) {
   Plan p{};
   uint32_t* idx_copy = nullptr;
-  return items_bytes(workspace_items(p, &idx_copy, caps_for(n, tun.local_threshold),
+  return items_bytes(workspace_items(p, &idx_copy, caps_for(n, tun.local_threshold, tun.tile),
                                      tun.radix_bits));
 }
 
@@ -832,7 +840,7 @@ int ctx_reserve(gss_ctx* h, uint64_t n, uint32_t nreq, const gss_tuning& tun) {
   int dev = -1;
   GSS_HIP(hipGetDevice(&dev));
   if (dev != c.device) return fail(GSS_E_INVALID, "the context belongs to another device");
-  int rc = ensure_workspace(c, n, tun.radix_bits, tun.local_threshold);
+  int rc = ensure_workspace(c, n, tun.radix_bits, tun.local_threshold, tun.tile);
   if (rc) return rc;
   return nreq ? ensure_init_lists(c.ws, nreq) : GSS_OK;
 }

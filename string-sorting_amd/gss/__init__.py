@@ -30,7 +30,8 @@ class GssError(RuntimeError):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("local_threshold", C.c_uint32), ("radix_bits", C.c_uint32), ("tile", C.c_uint32)]
+    _fields_ = [("local_threshold", C.c_uint32), ("radix_bits", C.c_uint32), ("tile", C.c_uint32),
+                ("key_bytes", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -539,7 +540,7 @@ def get_tuning() -> Tuning:
     return t
 
 
-def set_tuning(local_threshold=None, radix_bits=None, tile=None) -> Tuning:
+def set_tuning(local_threshold=None, radix_bits=None, tile=None, key_bytes=None) -> Tuning:
     t = get_tuning()
     if local_threshold is not None:
         t.local_threshold = local_threshold
@@ -547,6 +548,8 @@ def set_tuning(local_threshold=None, radix_bits=None, tile=None) -> Tuning:
         t.radix_bits = radix_bits
     if tile is not None:
         t.tile = tile
+    if key_bytes is not None:
+        t.key_bytes = key_bytes
     _check(lib().gss_set_tuning(C.byref(t)), "gss_set_tuning")
     return t
 

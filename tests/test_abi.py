@@ -58,10 +58,15 @@ def test_tuning_validation():
             gss.set_tuning(local_threshold=20000)
         with pytest.raises(gss.GssError):
             gss.set_tuning(radix_bits=9)
-        gss.set_tuning(local_threshold=1024, radix_bits=11)
+        with pytest.raises(gss.GssError, match="tile"):
+            gss.set_tuning(tile=8192)
+        with pytest.raises(gss.GssError, match="key_bytes"):
+            gss.set_tuning(key_bytes=16)
+        gss.set_tuning(local_threshold=1024, radix_bits=11, tile=2048, key_bytes=32)
+        assert (gss.get_tuning().tile, gss.get_tuning().key_bytes) == (2048, 32)
         assert gss.get_tuning().radix_bits == 11
     finally:
-        gss.set_tuning(t.local_threshold, t.radix_bits, t.tile)
+        gss.set_tuning(t.local_threshold, t.radix_bits, t.tile, t.key_bytes)
 
 
 def test_device_entry_rejects_host_tensors():

@@ -27,13 +27,16 @@ def test_autotune_picks_a_candidate():
     before = gss.get_tuning()
     r = gss.autotune(text, off, text_bytes=ss.text_bytes)
     log(f"best threshold={r.best.local_threshold} radix={r.best.radix_bits} "
+        f"tile={r.best.tile} key_bytes={r.best.key_bytes} "
         f"{r.best_ms:.3f} ms (current tuning {r.default_ms:.3f} ms), sample {r.sample}")
-    assert r.candidates == 8 and r.sample == ss.n
+    # 4 thresholds x 2 digit widths, then the other tile size, then the other key window
+    assert r.candidates == 10 and r.sample == ss.n
     assert r.best.local_threshold in (1024, 2048, 4096, 6144) and r.best.radix_bits in (8, 11)
+    assert r.best.tile in (2048, 4096) and r.best.key_bytes in (32, 64)
     assert 0 < r.best_ms <= r.default_ms
     after = gss.get_tuning()
-    assert (after.local_threshold, after.radix_bits) == (before.local_threshold,
-                                                         before.radix_bits)
+    key = lambda t: (t.local_threshold, t.radix_bits, t.tile, t.key_bytes)  # noqa: E731
+    assert key(after) == key(before)
     assert gss.autotune_cached() >= 1
 
 

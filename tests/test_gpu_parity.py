@@ -17,7 +17,7 @@ def default_tuning():
     gss.lib()
     t = gss.get_tuning()
     yield
-    gss.set_tuning(t.local_threshold, t.radix_bits, t.tile)
+    gss.set_tuning(t.local_threshold, t.radix_bits, t.tile, t.key_bytes)
 
 
 @pytest.mark.parametrize("config", [0, 1, 2, 3])
@@ -149,13 +149,19 @@ def test_deep_shared_prefix(plen, oracle_mod):
     assert_matches_oracle(ss, off, lcp, oracle_mod)
 
 
-@pytest.mark.parametrize("threshold", [64, 1024, 6144])
-@pytest.mark.parametrize("radix_bits", [8, 11])
-def test_tuning_knobs(threshold, radix_bits, oracle_mod):
-    gss.set_tuning(local_threshold=threshold, radix_bits=radix_bits)
+@pytest.mark.parametrize("threshold,radix_bits,tile,key_bytes",
+                         [(64, 8, 4096, 64), (64, 11, 2048, 32), (1024, 8, 2048, 64),
+                          (1024, 11, 4096, 32), (6144, 8, 4096, 32), (6144, 11, 4096, 64),
+                          (6144, 11, 2048, 64), (6144, 8, 2048, 32)])
+def test_tuning_knobs(threshold, radix_bits, tile, key_bytes, oracle_mod):
+    """Every knob of the tuning (local threshold, digit width, tile size,
+    the largest class's first-round key window) keeps the output exact."""
+    gss.set_tuning(local_threshold=threshold, radix_bits=radix_bits, tile=tile,
+                   key_bytes=key_bytes)
     for config in (0, 1, 3):
         ss = gen.make(config, n=40000, seed=7 + config)
-        log(f"tuning threshold={threshold} radix_bits={radix_bits} workload {ss.name}")
+        log(f"tuning threshold={threshold} radix_bits={radix_bits} tile={tile} "
+            f"key_bytes={key_bytes} workload {ss.name}")
         off, lcp = gpu_sort(ss)
         assert_matches_oracle(ss, off, lcp, oracle_mod)
     ss = _set(_deep_prefix(30000, 40, 3), "deep40")
