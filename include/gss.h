@@ -44,6 +44,32 @@ void gss_msd(unsigned char **strings, size_t n);
  * reference computes this array inside mergesort_lcp but never returns it. */
 void gss_msd_lcp(unsigned char **strings, size_t n, size_t *lcp);
 
+/* How the pointer entry points move the text (no reference counterpart:
+ * the reference sorts in place in host memory).  When the strings lie in
+ * one readable address range of at most max_text_bytes that is not much
+ * larger than their bytes, that range is copied to the device as it is and
+ * the pointers become 32-bit distances into it.  Otherwise the strings are
+ * packed one after another; a set whose packed bytes exceed max_text_bytes
+ * or whose count exceeds max_strings is first split on the host by leading
+ * bytes, MSD fashion (src/msd_ce.cpp:151-165), into pieces that each fit
+ * one device sort, so gss_msd takes any n and any total size (the
+ * reference's size_t n), e.g. a sortstring input of more than 4 GiB.
+ * Strings that overlap (suffix sets) must lie within max_text_bytes.
+ * Defaults (also set by passing 0): max_text_bytes = 2^32-64 (32-bit
+ * offsets), max_strings = 2^31-1; smaller values bound the device memory
+ * one call takes.  Returns GSS_E_INVALID outside [64, 2^32-64] x
+ * [2, 2^31-1]. */
+int gss_set_pointer_limits(uint64_t max_text_bytes, uint64_t max_strings);
+
+/* What the last gss_msd / gss_msd_lcp call of this process did. */
+typedef struct gss_pointer_info {
+	uint64_t uploaded_text_bytes; /* text bytes moved to the device      */
+	uint64_t pieces;              /* device sorts the call was split into */
+	uint32_t packed;              /* 1: packed; 0: the address range copied */
+	uint32_t pad;
+} gss_pointer_info;
+int gss_pointer_last_info(gss_pointer_info *out);
+
 /* ------------------------------------------------------------------------
  * Device-resident entry (no reference counterpart: the reference has no
  * device).  The text and the 32-bit string offsets already sit in device

@@ -486,9 +486,9 @@ hipError_t launch_upload(UploadBatch& b, hipStream_t s);
 hipError_t launch_publish(const void* a, size_t a_bytes, const void* b, const uint32_t* count,
                           uint32_t cap, size_t rec_bytes, void* h_dst, uint64_t* h_flag,
                           uint64_t seq, hipStream_t s);
-hipError_t launch_off_to_orig_ptr(const uint32_t* off, const uint32_t* starts,
-                                  const uint64_t* orig, uint64_t* ptrs, uint64_t n,
-                                  hipStream_t s);
+// idx[i] = the position of off[i] in the ascending array starts[0..n).
+hipError_t launch_off_to_index(const uint32_t* off, const uint32_t* starts, uint32_t* idx,
+                              uint64_t n, hipStream_t s);
 
 // 2-way LCP merge (pipeline.hip).  merge_splits(total) is the number of
 // merge-path split entries a merge of `total` strings uses; merge_launch

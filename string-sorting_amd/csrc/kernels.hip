@@ -55,10 +55,9 @@ namespace gss {
 
 // Packed texts: offsets were assigned in input order, so the input index of
 // a sorted offset is found by binary search in the ascending start array.
-__global__ void k_off_to_orig_ptr(const uint32_t* __restrict__ off,
-                                  const uint32_t* __restrict__ starts,
-                                  const uint64_t* __restrict__ orig,
-                                  uint64_t* __restrict__ ptrs, uint64_t n) {
+__global__ void k_off_to_index(const uint32_t* __restrict__ off,
+                               const uint32_t* __restrict__ starts, uint32_t* __restrict__ idx,
+                               uint64_t n) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const uint32_t o = off[i];
@@ -67,7 +66,7 @@ __global__ void k_off_to_orig_ptr(const uint32_t* __restrict__ off,
       const uint64_t mid = (lo + hi) >> 1;
       if (starts[mid] < o) lo = mid + 1; else hi = mid;
     }
-    ptrs[i] = orig[lo];
+    idx[i] = (uint32_t)lo;
   }
 }
 
@@ -1147,10 +1146,9 @@ hipError_t launch_retire(const Plan& p, uint32_t grid, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_off_to_orig_ptr(const uint32_t* off, const uint32_t* starts,
-                                  const uint64_t* orig, uint64_t* ptrs, uint64_t n,
-                                  hipStream_t s) {
-  k_off_to_orig_ptr<<<grid_for(n, 256), 256, 0, s>>>(off, starts, orig, ptrs, n);
+hipError_t launch_off_to_index(const uint32_t* off, const uint32_t* starts, uint32_t* idx,
+                              uint64_t n, hipStream_t s) {
+  k_off_to_index<<<grid_for(n, 256), 256, 0, s>>>(off, starts, idx, n);
   return hipGetLastError();
 }
 

@@ -53,6 +53,11 @@ class AutotuneResult(C.Structure):
                 ("sample", C.c_uint64), ("signature", C.c_uint32), ("candidates", C.c_uint32)]
 
 
+class PointerInfo(C.Structure):
+    _fields_ = [("uploaded_text_bytes", C.c_uint64), ("pieces", C.c_uint64),
+                ("packed", C.c_uint32), ("pad", C.c_uint32)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("slots", C.c_uint32),
                 ("max_batch_requests", C.c_uint32)]
@@ -77,6 +82,8 @@ class ServerStats(C.Structure):
 _SIGNATURES = {
     "gss_msd": (None, [C.POINTER(C.c_void_p), C.c_size_t]),
     "gss_msd_lcp": (None, [C.POINTER(C.c_void_p), C.c_size_t, C.POINTER(C.c_size_t)]),
+    "gss_set_pointer_limits": (C.c_int, [C.c_uint64, C.c_uint64]),
+    "gss_pointer_last_info": (C.c_int, [C.POINTER(PointerInfo)]),
     "gss_sort_device32": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_uint64, C.c_void_p]),
     "gss_sort_batch_device32": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
@@ -177,6 +184,21 @@ def sort_pointers(ptrs, n: int, lcp: bool = False):
         return list(out[: max(0, n - 1)])
     lib().gss_msd(C.cast(ptrs, C.POINTER(C.c_void_p)), n)
     return None
+
+
+def set_pointer_limits(max_text_bytes: int = 0, max_strings: int = 0) -> None:
+    """Largest device sort of the pointer entry points (0: the default,
+    2^32-64 text bytes and 2^31-1 strings); larger inputs are split on the
+    host by leading bytes first."""
+    _check(lib().gss_set_pointer_limits(max_text_bytes, max_strings), "gss_set_pointer_limits")
+
+
+def pointer_info() -> PointerInfo:
+    """What the last gss_msd / gss_msd_lcp call did (bytes uploaded, device
+    sorts, packed or address range copied)."""
+    info = PointerInfo()
+    _check(lib().gss_pointer_last_info(C.byref(info)), "gss_pointer_last_info")
+    return info
 
 
 def sort_strings(strs, lcp: bool = False):

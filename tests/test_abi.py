@@ -69,6 +69,21 @@ def test_tuning_validation():
         gss.set_tuning(t.local_threshold, t.radix_bits, t.tile, t.key_bytes)
 
 
+def test_pointer_limits_validation():
+    with pytest.raises(gss.GssError, match="max_text_bytes"):
+        gss.set_pointer_limits(max_text_bytes=63)
+    with pytest.raises(gss.GssError, match="max_text_bytes"):
+        gss.set_pointer_limits(max_text_bytes=1 << 32)
+    with pytest.raises(gss.GssError, match="max_strings"):
+        gss.set_pointer_limits(max_strings=1)
+    with pytest.raises(gss.GssError, match="max_strings"):
+        gss.set_pointer_limits(max_strings=1 << 31)
+    gss.set_pointer_limits(4096, 100)
+    gss.set_pointer_limits()  # back to the defaults
+    info = gss.pointer_info()  # nothing sorted yet in this process: all zero
+    assert (info.uploaded_text_bytes, info.pieces, info.packed) == (0, 0, 0)
+
+
 def test_device_entry_rejects_host_tensors():
     torch = pytest.importorskip("torch")
     text = torch.zeros(16, dtype=torch.uint8)

@@ -123,6 +123,133 @@ def test_pointer_packed_path(oracle_mod):
         libc.munmap(addr + 2 * size, size)
 
 
+def _mmap_anon(size):
+    libc = C.CDLL(None, use_errno=True)
+    libc.mmap.restype = C.c_void_p
+    libc.mmap.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_long]
+    libc.munmap.argtypes = [C.c_void_p, C.c_size_t]
+    addr = libc.mmap(None, size, mmap.PROT_READ | mmap.PROT_WRITE,
+                     mmap.MAP_PRIVATE | mmap.MAP_ANONYMOUS, -1, 0)
+    assert addr not in (None, C.c_void_p(-1).value)
+    return libc, addr
+
+
+def _lcps(strs):
+    return [len(os.path.commonprefix([a, b])) for a, b in zip(strs[:-1], strs[1:])]
+
+
+def test_pointer_sparse_span_is_packed():
+    """A few thousand strings spread over one 512 MiB mapping: the span is
+    readable and under 4 GiB but mostly empty, so the strings are packed and
+    only their own bytes cross to the device."""
+    size = 512 << 20
+    libc, addr = _mmap_anon(size)
+    try:
+        rng = np.random.default_rng(5)
+        slots = np.sort(rng.choice(size // 4096 - 1, size=3000, replace=False))
+        strs, ptrs = [], []
+        for k, slot in enumerate(slots):
+            s = bytes(rng.integers(97, 100, size=int(rng.integers(0, 20))).astype(np.uint8))
+            pos = addr + int(slot) * 4096 + int(rng.integers(0, 2048))
+            C.memmove(pos, s + b"\0", len(s) + 1)
+            strs.append(s)
+            ptrs.append(pos)
+        order = rng.permutation(len(ptrs))
+        arr = (C.c_void_p * len(ptrs))(*[ptrs[i] for i in order])
+        lcp = gss.sort_pointers(arr, len(ptrs), lcp=True)
+        info = gss.pointer_info()
+        log(f"sparse span: packed={info.packed} uploaded={info.uploaded_text_bytes} "
+            f"pieces={info.pieces}")
+        assert info.packed == 1 and info.pieces == 1
+        assert info.uploaded_text_bytes == sum(len(s) + 1 for s in strs)
+        got = [C.string_at(p) for p in arr]
+        assert got == sorted(strs)
+        # stable: equal strings keep their input order
+        assert list(arr) == [ptrs[i] for i in sorted(order, key=lambda i: strs[i])]
+        assert lcp == _lcps(got)
+    finally:
+        libc.munmap(addr, size)
+
+
+def _deep_dups(n, seed):
+    """Strings with a 300-byte shared prefix, exact duplicates, strings that
+    end exactly where others go on, and an empty string."""
+    rng = np.random.default_rng(seed)
+    pre = b"p" * 300
+    out = [pre + bytes(rng.integers(97, 100, size=int(rng.integers(0, 6))).astype(np.uint8))
+           for _ in range(n)]
+    out += out[: n // 4] + [pre, pre[:150], b""]
+    return [out[i] for i in rng.permutation(len(out))]
+
+
+@pytest.mark.parametrize("case", ["url_like", "zipf_words", "deep_dups"])
+@pytest.mark.parametrize("limits", [(4096, 1 << 20), (1 << 20, 64), (2048, 32)])
+def test_pointer_pieces(case, limits, oracle_mod):
+    """With small device-sort limits the pointer path splits the input on
+    the host by leading bytes into many device sorts: the output pointers
+    and LCPs equal msd_CE2's order and the oracle's LCPs, as one device sort
+    gives them (the same code path takes a sortstring input of over 4 GiB
+    under the default limits)."""
+    if case == "deep_dups":
+        strs = _deep_dups(3000, 11)
+        ss = gen.from_strings(strs, "deep_dups")
+    else:
+        ss = gen.make(1 if case == "url_like" else 3, n=20000, seed=21)
+    base = ss.text.ctypes.data
+    ptrs = (np.uint64(base) + ss.offsets.astype(np.uint64)).astype(np.uint64)
+    lcp = np.zeros(ss.n - 1, dtype=np.uint64)
+    gss.set_pointer_limits(*limits)
+    try:
+        log(f"pointer pieces {case} n={ss.n} limits={limits}")
+        gss.lib().gss_msd_lcp(ptrs.ctypes.data_as(C.POINTER(C.c_void_p)), ss.n,
+                              lcp.ctypes.data_as(C.POINTER(C.c_size_t)))
+        info = gss.pointer_info()
+    finally:
+        gss.set_pointer_limits()
+    log(f"  pieces={info.pieces} packed={info.packed} uploaded={info.uploaded_text_bytes}")
+    assert info.packed == 1 and info.pieces > 1
+    got = (ptrs - np.uint64(base)).astype(np.uint32)
+    want, want_lcp = oracle_mod.sort_offsets(ss.text, ss.offsets, lcp=True)
+    assert np.array_equal(got, want)
+    assert np.array_equal(lcp, want_lcp.astype(np.uint64))
+
+
+def test_pointer_text_over_4gib():
+    """1,120,000 lines of 3,900 bytes in one 4.37 GB buffer (sortstring's
+    layout for a file of over 4 GiB): the pointer path splits it into
+    device sorts under 4 GiB each, and the order and LCPs equal what the
+    strings' first 16 bytes fix (the other 3,884 bytes are one constant
+    tail, and 1% of the lines repeat another line whole)."""
+    n, width = 1_120_000, 3901
+    rng = np.random.default_rng(44)
+    text = np.empty((n, width), dtype=np.uint8)
+    text[:, 16:] = ord("t")
+    text[:, -1] = 0
+    head = rng.integers(97, 123, size=(n, 16), dtype=np.uint8)
+    dup = rng.choice(n, size=n // 100, replace=False)
+    head[dup] = head[rng.choice(n, size=n // 100)]
+    text[:, :16] = head
+    flat = text.reshape(-1)
+    assert flat.size > (1 << 32)
+    base = flat.ctypes.data
+    ptrs = np.uint64(base) + np.arange(n, dtype=np.uint64) * np.uint64(width)
+    lcp = np.zeros(n - 1, dtype=np.uint64)
+    log(f"pointer sort over 4 GiB: {flat.size} bytes, {n} lines")
+    gss.lib().gss_msd_lcp(ptrs.ctypes.data_as(C.POINTER(C.c_void_p)), n,
+                          lcp.ctypes.data_as(C.POINTER(C.c_size_t)))
+    info = gss.pointer_info()
+    log(f"  pieces={info.pieces} uploaded={info.uploaded_text_bytes}")
+    assert info.packed == 1 and info.pieces >= 2 and info.uploaded_text_bytes == flat.size
+    be = head.view(">u8")  # (n, 2) big-endian words: their order is the strings' order
+    want = np.lexsort((np.arange(n), be[:, 1], be[:, 0]))
+    got = ((ptrs - np.uint64(base)) // np.uint64(width)).astype(np.int64)
+    assert np.array_equal(got, want)
+    h = head[want]
+    diff = h[1:] != h[:-1]
+    first = np.where(diff.any(axis=1), diff.argmax(axis=1), width - 1)
+    assert np.array_equal(lcp, first.astype(np.uint64))
+
+
 @pytest.mark.skipif(not (REF / "unit-test").exists(), reason="oracle/_ref not built")
 def test_reference_unit_test():
     """The reference's own unit test (unit-test/main.cpp), unchanged, with the
