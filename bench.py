@@ -61,9 +61,11 @@ def log(msg):
 
 
 # Library profile names -> device symbol fragments in the rocprofv3 tables.
+# (The local kernels' last template argument is the first round's key
+# words, so the fragments stop before it.)
 SYMBOLS = {"gather": "k_gather_minmax", "histogram": "k_histogram", "scatter": "k_scatter",
-           "segscan": "k_segscan", "local128": "k_local<64, 128, false>",
-           "local1024": "k_local<256, 1024, false>", "local6144": "k_local<1024, 6144, false>"}
+           "segscan": "k_segscan", "local128": "k_local<64, 128, false,",
+           "local1024": "k_local<256, 1024, false,", "local6144": "k_local<1024, 6144, false,"}
 PROFILE = ROOT / "profiles" / "url_like_10M_kernels.md"
 
 

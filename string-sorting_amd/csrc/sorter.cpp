@@ -646,10 +646,13 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
         if ((rc = run_pass(&ran))) return rc;
       if ((rc = read_back(true))) return rc;
     } else {
-      // A hint from a sort of another size would queue empty passes (or
-      // too few): then one pass at a time.
-      const bool similar = c.hint_strings && pass0_strings <= 2 * c.hint_strings &&
-                           c.hint_strings <= 2 * pass0_strings;
+      // A hint from a single sort of another size would queue empty passes
+      // (or too few): then one pass at a time.  The batches of a served
+      // loop's slot mix many requests, and their pass counts vary little
+      // with their sizes: they always take the last batch's count.
+      const bool similar =
+          c.hint_strings && (nreq > 1 || (pass0_strings <= 2 * c.hint_strings &&
+                                          c.hint_strings <= 2 * pass0_strings));
       uint32_t queue = similar ? std::max<uint32_t>(1, c.pass_hint) : 1u;
       while (queue) {
         for (uint32_t k = 0; k < queue; ++k)
