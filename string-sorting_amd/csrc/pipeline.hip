@@ -351,7 +351,7 @@ __global__ __launch_bounds__(256) void k_write(const uint64_t* __restrict__ t64,
 // merge of two stable sorts is the stable sort of A followed by B.
 
 constexpr int kMergeThreads = 256;
-constexpr int kMergeItems = 16;
+constexpr int kMergeItems = 8;
 constexpr uint64_t kMergeTile = (uint64_t)kMergeThreads * kMergeItems;
 
 struct StrCmp {
@@ -360,13 +360,20 @@ struct StrCmp {
 };
 
 // Compares the strings at byte offsets x and y, known equal on their first
-// h bytes, 8 bytes at a time (the key packing of multikey_cache).
+// h bytes, as 8-byte keys (the key packing of multikey_cache), four keys of
+// each string per step: the loads of a step are issued together, so a
+// comparison waits for one memory round trip per 32 bytes.
 __device__ __forceinline__ StrCmp str_compare(const uint64_t* __restrict__ t64, uint64_t tlen,
                                               uint64_t x, uint64_t y, uint64_t h) {
-  for (uint64_t d = h;; d += 8) {
-    const uint64_t kx = load_key(t64, tlen, x + d), ky = load_key(t64, tlen, y + d);
-    if (kx != ky) return StrCmp{kx < ky ? -1 : 1, d + key_lcp(kx, ky)};
-    if ((kx & 0xFFu) == 0) return StrCmp{0, d + key_len(kx)};
+  for (uint64_t d = h;; d += 32) {
+    uint64_t kx[4], ky[4];
+    load_keys<4>(t64, tlen, x + d, kx);
+    load_keys<4>(t64, tlen, y + d, ky);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (kx[q] != ky[q]) return StrCmp{kx[q] < ky[q] ? -1 : 1, d + 8 * q + key_lcp(kx[q], ky[q])};
+      if ((kx[q] & 0xFFu) == 0) return StrCmp{0, d + 8 * q + key_len(kx[q])};
+    }
   }
 }
 
@@ -407,50 +414,169 @@ __global__ __launch_bounds__(256) void k_merge_partition(const uint64_t* __restr
 // ha > hb means A[i] < B[j] (and lcp(A[i], B[j]) = hb), ha < hb the
 // reverse; only equal values need a comparison, starting at that depth.
 //
-// The block's outputs are A[i0, i1) and B[j0, j1); in sorted order any two
-// of them share D = lcp(first, last) bytes.  Their offsets and their 8-byte
-// keys at depth D are staged in LDS, so the per-thread merge-path search
-// and the merge's comparisons are decided by the keys; the text is read
-// only past depth D + 8, for strings equal on those 8 bytes.
+// Comparisons inside a block go through a pivot P, the block's first A
+// string (its first B string when it has none).  Every staged string e
+// keeps s_e, its side of P (-1, 0 for equal, +1), d_e = lcp(e, P), and its
+// 8-byte key at depth d_e.  Two strings on different sides order by side
+// and share min(d_e, d_f) bytes; on the same side the one that leaves P
+// first (smaller d) is the farther from P; at the same d their keys at
+// that depth decide, and only keys equal without a NUL send the comparison
+// to the text, from d + 8.  The relations come from the runs' LCP arrays:
+// an A string's d is the minimum of A's LCPs from P to it (A is sorted, so
+// it lies on P's +1 side or equals P); a B string's follows from its
+// prefix-minimum LCP y with the block's first B string B0 and from
+// (z, c) = lcp and order of P and B0: y > z puts it where B0 is, at z;
+// y < z puts it above P at y; y = z compares its key at z with P's.
 __global__ __launch_bounds__(kMergeThreads) void k_merge(
     const uint64_t* __restrict__ t64, uint64_t tlen, const uint32_t* __restrict__ a,
     const uint64_t* __restrict__ alcp, uint64_t na, const uint32_t* __restrict__ b,
     const uint64_t* __restrict__ blcp, uint64_t nb, const uint64_t* __restrict__ split,
     uint32_t* __restrict__ out, uint64_t* __restrict__ olcp) {
-  __shared__ uint64_t block_depth;
-  __shared__ uint64_t s_key[kMergeTile];  // A's keys at [0, ia), then B's
+  constexpr uint32_t kInf = 0xFFFFFFFFu;
+  constexpr int W = kMergeThreads / kWave;
+  __shared__ uint64_t s_key[kMergeTile];  // A's strings at [0, ia), then B's
   __shared__ uint32_t s_off[kMergeTile];
+  __shared__ uint32_t s_dep[kMergeTile];
+  __shared__ uint32_t s_prev[kMergeTile];  // LCP with the previous string of its run
+  __shared__ int8_t s_side[kMergeTile];
+  __shared__ uint32_t s_wmin[2][W];
+  __shared__ uint64_t s_pz[3];  // z, B0's side of P, P's key at depth z
   const uint64_t total = na + nb;
   const uint64_t k0 = (uint64_t)blockIdx.x * kMergeTile;
   const uint64_t k1 = k0 + kMergeTile < total ? k0 + kMergeTile : total;
   const uint64_t i0 = split[blockIdx.x], i1 = split[blockIdx.x + 1];
   const uint64_t j0 = k0 - i0, j1 = k1 - i1;
   const uint32_t ia = (uint32_t)(i1 - i0), jb = (uint32_t)(j1 - j0);  // ia + jb = k1 - k0
-  if (threadIdx.x == 0) {
-    uint64_t first, last;
-    if (i0 == i1) first = b[j0];
-    else if (j0 == j1) first = a[i0];
-    else first = str_compare(t64, tlen, a[i0], b[j0], 0).sign <= 0 ? a[i0] : b[j0];
-    if (i0 == i1) last = b[j1 - 1];
-    else if (j0 == j1) last = a[i1 - 1];
-    else last = str_compare(t64, tlen, a[i1 - 1], b[j1 - 1], 0).sign <= 0 ? b[j1 - 1] : a[i1 - 1];
-    block_depth = str_compare(t64, tlen, first, last, 0).lcp;
+  const uint32_t m = ia + jb;
+  const uint32_t pivot = ia ? a[i0] : b[j0];
+  const uint32_t wid = threadIdx.x / kWave, lane = lane_id();
+
+  // P against B0 (one text comparison per block, overlapping the scan).
+  if (threadIdx.x == 0 && ia && jb) {
+    const StrCmp c = str_compare(t64, tlen, pivot, b[j0], 0);
+    s_pz[0] = c.lcp;
+    s_pz[1] = (uint64_t)(int64_t)(-c.sign);
+    s_pz[2] = load_key(t64, tlen, (uint64_t)pivot + c.lcp);
+  }
+  // Blocked items: thread t holds items [t * kMergeItems, (t + 1) *
+  // kMergeItems) of [A part | B part].  v = the LCP with the previous item
+  // of the same run (kInf at each part's first), then the prefix minimum of
+  // v inside each part.
+  uint32_t v[kMergeItems], o[kMergeItems];
+  const uint32_t x0 = threadIdx.x * kMergeItems;
+  uint32_t run_a = kInf, run_b = kInf;  // running minima of this thread
+#pragma unroll
+  for (int q = 0; q < kMergeItems; ++q) {
+    const uint32_t x = x0 + q;
+    v[q] = kInf;
+    o[q] = 0;
+    if (x < ia) {
+      o[q] = a[i0 + x];
+      if (x) v[q] = (uint32_t)alcp[i0 + x - 1];
+      run_a = min(run_a, v[q]);
+    } else if (x < m) {
+      o[q] = b[j0 + (x - ia)];
+      if (x > ia) v[q] = (uint32_t)blcp[j0 + (x - ia) - 1];
+      run_b = min(run_b, v[q]);
+    }
+  }
+  // Exclusive prefix minima of the threads' running minima, per part.
+  {
+    uint32_t ea = run_a, eb = run_b;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const uint32_t ta = __shfl_up(ea, off, kWave), tb = __shfl_up(eb, off, kWave);
+      if ((int)lane >= off) {
+        ea = min(ea, ta);
+        eb = min(eb, tb);
+      }
+    }
+    if (lane == kWave - 1) {
+      s_wmin[0][wid] = ea;
+      s_wmin[1][wid] = eb;
+    }
+    __syncthreads();
+    uint32_t pa = __shfl_up(ea, 1, kWave), pb = __shfl_up(eb, 1, kWave);
+    if (lane == 0) pa = pb = kInf;
+    for (uint32_t w = 0; w < wid; ++w) {
+      pa = min(pa, s_wmin[0][w]);
+      pb = min(pb, s_wmin[1][w]);
+    }
+    run_a = pa;
+    run_b = pb;
+  }
+  const uint32_t z = ia && jb ? (uint32_t)s_pz[0] : 0;
+  const int b0_side = ia && jb ? (int)(int64_t)s_pz[1] : 0;
+  const uint64_t pkey = ia && jb ? s_pz[2] : 0;
+#pragma unroll
+  for (int q = 0; q < kMergeItems; ++q) {
+    const uint32_t x = x0 + q;
+    if (x >= m) continue;
+    int side;
+    uint32_t d;
+    const bool own = x < ia || ia == 0;  // a string of the pivot's own run
+    if (x < ia)
+      run_a = min(run_a, v[q]);
+    else
+      run_b = min(run_b, v[q]);
+    if (own) {
+      d = x < ia ? run_a : run_b;
+      side = x == 0 ? 0 : 1;
+    } else {  // a B string, the pivot being A's first
+      const uint32_t y = run_b;
+      if (y > z) {
+        d = z;
+        side = b0_side;
+      } else if (y < z) {
+        d = y;
+        side = 1;
+      } else {
+        const uint64_t kb = load_key(t64, tlen, (uint64_t)o[q] + z);
+        if (kb != pkey) {
+          side = kb > pkey ? 1 : -1;
+          d = z + key_lcp(kb, pkey);
+        } else if ((kb & 0xFFu) == 0) {
+          side = 0;
+          d = z + key_len(kb);
+        } else {
+          const StrCmp c = str_compare(t64, tlen, pivot, o[q], (uint64_t)z + 8);
+          side = -c.sign;
+          d = (uint32_t)c.lcp;
+        }
+      }
+    }
+    uint64_t key = 0;
+    if (d != kInf) {
+      key = load_key(t64, tlen, (uint64_t)o[q] + d);
+      // A string of the pivot's run that ends where it leaves P equals P.
+      if (own && x != 0 && (key >> 56) == 0) side = 0;
+    }
+    s_off[x] = o[q];
+    s_prev[x] = v[q];
+    s_dep[x] = d;
+    s_key[x] = key;
+    s_side[x] = (int8_t)side;
   }
   __syncthreads();
-  const uint64_t depth = block_depth;
-  for (uint32_t x = threadIdx.x; x < ia + jb; x += kMergeThreads) {
-    const uint32_t o = x < ia ? a[i0 + x] : b[j0 + (x - ia)];
-    s_off[x] = o;
-    s_key[x] = load_key(t64, tlen, (uint64_t)o + depth);
-  }
-  __syncthreads();
-  // A[i] against B[j] for strings of the block (sharing `depth` bytes), as
-  // str_compare: the staged keys decide unless they are equal without a NUL.
-  auto cmp_block = [&](uint64_t i, uint64_t j) -> StrCmp {
-    const uint64_t ka = s_key[i - i0], kb = s_key[ia + (j - j0)];
-    if (ka != kb) return StrCmp{ka < kb ? -1 : 1, depth + key_lcp(ka, kb)};
-    if ((ka & 0xFFu) == 0) return StrCmp{0, depth + key_len(ka)};
-    return str_compare(t64, tlen, s_off[i - i0], s_off[ia + (j - j0)], depth + 8);
+
+  // A[i] against B[j], both staged, through the pivot.  `known` (bytes the
+  // two are known to share) only moves the text comparison's start.
+  auto cmp_block = [&](uint64_t i, uint64_t j, uint64_t known) -> StrCmp {
+    const uint32_t e = (uint32_t)(i - i0), f = ia + (uint32_t)(j - j0);
+    const int se = s_side[e], sf = s_side[f];
+    const uint32_t de = s_dep[e], df = s_dep[f];
+    const uint32_t lo = min(de, df);
+    if (se != sf) return StrCmp{se < sf ? -1 : 1, lo};
+    if (se == 0) return StrCmp{0, lo};  // both equal P (P itself has d = kInf)
+    if (de != df) {
+      const bool e_less = se > 0 ? de > df : de < df;
+      return StrCmp{e_less ? -1 : 1, lo};
+    }
+    const uint64_t ke = s_key[e], kf = s_key[f];
+    if (ke != kf) return StrCmp{ke < kf ? -1 : 1, de + key_lcp(ke, kf)};
+    if ((ke & 0xFFu) == 0) return StrCmp{0, de + key_len(ke)};
+    const uint64_t from = (uint64_t)de + 8;
+    return str_compare(t64, tlen, s_off[e], s_off[f], known > from ? known : from);
   };
   const uint64_t d = k0 + (uint64_t)threadIdx.x * kMergeItems;
   if (d >= k1) return;
@@ -460,7 +586,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
   uint64_t hi = d - j0 < i1 ? d - j0 : i1;
   while (lo < hi) {
     const uint64_t mid = (lo + hi) >> 1;
-    if (cmp_block(mid, d - 1 - mid).sign <= 0)
+    if (cmp_block(mid, d - 1 - mid, 0).sign <= 0)
       lo = mid + 1;
     else
       hi = mid;
@@ -483,29 +609,30 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
       take_a = true;
       lprev = ha;
     } else if (!have_s) {
-      const StrCmp c = inside ? cmp_block(i, j) : str_compare(t64, tlen, a[i], b[j], 0);
+      const StrCmp c = inside ? cmp_block(i, j, 0) : str_compare(t64, tlen, a[i], b[j], 0);
       take_a = c.sign <= 0;
       if (take_a) hb = c.lcp; else ha = c.lcp;
     } else if (ha != hb) {
       take_a = ha > hb;
       lprev = take_a ? ha : hb;
     } else {
-      // Both share ha >= depth bytes with the last output, hence with each
-      // other: the keys at `depth` decide as a comparison from ha would.
-      const StrCmp c = inside ? cmp_block(i, j) : str_compare(t64, tlen, a[i], b[j], ha);
+      // Both share ha bytes with the last output, hence with each other.
+      const StrCmp c = inside ? cmp_block(i, j, ha) : str_compare(t64, tlen, a[i], b[j], ha);
       take_a = c.sign <= 0;
       lprev = ha;
       if (take_a) hb = c.lcp; else ha = c.lcp;
     }
     if (have_s && olcp) olcp[k - 1] = lprev;
     if (k == end) break;  // the extra step: only the LCP above
+    // The next head's LCP with this output: staged inside the block (the
+    // merge's chain of decisions then waits on no global load).
     if (take_a) {
       out[k] = i < i1 ? s_off[i - i0] : a[i];
-      ha = i + 1 < na ? alcp[i] : 0;
+      ha = i + 1 < i1 ? s_prev[i + 1 - i0] : (i + 1 < na ? alcp[i] : 0);
       ++i;
     } else {
       out[k] = j < j1 ? s_off[ia + (j - j0)] : b[j];
-      hb = j + 1 < nb ? blcp[j] : 0;
+      hb = j + 1 < j1 ? s_prev[ia + (j + 1 - j0)] : (j + 1 < nb ? blcp[j] : 0);
       ++j;
     }
     have_s = true;
@@ -663,6 +790,8 @@ int merge_check(const unsigned char* d_text, uint64_t text_bytes, const uint32_t
   if (reinterpret_cast<uintptr_t>(d_text) & 7)
     return fail(GSS_E_INVALID, "the device text must be 8-byte aligned");
   if (total >= (1ull << 32)) return fail(GSS_E_TOO_LARGE, "at most 2^32-1 strings");
+  if (text_bytes >= (1ull << 32))
+    return fail(GSS_E_TOO_LARGE, "text of 4 GiB or more: offsets and LCPs are 32-bit here");
   return GSS_OK;
 }
 
@@ -711,6 +840,8 @@ int gss_merge_lcp_kway_device32(const unsigned char* d_text, uint64_t text_bytes
   if (reinterpret_cast<uintptr_t>(d_text) & 7)
     return fail(GSS_E_INVALID, "the device text must be 8-byte aligned");
   if (total >= (1ull << 32)) return fail(GSS_E_TOO_LARGE, "at most 2^32-1 strings");
+  if (text_bytes >= (1ull << 32))
+    return fail(GSS_E_TOO_LARGE, "text of 4 GiB or more: offsets and LCPs are 32-bit here");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   // A run in the current round: offsets, LCPs (len - 1 of them), length.
   struct Run {

@@ -194,6 +194,45 @@ def test_merge_lcp_matches_oracle(sizes, oracle_mod):
     assert np.array_equal(got, mo) and np.array_equal(ol.cpu().numpy().view(np.uint64), mlo)
 
 
+def _prefix_family(n, seed):
+    """Strings that are prefixes of one another, share 0-300 bytes, repeat
+    exactly, or are empty: the cases where a merge must tell "shorter" from
+    "smaller" and "equal" from "longer"."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        k = int(rng.choice([0, 1, 7, 8, 9, 31, 32, 33, 150, 300]))
+        tail = bytes(rng.integers(97, 99, size=int(rng.integers(0, 4))).astype(np.uint8))
+        out.append(b"p" * k + tail)
+    out += out[: n // 3]
+    return gen.from_strings([out[i] for i in rng.permutation(len(out))], "prefix_family")
+
+
+@pytest.mark.parametrize("k", [2, 3, 8])
+def test_merge_lcp_prefix_family(k, oracle_mod):
+    """Runs of strings that are prefixes of each other and exact duplicates,
+    spread over merge blocks of their own and of mixed runs: the k-way (and,
+    for k = 2, the 2-way) merge equals the stable sort of the concatenation
+    with exact LCPs."""
+    ss = _prefix_family(12_000, 90 + k)
+    text = _dev(ss.text)
+    bounds = np.linspace(0, ss.n, k + 1).astype(np.int64)
+    runs, lcps = [], []
+    for r in range(k):
+        o, lc = _sorted_run(text, ss.offsets[bounds[r]:bounds[r + 1]], ss.text_bytes)
+        runs.append(o)
+        lcps.append(lc[: max(0, o.numel() - 1)])
+    log(f"prefix-family merge k={k} n={ss.n}")
+    want, want_lcp = oracle_mod.sort_offsets(ss.text, ss.offsets, lcp=True)
+    out, ol = gss.merge_lcp_kway(text, runs, lcps, text_bytes=ss.text_bytes)
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), want)
+    assert np.array_equal(ol.cpu().numpy().view(np.uint64), want_lcp)
+    if k == 2:
+        o2, l2 = gss.merge_lcp(text, runs[0], lcps[0], runs[1], lcps[1], text_bytes=ss.text_bytes)
+        assert np.array_equal(o2.cpu().numpy().view(np.uint32), want)
+        assert np.array_equal(l2.cpu().numpy().view(np.uint64), want_lcp)
+
+
 @pytest.mark.parametrize("sizes", [(7_000,), (9_000, 0, 4_000), (3_000, 3_000, 3_000),
                                    (1, 20_000, 0, 5, 2_500), (800,) * 8, (0, 0)])
 def test_merge_lcp_kway_matches_oracle(sizes, oracle_mod):
