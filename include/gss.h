@@ -88,8 +88,10 @@ int gss_pointer_last_info(gss_pointer_info *out);
  *
  * Returns 0 on success, a negative GSS_E* code otherwise (gss_last_error()
  * explains).  The call returns once the result is complete on the stream.
- * The calling thread polls while the passes run (the host decides each
- * pass from the previous one's counters): it busy-waits rather than sleeps.
+ * The global passes are queued back to back (each later pass's tile table
+ * is built on the device); the calling thread then polls a pinned word the
+ * device writes with the counters, before the local phase and at the end:
+ * it busy-waits rather than sleeps.
  */
 int gss_sort_device32(const unsigned char *d_text, uint64_t text_bytes,
 		      const uint32_t *d_off_in, uint32_t *d_off_out,
@@ -243,10 +245,10 @@ int gss_merge_lcp_device32(const unsigned char *d_text, uint64_t text_bytes,
  * reference: it is for callers that cannot allocate or block, such as code
  * under HIP graph capture or a tight serving loop).  Same arguments, same
  * output, bit for bit; but it allocates nothing (its one scratch array, the
- * merge-path splits, is a fixed 1 MiB device array of the library, one per
+ * merge-path splits, is a fixed 2 MiB device array of the library, one per
  * GPU, held until the process ends) and does not synchronize: it queues the
  * merge on `stream` and returns, and the result is complete when the
- * stream's work is.  It takes at most 536,866,816 strings (na + nb), beyond
+ * stream's work is.  It takes at most 536,868,864 strings (na + nb), beyond
  * that it returns GSS_E_TOO_LARGE.  Calls on one GPU share that GPU's scratch
  * array: calls queued in order on one stream are safe, calls whose work
  * overlaps (other streams, or host threads racing on one stream) race. */

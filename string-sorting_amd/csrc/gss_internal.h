@@ -490,12 +490,14 @@ hipError_t launch_publish(const void* a, size_t a_bytes, const void* b, const ui
 hipError_t launch_off_to_index(const uint32_t* off, const uint32_t* starts, uint32_t* idx,
                               uint64_t n, hipStream_t s);
 
-// 2-way LCP merge (pipeline.hip).  merge_splits(total) is the number of
+// 2-way LCP merge (pipeline.hip).  Outputs per merge tile (one workgroup
+// each); merge_splits(total) = ceil(total / kMergeTile) + 1 is the number of
 // merge-path split entries a merge of `total` strings uses; merge_launch
 // queues the partition and merge kernels on `st` (no sync), with `split` a
 // device array of at least merge_splits(na + nb) entries, each written
 // before it is read.  merge_check validates the arguments of a merge entry
 // point (GSS_OK or the GSS_E_* code, with the message set).
+constexpr uint64_t kMergeTile = 2048;
 uint64_t merge_splits(uint64_t total);
 hipError_t merge_launch(const uint64_t* t64, uint64_t tlen, const uint32_t* a,
                         const uint64_t* alcp, uint64_t na, const uint32_t* b,

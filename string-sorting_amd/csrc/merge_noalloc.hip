@@ -25,11 +25,12 @@
 
 namespace gss {
 
-// Split entries of the largest merge this entry accepts: 131,072 entries of
-// 8 bytes (1 MiB per GPU) cover merges of up to 131,071 tiles of 4096
-// outputs, 536,866,816 strings.
-constexpr uint64_t kNoAllocSplits = 131072;
-constexpr uint64_t kNoAllocMaxStrings = (kNoAllocSplits - 1) * 4096;
+// Split entries of the largest merge this entry accepts: 262,144 entries of
+// 8 bytes (2 MiB per GPU) cover merges of up to 262,143 tiles of 2048
+// outputs, 536,868,864 strings.
+constexpr uint64_t kNoAllocSplits = 262144;
+constexpr uint64_t kNoAllocMaxStrings = (kNoAllocSplits - 1) * kMergeTile;
+static_assert(kNoAllocMaxStrings == 536868864ull, "the limit gss.h states");
 
 // The scratch array.  In host code `scratch` is only a stand-in for the
 // symbol: its device address on each GPU comes from hipGetSymbolAddress.

@@ -352,7 +352,7 @@ __global__ __launch_bounds__(256) void k_write(const uint64_t* __restrict__ t64,
 
 constexpr int kMergeThreads = 256;
 constexpr int kMergeItems = 8;
-constexpr uint64_t kMergeTile = (uint64_t)kMergeThreads * kMergeItems;
+static_assert((uint64_t)kMergeThreads * kMergeItems == kMergeTile, "merge tile");
 
 struct StrCmp {
   int sign;      // <0, 0, >0 as strcmp

@@ -192,10 +192,10 @@ def test_noalloc_merge_validates_before_touching_a_device():
 
     lib = gss.lib()
     fake = C.c_void_p(1 << 20)
-    na = nb = 300_000_000  # 6e8 strings > 536,866,816
+    na = nb = 300_000_000  # 6e8 strings > 536,868,864
     assert lib.gss_merge_lcp_device32_noalloc(fake, 4096, fake, fake, na, fake, fake, nb, fake,
                                               None, None) == -4
-    assert "at most 536866816 strings" in gss.last_error()
+    assert "at most 536868864 strings" in gss.last_error()
     assert lib.gss_merge_lcp_device32_noalloc(fake, 0, fake, fake, 5, fake, fake, 5, fake,
                                               None, None) == -1
     assert "empty text" in gss.last_error()
