@@ -596,45 +596,71 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
   const uint64_t end = d + kMergeItems < k1 ? d + kMergeItems : k1;
   bool have_s = false;
   uint64_t ha = 0, hb = 0;
+  // ha / hb hold lcp(s, head) only while these are set: a head passed over
+  // without a comparison (its run's part of the tile is used up) loses it,
+  // and the text gives it again if it is ever needed (at the tile's end).
+  bool ha_ok = false, hb_ok = false;
+  uint32_t s_last = 0;  // offset of the last output
+  auto fix_heads = [&]() {
+    if (!ha_ok && i < na) ha = str_compare(t64, tlen, s_last, a[i], 0).lcp;
+    if (!hb_ok && j < nb) hb = str_compare(t64, tlen, s_last, b[j], 0).lcp;
+    ha_ok = hb_ok = true;
+  };
   for (uint64_t k = d; k < total && k <= end; ++k) {
     bool take_a;
     uint64_t lprev = 0;  // lcp(previous output, this output)
     // Heads past the block's ranges (only on the extra step) are compared
     // on the text.
     const bool inside = i < i1 && j < j1;
+    if (have_s && (i == na || j == nb)) fix_heads();
     if (i == na) {
       take_a = false;
       lprev = hb;
     } else if (j == nb) {
       take_a = true;
       lprev = ha;
+    } else if (k < k1 && (i == i1 || j == j1)) {
+      // The tile's outputs are A[i0, i1) and B[j0, j1): with one part used
+      // up, the next output is the other's head, no comparison needed.  (Its
+      // LCP with the last output is known: that head followed an output of
+      // its own run or won a comparison against it.)
+      take_a = j == j1;
+      lprev = take_a ? ha : hb;
+      if (take_a) hb_ok = false; else ha_ok = false;
     } else if (!have_s) {
       const StrCmp c = inside ? cmp_block(i, j, 0) : str_compare(t64, tlen, a[i], b[j], 0);
       take_a = c.sign <= 0;
       if (take_a) hb = c.lcp; else ha = c.lcp;
-    } else if (ha != hb) {
-      take_a = ha > hb;
-      lprev = take_a ? ha : hb;
+      ha_ok = hb_ok = true;
     } else {
-      // Both share ha bytes with the last output, hence with each other.
-      const StrCmp c = inside ? cmp_block(i, j, ha) : str_compare(t64, tlen, a[i], b[j], ha);
-      take_a = c.sign <= 0;
-      lprev = ha;
-      if (take_a) hb = c.lcp; else ha = c.lcp;
+      fix_heads();
+      if (ha != hb) {
+        take_a = ha > hb;
+        lprev = take_a ? ha : hb;
+      } else {
+        // Both share ha bytes with the last output, hence with each other.
+        const StrCmp c = inside ? cmp_block(i, j, ha) : str_compare(t64, tlen, a[i], b[j], ha);
+        take_a = c.sign <= 0;
+        lprev = ha;
+        if (take_a) hb = c.lcp; else ha = c.lcp;
+      }
     }
     if (have_s && olcp) olcp[k - 1] = lprev;
     if (k == end) break;  // the extra step: only the LCP above
     // The next head's LCP with this output: staged inside the block (the
     // merge's chain of decisions then waits on no global load).
     if (take_a) {
-      out[k] = i < i1 ? s_off[i - i0] : a[i];
+      s_last = i < i1 ? s_off[i - i0] : a[i];
       ha = i + 1 < i1 ? s_prev[i + 1 - i0] : (i + 1 < na ? alcp[i] : 0);
+      ha_ok = true;
       ++i;
     } else {
-      out[k] = j < j1 ? s_off[ia + (j - j0)] : b[j];
+      s_last = j < j1 ? s_off[ia + (j - j0)] : b[j];
       hb = j + 1 < j1 ? s_prev[ia + (j + 1 - j0)] : (j + 1 < nb ? blcp[j] : 0);
+      hb_ok = true;
       ++j;
     }
+    out[k] = s_last;
     have_s = true;
   }
 }
