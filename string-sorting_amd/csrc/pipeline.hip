@@ -414,19 +414,27 @@ __global__ __launch_bounds__(256) void k_merge_partition(const uint64_t* __restr
 // ha > hb means A[i] < B[j] (and lcp(A[i], B[j]) = hb), ha < hb the
 // reverse; only equal values need a comparison, starting at that depth.
 //
-// Comparisons inside a block go through a pivot P, the block's first A
-// string (its first B string when it has none).  Every staged string e
-// keeps s_e, its side of P (-1, 0 for equal, +1), d_e = lcp(e, P), and its
-// 8-byte key at depth d_e.  Two strings on different sides order by side
-// and share min(d_e, d_f) bytes; on the same side the one that leaves P
-// first (smaller d) is the farther from P; at the same d their keys at
-// that depth decide, and only keys equal without a NUL send the comparison
-// to the text, from d + 8.  The relations come from the runs' LCP arrays:
-// an A string's d is the minimum of A's LCPs from P to it (A is sorted, so
-// it lies on P's +1 side or equals P); a B string's follows from its
-// prefix-minimum LCP y with the block's first B string B0 and from
-// (z, c) = lcp and order of P and B0: y > z puts it where B0 is, at z;
-// y < z puts it above P at y; y = z compares its key at z with P's.
+// Comparisons inside a block go through two pivots, P1 and P2, the block's
+// first and last A string (first and last B string when it has none).
+// Every staged string e keeps, for each pivot P, its side of P (-1, 0 for
+// equal, +1) and d = lcp(e, P), and its 8-byte key at the larger of the
+// two d.  Against one pivot, two strings on different sides order by side
+// and share min(d) bytes; on the same side the one that leaves P first
+// (smaller d) is the farther from P; at the same d their keys at that depth
+// decide when both are staged there.  Only a pair that neither pivot
+// decides (same side and depth for both, keys equal without a NUL) reads
+// the text, from the deepest byte the pivots show equal.  A block that
+// spans two groups of strings with long shared prefixes (URL prefix
+// groups) has a pivot in each: every string leaves the pivot of its own
+// group deep, where one pivot in the other group would see the whole group
+// leave it at one depth with one key.  The relations come from the runs'
+// LCP arrays: an A string's d to P1 is the minimum of A's LCPs from P1 to
+// it (A is sorted: it is above P1 or equal to it), its d to P2 the minimum
+// from it to P2 (below P2 or equal); a B string's follow from its LCP y
+// with the block's first (last) B string B0 (Bl), the minimum of B's LCPs
+// between them, and from (z, c), the LCP and order of P1 and B0 (P2 and
+// Bl): y > z puts it where B0 (Bl) is, at depth z; y < z puts it above P1
+// (below P2) at depth y; y = z compares its key at z with the pivot's.
 __global__ __launch_bounds__(kMergeThreads) void k_merge(
     const uint64_t* __restrict__ t64, uint64_t tlen, const uint32_t* __restrict__ a,
     const uint64_t* __restrict__ alcp, uint64_t na, const uint32_t* __restrict__ b,
@@ -434,13 +442,18 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
     uint32_t* __restrict__ out, uint64_t* __restrict__ olcp) {
   constexpr uint32_t kInf = 0xFFFFFFFFu;
   constexpr int W = kMergeThreads / kWave;
+  // Relation bits: side to P1 + 1 (bits 0-1), side to P2 + 1 (bits 2-3),
+  // key staged at d1 (bit 4), at d2 (bit 5).
+  constexpr uint32_t kKeyAtD1 = 16, kKeyAtD2 = 32;
   __shared__ uint64_t s_key[kMergeTile];  // A's strings at [0, ia), then B's
   __shared__ uint32_t s_off[kMergeTile];
-  __shared__ uint32_t s_dep[kMergeTile];
+  __shared__ uint32_t s_d1[kMergeTile];
+  __shared__ uint32_t s_d2[kMergeTile];
   __shared__ uint32_t s_prev[kMergeTile];  // LCP with the previous string of its run
-  __shared__ int8_t s_side[kMergeTile];
-  __shared__ uint32_t s_wmin[2][W];
-  __shared__ uint64_t s_pz[3];  // z, B0's side of P, P's key at depth z
+  __shared__ uint8_t s_rel[kMergeTile];
+  __shared__ uint32_t s_wmin[4][W];
+  __shared__ uint64_t s_pz[6];  // z, B end's side, pivot's key at z: for P1 / B0, P2 / Bl
+  __shared__ uint64_t s_len2;   // length of P2
   const uint64_t total = na + nb;
   const uint64_t k0 = (uint64_t)blockIdx.x * kMergeTile;
   const uint64_t k1 = k0 + kMergeTile < total ? k0 + kMergeTile : total;
@@ -448,23 +461,34 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
   const uint64_t j0 = k0 - i0, j1 = k1 - i1;
   const uint32_t ia = (uint32_t)(i1 - i0), jb = (uint32_t)(j1 - j0);  // ia + jb = k1 - k0
   const uint32_t m = ia + jb;
-  const uint32_t pivot = ia ? a[i0] : b[j0];
+  const uint32_t p1 = ia ? a[i0] : b[j0];
+  const uint32_t p2 = ia ? a[i1 - 1] : b[j1 - 1];
+  const uint32_t own_end = ia ? ia : m;  // the pivots' run is items [0, own_end)
   const uint32_t wid = threadIdx.x / kWave, lane = lane_id();
 
-  // P against B0 (one text comparison per block, overlapping the scan).
-  if (threadIdx.x == 0 && ia && jb) {
-    const StrCmp c = str_compare(t64, tlen, pivot, b[j0], 0);
+  // The pivots against B's ends and P2's length: three text comparisons per
+  // block, by threads of different waves, overlapping the scans.
+  if (ia && jb && threadIdx.x == 0) {
+    const StrCmp c = str_compare(t64, tlen, p1, b[j0], 0);
     s_pz[0] = c.lcp;
     s_pz[1] = (uint64_t)(int64_t)(-c.sign);
-    s_pz[2] = load_key(t64, tlen, (uint64_t)pivot + c.lcp);
+    s_pz[2] = load_key(t64, tlen, (uint64_t)p1 + c.lcp);
   }
+  if (ia && jb && threadIdx.x == kWave) {
+    const StrCmp c = str_compare(t64, tlen, p2, b[j1 - 1], 0);
+    s_pz[3] = c.lcp;
+    s_pz[4] = (uint64_t)(int64_t)(-c.sign);
+    s_pz[5] = load_key(t64, tlen, (uint64_t)p2 + c.lcp);
+  }
+  if (threadIdx.x == 2 * kWave) s_len2 = str_compare(t64, tlen, p2, p2, 0).lcp;
   // Blocked items: thread t holds items [t * kMergeItems, (t + 1) *
   // kMergeItems) of [A part | B part].  v = the LCP with the previous item
-  // of the same run (kInf at each part's first), then the prefix minimum of
-  // v inside each part.
+  // of the same run (kInf at each part's first); each item's LCP with its
+  // part's first item is the prefix minimum of v, with its part's last item
+  // the minimum of v over the items after it.
   uint32_t v[kMergeItems], o[kMergeItems];
   const uint32_t x0 = threadIdx.x * kMergeItems;
-  uint32_t run_a = kInf, run_b = kInf;  // running minima of this thread
+  uint32_t tot_a = kInf, tot_b = kInf;  // minima of this thread's v per part
 #pragma unroll
   for (int q = 0; q < kMergeItems; ++q) {
     const uint32_t x = x0 + q;
@@ -473,110 +497,176 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
     if (x < ia) {
       o[q] = a[i0 + x];
       if (x) v[q] = (uint32_t)alcp[i0 + x - 1];
-      run_a = min(run_a, v[q]);
+      tot_a = min(tot_a, v[q]);
     } else if (x < m) {
       o[q] = b[j0 + (x - ia)];
       if (x > ia) v[q] = (uint32_t)blcp[j0 + (x - ia) - 1];
-      run_b = min(run_b, v[q]);
+      tot_b = min(tot_b, v[q]);
     }
   }
-  // Exclusive prefix minima of the threads' running minima, per part.
+  // Minima of the threads before (pre_*) and after (suf_*) this one, per part.
+  uint32_t pre_a, pre_b, suf_a, suf_b;
   {
-    uint32_t ea = run_a, eb = run_b;
+    uint32_t ea = tot_a, eb = tot_b, fa = tot_a, fb = tot_b;
 #pragma unroll
     for (int off = 1; off < kWave; off <<= 1) {
       const uint32_t ta = __shfl_up(ea, off, kWave), tb = __shfl_up(eb, off, kWave);
+      const uint32_t ua = __shfl_down(fa, off, kWave), ub = __shfl_down(fb, off, kWave);
       if ((int)lane >= off) {
         ea = min(ea, ta);
         eb = min(eb, tb);
+      }
+      if ((int)lane + off < kWave) {
+        fa = min(fa, ua);
+        fb = min(fb, ub);
       }
     }
     if (lane == kWave - 1) {
       s_wmin[0][wid] = ea;
       s_wmin[1][wid] = eb;
     }
-    __syncthreads();
-    uint32_t pa = __shfl_up(ea, 1, kWave), pb = __shfl_up(eb, 1, kWave);
-    if (lane == 0) pa = pb = kInf;
-    for (uint32_t w = 0; w < wid; ++w) {
-      pa = min(pa, s_wmin[0][w]);
-      pb = min(pb, s_wmin[1][w]);
+    if (lane == 0) {
+      s_wmin[2][wid] = fa;
+      s_wmin[3][wid] = fb;
     }
-    run_a = pa;
-    run_b = pb;
+    __syncthreads();
+    pre_a = __shfl_up(ea, 1, kWave);
+    pre_b = __shfl_up(eb, 1, kWave);
+    suf_a = __shfl_down(fa, 1, kWave);
+    suf_b = __shfl_down(fb, 1, kWave);
+    if (lane == 0) pre_a = pre_b = kInf;
+    if (lane == kWave - 1) suf_a = suf_b = kInf;
+    for (uint32_t w = 0; w < (uint32_t)W; ++w) {
+      if (w < wid) {
+        pre_a = min(pre_a, s_wmin[0][w]);
+        pre_b = min(pre_b, s_wmin[1][w]);
+      } else if (w > wid) {
+        suf_a = min(suf_a, s_wmin[2][w]);
+        suf_b = min(suf_b, s_wmin[3][w]);
+      }
+    }
   }
-  const uint32_t z = ia && jb ? (uint32_t)s_pz[0] : 0;
-  const int b0_side = ia && jb ? (int)(int64_t)s_pz[1] : 0;
-  const uint64_t pkey = ia && jb ? s_pz[2] : 0;
+  // LCP of each item with its part's last item (kInf for that item).
+  uint32_t to_last[kMergeItems];
+  {
+    uint32_t ra = suf_a, rb = suf_b;
+#pragma unroll
+    for (int q = kMergeItems - 1; q >= 0; --q) {
+      const uint32_t x = x0 + q;
+      to_last[q] = x < ia ? ra : rb;
+      if (x < ia)
+        ra = min(ra, v[q]);
+      else if (x < m)
+        rb = min(rb, v[q]);
+    }
+  }
+  const uint32_t len2 = (uint32_t)s_len2;
+  const bool cross = ia && jb;
+  const uint32_t z1 = cross ? (uint32_t)s_pz[0] : 0, z2 = cross ? (uint32_t)s_pz[3] : 0;
+  const int c1 = cross ? (int)(int64_t)s_pz[1] : 0, c2 = cross ? (int)(int64_t)s_pz[4] : 0;
+  const uint64_t pk1 = cross ? s_pz[2] : 0, pk2 = cross ? s_pz[5] : 0;
+  // A B string's relation to a pivot P from its LCP y with the B end E next
+  // to P and (z, c, pk) of P against E; `up` when E is B's first string
+  // (the string lies at or above E), else B's last (at or below E).
+  auto via_end = [&](uint32_t off, uint32_t y, uint32_t z, int c, uint64_t pk, uint32_t piv,
+                     bool up, int* side, uint32_t* d) {
+    if (y > z) {
+      *side = c;
+      *d = z;
+    } else if (y < z) {
+      *side = up ? 1 : -1;
+      *d = y;
+    } else {
+      const uint64_t kb = load_key(t64, tlen, (uint64_t)off + z);
+      if (kb != pk) {
+        *side = kb > pk ? 1 : -1;
+        *d = z + key_lcp(kb, pk);
+      } else if ((kb & 0xFFu) == 0) {
+        *side = 0;
+        *d = z + key_len(kb);
+      } else {
+        const StrCmp r = str_compare(t64, tlen, piv, off, (uint64_t)z + 8);
+        *side = -r.sign;
+        *d = (uint32_t)r.lcp;
+      }
+    }
+  };
+  uint32_t run_a = pre_a, run_b = pre_b;
 #pragma unroll
   for (int q = 0; q < kMergeItems; ++q) {
     const uint32_t x = x0 + q;
     if (x >= m) continue;
-    int side;
-    uint32_t d;
-    const bool own = x < ia || ia == 0;  // a string of the pivot's own run
     if (x < ia)
       run_a = min(run_a, v[q]);
     else
       run_b = min(run_b, v[q]);
+    const uint32_t to_first = x < ia ? run_a : run_b;  // kInf at the part's first
+    int s1, s2;
+    uint32_t d1, d2;
+    const bool own = x < own_end;
     if (own) {
-      d = x < ia ? run_a : run_b;
-      side = x == 0 ? 0 : 1;
-    } else {  // a B string, the pivot being A's first
-      const uint32_t y = run_b;
-      if (y > z) {
-        d = z;
-        side = b0_side;
-      } else if (y < z) {
-        d = y;
-        side = 1;
-      } else {
-        const uint64_t kb = load_key(t64, tlen, (uint64_t)o[q] + z);
-        if (kb != pkey) {
-          side = kb > pkey ? 1 : -1;
-          d = z + key_lcp(kb, pkey);
-        } else if ((kb & 0xFFu) == 0) {
-          side = 0;
-          d = z + key_len(kb);
-        } else {
-          const StrCmp c = str_compare(t64, tlen, pivot, o[q], (uint64_t)z + 8);
-          side = -c.sign;
-          d = (uint32_t)c.lcp;
-        }
+      d1 = to_first;
+      s1 = x == 0 ? 0 : 1;
+      d2 = to_last[q];
+      s2 = x + 1 == own_end ? 0 : -1;
+    } else {
+      via_end(o[q], to_first, z1, c1, pk1, p1, true, &s1, &d1);
+      via_end(o[q], to_last[q], z2, c2, pk2, p2, false, &s2, &d2);
+    }
+    // The key at the deeper of the two depths (both when they are equal).
+    uint32_t rel = 0;
+    uint64_t key = 0;
+    const uint32_t kd = d1 == kInf ? d2 : (d2 == kInf ? d1 : max(d1, d2));
+    if (kd != kInf) {
+      key = load_key(t64, tlen, (uint64_t)o[q] + kd);
+      if (kd == d1) rel |= kKeyAtD1;
+      if (kd == d2) rel |= kKeyAtD2;
+      // A string of the pivots' run that ends where it leaves P1 equals
+      // P1 (it is not below it); one that ends where it leaves P2 equals P2
+      // when P2 ends there too.
+      if (own && (key >> 56) == 0) {
+        if (kd == d1 && x != 0) s1 = 0;
+        if (kd == d2 && d2 == len2 && x + 1 != own_end) s2 = 0;
       }
     }
-    uint64_t key = 0;
-    if (d != kInf) {
-      key = load_key(t64, tlen, (uint64_t)o[q] + d);
-      // A string of the pivot's run that ends where it leaves P equals P.
-      if (own && x != 0 && (key >> 56) == 0) side = 0;
-    }
+    rel |= (uint32_t)(s1 + 1) | ((uint32_t)(s2 + 1) << 2);
     s_off[x] = o[q];
     s_prev[x] = v[q];
-    s_dep[x] = d;
+    s_d1[x] = d1;
+    s_d2[x] = d2;
     s_key[x] = key;
-    s_side[x] = (int8_t)side;
+    s_rel[x] = (uint8_t)rel;
   }
   __syncthreads();
 
-  // A[i] against B[j], both staged, through the pivot.  `known` (bytes the
+  // A[i] against B[j], both staged, through the pivots.  `known` (bytes the
   // two are known to share) only moves the text comparison's start.
   auto cmp_block = [&](uint64_t i, uint64_t j, uint64_t known) -> StrCmp {
     const uint32_t e = (uint32_t)(i - i0), f = ia + (uint32_t)(j - j0);
-    const int se = s_side[e], sf = s_side[f];
-    const uint32_t de = s_dep[e], df = s_dep[f];
-    const uint32_t lo = min(de, df);
-    if (se != sf) return StrCmp{se < sf ? -1 : 1, lo};
-    if (se == 0) return StrCmp{0, lo};  // both equal P (P itself has d = kInf)
-    if (de != df) {
-      const bool e_less = se > 0 ? de > df : de < df;
-      return StrCmp{e_less ? -1 : 1, lo};
+    const uint32_t re = s_rel[e], rf = s_rel[f];
+    uint64_t lower = known;
+#pragma unroll
+    for (int pv = 0; pv < 2; ++pv) {
+      const int se = (int)((re >> (2 * pv)) & 3u) - 1, sf = (int)((rf >> (2 * pv)) & 3u) - 1;
+      const uint32_t de = pv ? s_d2[e] : s_d1[e], df = pv ? s_d2[f] : s_d1[f];
+      const uint32_t lo = min(de, df);
+      if (se != sf) return StrCmp{se < sf ? -1 : 1, lo};
+      if (se == 0) return StrCmp{0, lo};  // both equal the pivot
+      if (de != df) {
+        const bool e_less = se > 0 ? de > df : de < df;
+        return StrCmp{e_less ? -1 : 1, lo};
+      }
+      const uint32_t at = pv ? kKeyAtD2 : kKeyAtD1;
+      if ((re & at) && (rf & at)) {
+        const uint64_t ke = s_key[e], kf = s_key[f];
+        if (ke != kf) return StrCmp{ke < kf ? -1 : 1, de + key_lcp(ke, kf)};
+        if ((ke & 0xFFu) == 0) return StrCmp{0, de + key_len(ke)};
+        lower = max(lower, (uint64_t)de + 8);
+      } else {
+        lower = max(lower, (uint64_t)de);
+      }
     }
-    const uint64_t ke = s_key[e], kf = s_key[f];
-    if (ke != kf) return StrCmp{ke < kf ? -1 : 1, de + key_lcp(ke, kf)};
-    if ((ke & 0xFFu) == 0) return StrCmp{0, de + key_len(ke)};
-    const uint64_t from = (uint64_t)de + 8;
-    return str_compare(t64, tlen, s_off[e], s_off[f], known > from ? known : from);
+    return str_compare(t64, tlen, s_off[e], s_off[f], lower);
   };
   const uint64_t d = k0 + (uint64_t)threadIdx.x * kMergeItems;
   if (d >= k1) return;
