@@ -394,18 +394,50 @@ __device__ __forceinline__ uint64_t merge_path(const uint64_t* __restrict__ t64,
   return lo;
 }
 
-// split[p] = A strings among the first min(p * kMergeTile, na + nb) outputs.
+// The merges of one launch: up to kMergePairsMax independent 2-way merges
+// (a k-way merge's round), each with its own runs, output, split entries
+// and tiles; every kernel finds its pair by its split entry or tile.
+constexpr int kMergePairsMax = 32;
+struct MergePair {
+  const uint32_t* a;
+  const uint64_t* alcp;
+  uint64_t na;
+  const uint32_t* b;
+  const uint64_t* blcp;
+  uint64_t nb;
+  uint32_t* out;
+  uint64_t* olcp;
+  uint64_t split0;  // first split entry (merge_splits(na + nb) of them)
+  uint64_t tile0;   // first tile (merge_splits(na + nb) - 1 of them)
+};
+struct MergeBatch {
+  MergePair p[kMergePairsMax];
+  uint32_t count;
+  uint64_t splits, tiles;  // totals over the pairs
+};
+
+// The pair whose range [first(q), first(q) + size(q)) holds g (the first
+// pair's range starts at 0 and they follow each other).
+template <typename F>
+__device__ __forceinline__ uint32_t merge_pair_of(const MergeBatch& mb, uint64_t g, F first) {
+  uint32_t q = 0;
+  while (q + 1 < mb.count && first(mb.p[q + 1]) <= g) ++q;
+  return q;
+}
+
+// split[p] = A strings among the first min(p * kMergeTile, na + nb) outputs
+// of the pair that owns split entry p.
 __global__ __launch_bounds__(256) void k_merge_partition(const uint64_t* __restrict__ t64,
-                                                        uint64_t tlen,
-                                                        const uint32_t* __restrict__ a, uint64_t na,
-                                                        const uint32_t* __restrict__ b, uint64_t nb,
-                                                        uint64_t nparts, uint64_t* __restrict__ split) {
-  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p > nparts) return;
-  const uint64_t total = na + nb;
+                                                        uint64_t tlen, const MergeBatch mb,
+                                                        uint64_t* __restrict__ split) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= mb.splits) return;
+  const MergePair& mp = mb.p[merge_pair_of(mb, g, [](const MergePair& x) { return x.split0; })];
+  const uint64_t p = g - mp.split0;
+  const uint64_t total = mp.na + mp.nb;
   const uint64_t d = p * kMergeTile < total ? p * kMergeTile : total;
-  const uint64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
-  split[p] = merge_path(t64, tlen, a, b, d, lo, hi);
+  const uint64_t lo = d > mp.nb ? d - mp.nb : 0, hi = d < mp.na ? d : mp.na;
+  split[g] = merge_path(t64, tlen, mp.a, mp.b, d, lo, hi);
 }
 
 // Each thread merges kMergeItems outputs from its own merge-path split and
@@ -435,11 +467,20 @@ __global__ __launch_bounds__(256) void k_merge_partition(const uint64_t* __restr
 // between them, and from (z, c), the LCP and order of P1 and B0 (P2 and
 // Bl): y > z puts it where B0 (Bl) is, at depth z; y < z puts it above P1
 // (below P2) at depth y; y = z compares its key at z with the pivot's.
-__global__ __launch_bounds__(kMergeThreads) void k_merge(
-    const uint64_t* __restrict__ t64, uint64_t tlen, const uint32_t* __restrict__ a,
-    const uint64_t* __restrict__ alcp, uint64_t na, const uint32_t* __restrict__ b,
-    const uint64_t* __restrict__ blcp, uint64_t nb, const uint64_t* __restrict__ split,
-    uint32_t* __restrict__ out, uint64_t* __restrict__ olcp) {
+__global__ __launch_bounds__(kMergeThreads) void k_merge(const uint64_t* __restrict__ t64,
+                                                        uint64_t tlen, const MergeBatch mb,
+                                                        const uint64_t* __restrict__ split_all) {
+  const MergePair& mp = mb.p[merge_pair_of(mb, blockIdx.x, [](const MergePair& x) { return x.tile0; })];
+  const uint64_t tile = blockIdx.x - mp.tile0;
+  const uint32_t* __restrict__ a = mp.a;
+  const uint64_t* __restrict__ alcp = mp.alcp;
+  const uint64_t na = mp.na;
+  const uint32_t* __restrict__ b = mp.b;
+  const uint64_t* __restrict__ blcp = mp.blcp;
+  const uint64_t nb = mp.nb;
+  const uint64_t* __restrict__ split = split_all + mp.split0;
+  uint32_t* __restrict__ out = mp.out;
+  uint64_t* __restrict__ olcp = mp.olcp;
   constexpr uint32_t kInf = 0xFFFFFFFFu;
   constexpr int W = kMergeThreads / kWave;
   // Relation bits: side to P1 + 1 (bits 0-1), side to P2 + 1 (bits 2-3),
@@ -455,9 +496,9 @@ __global__ __launch_bounds__(kMergeThreads) void k_merge(
   __shared__ uint64_t s_pz[6];  // z, B end's side, pivot's key at z: for P1 / B0, P2 / Bl
   __shared__ uint64_t s_len2;   // length of P2
   const uint64_t total = na + nb;
-  const uint64_t k0 = (uint64_t)blockIdx.x * kMergeTile;
+  const uint64_t k0 = tile * kMergeTile;
   const uint64_t k1 = k0 + kMergeTile < total ? k0 + kMergeTile : total;
-  const uint64_t i0 = split[blockIdx.x], i1 = split[blockIdx.x + 1];
+  const uint64_t i0 = split[tile], i1 = split[tile + 1];
   const uint64_t j0 = k0 - i0, j1 = k1 - i1;
   const uint32_t ia = (uint32_t)(i1 - i0), jb = (uint32_t)(j1 - j0);  // ia + jb = k1 - k0
   const uint32_t m = ia + jb;
@@ -882,16 +923,34 @@ namespace gss {
 
 uint64_t merge_splits(uint64_t total) { return (total + kMergeTile - 1) / kMergeTile + 1; }
 
+// Queues the merges of a batch (their splits in split[0, mb.splits)).
+hipError_t merge_batch_launch(const uint64_t* t64, uint64_t tlen, const MergeBatch& mb,
+                              uint64_t* split, hipStream_t st) {
+  if (!mb.count || !mb.tiles) return hipSuccess;
+  k_merge_partition<<<(uint32_t)((mb.splits + 255) / 256), 256, 0, st>>>(t64, tlen, mb, split);
+  k_merge<<<(uint32_t)mb.tiles, kMergeThreads, 0, st>>>(t64, tlen, mb, split);
+  return hipGetLastError();
+}
+
+// Adds a merge to the batch (false when the batch is full).
+bool merge_batch_add(MergeBatch& mb, const uint32_t* a, const uint64_t* alcp, uint64_t na,
+                     const uint32_t* b, const uint64_t* blcp, uint64_t nb, uint32_t* out,
+                     uint64_t* olcp) {
+  if (mb.count >= (uint32_t)kMergePairsMax) return false;
+  const uint64_t ns = merge_splits(na + nb);
+  mb.p[mb.count++] = MergePair{a, alcp, na, b, blcp, nb, out, olcp, mb.splits, mb.tiles};
+  mb.splits += ns;
+  mb.tiles += ns - 1;
+  return true;
+}
+
 hipError_t merge_launch(const uint64_t* t64, uint64_t tlen, const uint32_t* a,
                         const uint64_t* alcp, uint64_t na, const uint32_t* b,
                         const uint64_t* blcp, uint64_t nb, uint32_t* out, uint64_t* olcp,
                         uint64_t* split, hipStream_t st) {
-  const uint64_t nparts = merge_splits(na + nb) - 1;
-  k_merge_partition<<<(uint32_t)((nparts + 1 + 255) / 256), 256, 0, st>>>(t64, tlen, a, na, b,
-                                                                         nb, nparts, split);
-  k_merge<<<(uint32_t)nparts, kMergeThreads, 0, st>>>(t64, tlen, a, alcp, na, b, blcp, nb,
-                                                      split, out, olcp);
-  return hipGetLastError();
+  MergeBatch mb{};
+  merge_batch_add(mb, a, alcp, na, b, blcp, nb, out, olcp);
+  return merge_batch_launch(t64, tlen, mb, split, st);
 }
 
 int merge_check(const unsigned char* d_text, uint64_t text_bytes, const uint32_t* d_a,
@@ -984,7 +1043,7 @@ int gss_merge_lcp_kway_device32(const unsigned char* d_text, uint64_t text_bytes
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(sc->mu);
   const size_t b_off = al256(4 * total), b_lcp = al256(8 * total);
-  const size_t b_split = al256(8 * (merge_splits(total) + cur.size()));
+  const size_t b_split = al256(8 * (merge_splits(total) + cur.size() + 2));
   rc = scratch_reserve(*sc, 2 * b_off + 2 * b_lcp + b_split, st);
   if (rc) return rc;
   char* base = sc->dev;
@@ -996,32 +1055,32 @@ int gss_merge_lcp_kway_device32(const unsigned char* d_text, uint64_t text_bytes
   const uint64_t tlen = text_bytes;
   // Rounds of pairwise merges of adjacent runs (run 2i with 2i + 1), so
   // that equal strings keep the order of their runs; the last round writes
-  // the output.  The rounds run in stream order, so one round's buffer and
-  // split table are free again two rounds later.
+  // the output.  All merges of a round go out in one pair of launches (up
+  // to kMergePairsMax at a time); an odd run is a merge with an empty run,
+  // which copies it with its LCPs.  The rounds run in stream order, so one
+  // round's buffer is free again two rounds later and the split table after
+  // each launch.
   for (int round = 0; cur.size() > 1; ++round) {
     const bool last = cur.size() == 2;
     uint32_t* o_off = last ? d_out : t_off[round & 1];
     uint64_t* o_lcp = last ? d_out_lcp : t_lcp[round & 1];
     std::vector<Run> next;
-    uint64_t p = 0, sp = 0;
+    uint64_t p = 0;
+    MergeBatch mb{};
     for (size_t r = 0; r < cur.size(); r += 2) {
-      if (r + 1 < cur.size()) {
-        const uint64_t len = cur[r].len + cur[r + 1].len;
-        PL_HIP(merge_launch(t64, tlen, cur[r].off, cur[r].lcp, cur[r].len, cur[r + 1].off,
-                            cur[r + 1].lcp, cur[r + 1].len, o_off + p,
-                            o_lcp ? o_lcp + p : nullptr, split + sp, st));
-        next.push_back(Run{o_off + p, o_lcp ? o_lcp + p : nullptr, len});
-        p += len;
-        sp += merge_splits(len);
-      } else {  // the odd run moves to this round's buffer unchanged
-        PL_HIP(hipMemcpyAsync(o_off + p, cur[r].off, 4 * cur[r].len, hipMemcpyDeviceToDevice, st));
-        if (cur[r].len > 1)
-          PL_HIP(hipMemcpyAsync(o_lcp + p, cur[r].lcp, 8 * (cur[r].len - 1),
-                                hipMemcpyDeviceToDevice, st));
-        next.push_back(Run{o_off + p, o_lcp + p, cur[r].len});
-        p += cur[r].len;
+      const Run& x = cur[r];
+      const Run y = r + 1 < cur.size() ? cur[r + 1] : Run{nullptr, nullptr, 0};
+      const uint64_t len = x.len + y.len;
+      if (mb.count == (uint32_t)kMergePairsMax) {
+        PL_HIP(merge_batch_launch(t64, tlen, mb, split, st));
+        mb = MergeBatch{};
       }
+      merge_batch_add(mb, x.off, x.lcp, x.len, y.off, y.lcp, y.len, o_off + p,
+                      o_lcp ? o_lcp + p : nullptr);
+      next.push_back(Run{o_off + p, o_lcp ? o_lcp + p : nullptr, len});
+      p += len;
     }
+    PL_HIP(merge_batch_launch(t64, tlen, mb, split, st));
     cur.swap(next);
   }
   PL_HIP(hipStreamSynchronize(st));
