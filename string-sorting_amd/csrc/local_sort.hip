@@ -335,7 +335,14 @@ __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, u
   // that stops at key e consumed e + 1 of them, a skipping group KW.  Built
   // from ballots, so the count is wave-uniform (scalar registers).
   unsigned long long wave_levels = 0;
+  // A launch with the top bit of `first` set was sized before the host knew
+  // the class's segment count (the largest class, launched right after the
+  // passes): blocks past the device count have no segment.  Both loads are
+  // issued before the test.
+  const bool bounded = (first & 0x80000000u) != 0;
+  first &= 0x7FFFFFFFu;
   const LocalSeg sg = p.local[kClass][first + blockIdx.x];
+  if (bounded && first + blockIdx.x >= min(p.ctr->local[kClass], p.cap_local[kClass])) return;
   const uint32_t n = sg.size;
   const uint32_t gb = sg.begin;
   const uint32_t* __restrict__ src = p.idx[sg.buf];

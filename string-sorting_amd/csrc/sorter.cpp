@@ -98,6 +98,10 @@ struct Context {
   // it first looks at the counters.
   uint32_t pass_hint = 1;
   uint64_t hint_strings = 0;
+  // Segments of the largest local class the last sort on this context
+  // sorted on its main stream (a sort of a similar size then launches that
+  // class before it reads the counters back, for about as many segments).
+  uint32_t hint_largest = 0;
   std::mutex mu;
   gss_stats stats{};
   std::vector<Pending> pending;
@@ -554,22 +558,39 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   // string reaches past the text, so the pass count is bounded.
   const uint64_t max_passes = text_bytes + 64;
   uint32_t pass = 0, cur = 0;
+  // The local phase's profile bracket and the largest class's launch token,
+  // opened early when that class goes out before the read-back.
+  ProfToken phase_tok{};
+  bool phase_open = false;
+  uint32_t largest_main = 0;  // largest-class segments the main stream took
+  long local_tok[3] = {-1, -1, -1};
   // Publishes `bytes` from the start of the statistics block (all of it and
   // the counters behind it, or from the counters on) and waits for them.
-  auto read_back = [&](bool counters_only) -> int {
+  // (Queued and awaited in two steps, so that work queued in between runs
+  // while the host waits.)
+  uint64_t pub_wait = 0;
+  auto publish = [&](bool counters_only) -> int {
     const uint64_t seq = ++w.pub_seq;
     const size_t skip = counters_only ? w.init_ctr_off : 0;
     GSS_HIP(launch_publish(reinterpret_cast<const char*>(p.lstats) + skip,
                            w.init_ctr_off + sizeof(Counters) - skip, nullptr, nullptr, 0, 8,
                            w.h_pub + kPubHead + skip, reinterpret_cast<uint64_t*>(w.h_pub), seq,
                            st));
-    const int r = wait_published(w, st, seq);
+    pub_wait = seq;
+    return GSS_OK;
+  };
+  auto collect = [&]() -> int {
+    const int r = wait_published(w, st, pub_wait);
     if (r) return r;
     memcpy(w.h_ctr, w.h_pub + kPubHead + w.init_ctr_off, sizeof(Counters));
     if (w.h_ctr->error)
       return fail(GSS_E_INTERNAL,
                   "device sort failed (code " + std::to_string(w.h_ctr->error) + ")");
     return GSS_OK;
+  };
+  auto read_back = [&](bool counters_only) -> int {
+    const int r = publish(counters_only);
+    return r ? r : collect();
   };
   // When the host has read the counters between two passes: the local
   // segments the passes so far left are sorted on the side stream while the
@@ -654,10 +675,45 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
           c.hint_strings && (nreq > 1 || (pass0_strings <= 2 * c.hint_strings &&
                                           c.hint_strings <= 2 * pass0_strings));
       uint32_t queue = similar ? std::max<uint32_t>(1, c.pass_hint) : 1u;
+      // After the hinted passes the largest local class usually has all its
+      // segments: it is launched then, before the read-back, with a grid of
+      // the most segments it can hold (blocks past the device count exit),
+      // so the host round trip overlaps its run.  Segments a further pass
+      // adds are launched with the rest of the local phase.
+      bool speculate = similar && c.hint_largest;
       while (queue) {
         for (uint32_t k = 0; k < queue; ++k)
           if ((rc = run_pass(&ran))) return rc;
-        if ((rc = read_back(true))) return rc;
+        if ((rc = publish(true))) return rc;
+        uint32_t spec_grid = 0;
+        if (speculate) {
+          // An empty block still holds a CU's LDS while it reads the count,
+          // so the grid is the last sort's count with some slack, not the
+          // class's capacity; segments beyond it wait for the local phase.
+          const uint64_t ub = std::min<uint64_t>(
+              p.cap_local[2], n / ((uint64_t)kLocalClassCap[1] + 1) + nreq + 1);
+          const uint64_t want = std::min<uint64_t>(
+              ub, (uint64_t)launched[2] + c.hint_largest + c.hint_largest / 8 + 64);
+          if (want > launched[2]) {
+            phase_tok = prof_begin(c, "local_phase", 0, st);
+            phase_open = true;
+            spec_grid = (uint32_t)(want - launched[2]);
+            const size_t mark = c.pending.size();
+            GSS_LAUNCH(kLocalNames[2], 0,
+                       launch_local(p, 2, launched[2] | 0x80000000u, spec_grid, st,
+                                    count_levels));
+            if (c.pending.size() > mark) local_tok[2] = (long)mark;
+          }
+        }
+        if ((rc = collect())) return rc;
+        if (spec_grid) {
+          // It took the segments the counters show now, up to its grid.
+          const uint32_t now = std::min(w.h_ctr->local[2], p.cap_local[2]);
+          const uint32_t took = std::min(now, launched[2] + spec_grid);
+          largest_main += took - std::min(took, launched[2]);
+          launched[2] = std::max(launched[2], took);
+        }
+        speculate = false;
         queue = w.h_ctr->big_next ? 1u : 0u;
         if (queue && (rc = maybe_launch_early(w.h_ctr->big_next_strings))) return rc;
       }
@@ -672,9 +728,8 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   uint32_t nlocal[3];
   for (int cls = 0; cls < 3; ++cls)
     nlocal[cls] = std::min(w.h_ctr->local[cls], p.cap_local[cls]) - launched[cls];
-  ProfToken phase_tok = prof_begin(c, "local_phase", 0, st);
+  if (!phase_open) phase_tok = prof_begin(c, "local_phase", 0, st);
   if (nretire) GSS_LAUNCH("retire", nretire, launch_retire(p, std::min<uint32_t>(nretire, 8192), st));
-  long local_tok[3] = {-1, -1, -1};
   for (int cls = 2; cls >= 0; --cls) {
     if (!nlocal[cls]) continue;
     const size_t mark = c.pending.size();
@@ -702,6 +757,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   if (nbig) {
     c.pass_hint = std::max<uint32_t>(1, passes);
     c.hint_strings = pass0_strings;
+    c.hint_largest = largest_main + nlocal[2];
   }
   for (int cls = 0; cls < 3; ++cls) {
     stats.local_segments += w.h_ctr->local[cls];

@@ -188,6 +188,37 @@ def test_repeat_calls_reuse_workspace(oracle_mod):
         assert_matches_oracle(ss, off, lcp, oracle_mod)
 
 
+def _grouped(groups, sub, per, seed):
+    """groups x sub groups of `per` strings: every group has its own two
+    first bytes, every subgroup its own next two, then 6 random bytes."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for g in range(groups):
+        for h in range(sub):
+            head = bytes([97 + g % 26, 97 + g // 26, 65 + h % 26, 65 + h // 26])
+            tails = rng.integers(97, 123, size=(per, 6)).astype(np.uint8)
+            out += [head + bytes(t) for t in tails]
+    return [out[i] for i in rng.permutation(len(out))]
+
+
+def test_largest_class_launched_before_readback(oracle_mod):
+    """The largest local class goes out before the counters are read back
+    when the last similar sort on the context had work for it: repeated
+    sorts (its segments all known after the hinted pass), a sort that needs
+    one more pass than the hint (the early launch finds no segment; the
+    later pass's segments are launched after it), and a sort with one pass
+    fewer than the hint (an empty pass is queued before the early launch).
+    Every result equals msd_CE2's with exact LCPs."""
+    one_pass = _set(_grouped(12, 1, 3000, 1), "one_pass")   # 12 segments of 3,000 after pass 0
+    two_pass = _set(_grouped(4, 3, 3000, 2), "two_pass")    # 4 x 9,000, then 12 x 3,000
+    for ss in (one_pass, one_pass, one_pass, two_pass, two_pass, one_pass, one_pass):
+        log(f"largest-class early launch: {ss.name}")
+        off, lcp = gpu_sort(ss)
+        assert_matches_oracle(ss, off, lcp, oracle_mod)
+        st = gss.stats()
+        assert st.global_passes == (1 if ss.name == "one_pass" else 2)
+
+
 def test_without_lcp(oracle_mod):
     ss = gen.make(1, n=30000, seed=5)
     off, _ = gpu_sort(ss, lcp=False)
