@@ -65,10 +65,22 @@ constexpr bool counts_ranks(uint32_t cap) {
   return cap > kLocalClassCap[1] || cap <= kLocalClassCap[0];
 }
 
+// A segment's first sorting round in the largest class ranks its strings by
+// one digit of the two most significant varying key bytes (each byte ranked
+// in the alphabet found at its position) when the digit takes at most this
+// many values.
+constexpr uint32_t kFirstDigits = 2048;
+
 template <int BLOCK, int CAP>
 struct LocalSmem {
   static constexpr int W = BLOCK / kWave;
   static constexpr int SS = CAP / 2 + 2;
+  static constexpr bool kFirst = CAP > kLocalClassCap[1];  // first-round counting pass
+  // First-round counting pass: bucket counts, then starts (entry nbu = total);
+  // the byte values present at the two digit positions and their ranks.
+  uint32_t fcnt[kFirst ? kFirstDigits + 1 : 1];
+  uint8_t fpres[kFirst ? 2 : 1][256];
+  uint8_t frank[kFirst ? 2 : 1][256];
   uint64_t key[CAP];
   uint32_t off[CAP];
   uint16_t sid[CAP];
@@ -304,6 +316,133 @@ __device__ __forceinline__ bool local_two_byte_count(LocalSmem<BLOCK, CAP>& s, u
     if (cd[j] != 0xFFFFFFFFu) s.sid[base + j * kWave] = 0;  // the one group again
   __syncthreads();
   return !bail;
+}
+
+// A round with one group, at least two varying key bytes (largest class):
+// each string's digit is (rank of its top varying byte, rank of the next
+// one), both ranked among the byte values that occur at that position, so
+// digits order strings as those two bytes do.  One counting pass (LDS
+// atomics) puts every string into its digit's bucket, tagged with its index
+// before the round; then each string's place inside its bucket is the number
+// of bucket members with a smaller key, or the same key and a smaller index
+// -- the stable order.  Returns 1 when the group is sorted; 0, nothing moved,
+// when the digits take more than kFirstDigits values (the caller sorts by the
+// two-byte passes); -1, nothing moved, when a bucket holds more than
+// kCountMax strings (the caller's LSD passes then sort the group).
+template <int BLOCK, int CAP>
+__device__ __forceinline__ int local_first_count(LocalSmem<BLOCK, CAP>& s, uint32_t nact,
+                                                 uint32_t base, uint32_t jn, uint32_t vm) {
+  constexpr int IPT = CAP / BLOCK;
+  static_assert(BLOCK == 1024 && kFirstDigits == 2 * BLOCK, "two digit counters per thread");
+  const uint32_t wid = threadIdx.x / kWave;
+  const uint32_t lane = lane_id();
+  const int top = 31 - __builtin_clz(vm);
+  const int sec = 31 - __builtin_clz(vm & ~(1u << top));
+  // Byte values present at both positions; the counters cleared meanwhile.
+  if (threadIdx.x < 128) reinterpret_cast<uint32_t*>(&s.fpres[0][0])[threadIdx.x] = 0u;
+  s.fcnt[2 * threadIdx.x] = 0u;
+  s.fcnt[2 * threadIdx.x + 1] = 0u;
+  uint64_t k[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t i = base + j * kWave;
+    k[j] = (uint32_t)j < jn && i < nact ? s.key[i] : 0ull;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    if ((uint32_t)j < jn && base + j * kWave < nact) {
+      s.fpres[0][(uint32_t)(k[j] >> (8 * top)) & 0xFFu] = 1;
+      s.fpres[1][(uint32_t)(k[j] >> (8 * sec)) & 0xFFu] = 1;
+    }
+  }
+  __syncthreads();
+  // Ranks: wave 0 for the top byte, wave 1 for the second.
+  if (wid < 2) {
+    const uint64_t lt = lanemask_lt();
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const uint64_t bal = __ballot(s.fpres[wid][64 * c + lane] != 0);
+      s.frank[wid][64 * c + lane] = (uint8_t)(cnt + (uint32_t)__popcll(bal & lt));
+      cnt += (uint32_t)__popcll(bal);
+    }
+    if (lane == 0) s.wtot[wid] = cnt;
+  }
+  __syncthreads();
+  const uint32_t a1 = s.wtot[0], a2 = s.wtot[1];
+  if (a1 * a2 > kFirstDigits) return 0;  // block-uniform
+  uint32_t d[IPT], r[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    d[j] = 0;
+    r[j] = 0;
+    if ((uint32_t)j < jn && base + j * kWave < nact) {
+      d[j] = (uint32_t)s.frank[0][(uint32_t)(k[j] >> (8 * top)) & 0xFFu] * a2 +
+             s.frank[1][(uint32_t)(k[j] >> (8 * sec)) & 0xFFu];
+      r[j] = atomicAdd(&s.fcnt[d[j]], 1u);
+    }
+  }
+  __syncthreads();
+  // Bucket starts; a bucket over kCountMax strings sends the group to the
+  // LSD passes before anything moves.
+  const uint32_t c0 = s.fcnt[2 * threadIdx.x], c1 = s.fcnt[2 * threadIdx.x + 1];
+  if (__syncthreads_or(c0 > kCountMax || c1 > kCountMax)) return -1;
+  uint32_t total;
+  const uint32_t st = block_excl_scan<BLOCK>(c0 + c1, s.scan, &total);
+  s.fcnt[2 * threadIdx.x] = st;
+  s.fcnt[2 * threadIdx.x + 1] = st + c0;
+  if (threadIdx.x == 0) s.fcnt[kFirstDigits] = total;
+  uint32_t o[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t i = base + j * kWave;
+    o[j] = (uint32_t)j < jn && i < nact ? s.off[i] : 0u;
+  }
+  __syncthreads();
+  // Into digit order, each string tagged with its index before the round.
+  uint32_t lo[IPT], hi[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    lo[j] = 0;
+    hi[j] = 0;
+    const uint32_t i = base + j * kWave;
+    if ((uint32_t)j < jn && i < nact) {
+      lo[j] = s.fcnt[d[j]];
+      hi[j] = s.fcnt[d[j] + 1];
+      const uint32_t pos = lo[j] + r[j];
+      s.key[pos] = k[j];
+      s.off[pos] = o[j];
+      s.sid[pos] = (uint16_t)i;
+    }
+  }
+  __syncthreads();
+  // Stable place inside the bucket.
+  uint32_t fin[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const uint32_t i = base + j * kWave;
+    fin[j] = 0xFFFFFFFFu;
+    if ((uint32_t)j < jn && i < nact) {
+      uint32_t c = 0;
+      for (uint32_t m = lo[j]; m < hi[j]; ++m) {
+        const uint64_t km = s.key[m];
+        c += (km < k[j] || (km == k[j] && s.sid[m] < i)) ? 1u : 0u;
+      }
+      fin[j] = lo[j] + c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    if (fin[j] != 0xFFFFFFFFu) {
+      s.key[fin[j]] = k[j];
+      s.off[fin[j]] = o[j];
+      s.sid[fin[j]] = 0;  // the one group again
+    }
+  }
+  __syncthreads();
+  return 1;
 }
 
 // COUNT: also count the 8-byte key levels the strings advance through (for
@@ -648,12 +787,23 @@ __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, u
           }
         }
         __syncthreads();
-      } else if (!(kClass == 2 && nss == 1 &&
-                   local_two_byte_count<BLOCK, CAP>(s, nact, ipw, base, jn, vm, kCountMax))) {
-        for (int b = 0; b < 8; ++b)
-          if ((vm >> b) & 1u) local_radix_pass<BLOCK, CAP>(s, nact, base, jn, b);
-        if (nss > 1) local_radix_pass<BLOCK, CAP>(s, nact, base, jn, 8);
-        if (nss > 256) local_radix_pass<BLOCK, CAP>(s, nact, base, jn, 9);
+      } else {
+        // One group with two or more varying bytes: the counting pass, else
+        // the two-byte passes; the LSD passes when a bucket is too large.
+        int first = 0;
+        if constexpr (kClass == 2) {
+          if (nss == 1 && (vm & (vm - 1u)))
+            first = local_first_count<BLOCK, CAP>(s, nact, base, jn, vm);
+          if (first == 0 && nss == 1 &&
+              local_two_byte_count<BLOCK, CAP>(s, nact, ipw, base, jn, vm, kCountMax))
+            first = 1;
+        }
+        if (first != 1) {
+          for (int b = 0; b < 8; ++b)
+            if ((vm >> b) & 1u) local_radix_pass<BLOCK, CAP>(s, nact, base, jn, b);
+          if (nss > 1) local_radix_pass<BLOCK, CAP>(s, nact, base, jn, 8);
+          if (nss > 256) local_radix_pass<BLOCK, CAP>(s, nact, base, jn, 9);
+        }
       }
     }
 
