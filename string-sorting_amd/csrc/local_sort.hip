@@ -322,10 +322,13 @@ __device__ __forceinline__ bool local_two_byte_count(LocalSmem<BLOCK, CAP>& s, u
 // each string's digit is (rank of its top varying byte, rank of the next
 // one), both ranked among the byte values that occur at that position, so
 // digits order strings as those two bytes do.  One counting pass (LDS
-// atomics) puts every string into its digit's bucket, tagged with its index
-// before the round; then each string's place inside its bucket is the number
-// of bucket members with a smaller key, or the same key and a smaller index
-// -- the stable order.  Returns 1 when the group is sorted; 0, nothing moved,
+// atomics) puts every string into its digit's bucket; then each string's
+// place inside its bucket is the number of bucket members with a smaller key,
+// or the same key and a smaller index -- the stable order.  Inside a bucket
+// the two digit bytes and every byte above them are the same, so a string
+// is ranked by one 64-bit value, its key bytes below the second digit byte
+// (at most six) shifted up 16 bits, OR its index before the round: distinct
+// values that order as (key, index) do.  Returns 1 when the group is sorted; 0, nothing moved,
 // when the digits take more than kFirstDigits values (the caller sorts by the
 // two-byte passes); -1, nothing moved, when a bucket holds more than
 // kCountMax strings (the caller's LSD passes then sort the group).
@@ -400,7 +403,9 @@ __device__ __forceinline__ int local_first_count(LocalSmem<BLOCK, CAP>& s, uint3
     o[j] = (uint32_t)j < jn && i < nact ? s.off[i] : 0u;
   }
   __syncthreads();
-  // Into digit order, each string tagged with its index before the round.
+  // Into digit order, as rank values (the offsets stay in registers).
+  static_assert(CAP <= 65536, "indices fit 16 bits");
+  const uint64_t low = sec ? (1ull << (8 * sec)) - 1ull : 0ull;  // sec <= 6
   uint32_t lo[IPT], hi[IPT];
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
@@ -410,25 +415,25 @@ __device__ __forceinline__ int local_first_count(LocalSmem<BLOCK, CAP>& s, uint3
     if ((uint32_t)j < jn && i < nact) {
       lo[j] = s.fcnt[d[j]];
       hi[j] = s.fcnt[d[j] + 1];
-      const uint32_t pos = lo[j] + r[j];
-      s.key[pos] = k[j];
-      s.off[pos] = o[j];
-      s.sid[pos] = (uint16_t)i;
+      s.key[lo[j] + r[j]] = ((k[j] & low) << 16) | i;
     }
   }
   __syncthreads();
-  // Stable place inside the bucket.
+  // Place inside the bucket: the members with a smaller rank value, four
+  // reads in flight at a time.
   uint32_t fin[IPT];
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
     const uint32_t i = base + j * kWave;
     fin[j] = 0xFFFFFFFFu;
     if ((uint32_t)j < jn && i < nact) {
-      uint32_t c = 0;
-      for (uint32_t m = lo[j]; m < hi[j]; ++m) {
-        const uint64_t km = s.key[m];
-        c += (km < k[j] || (km == k[j] && s.sid[m] < i)) ? 1u : 0u;
+      const uint64_t v = ((k[j] & low) << 16) | i;
+      uint32_t c = 0, m = lo[j];
+      for (; m + 4 <= hi[j]; m += 4) {
+        const uint64_t a0 = s.key[m], a1 = s.key[m + 1], a2 = s.key[m + 2], a3 = s.key[m + 3];
+        c += (a0 < v ? 1u : 0u) + (a1 < v ? 1u : 0u) + (a2 < v ? 1u : 0u) + (a3 < v ? 1u : 0u);
       }
+      for (; m < hi[j]; ++m) c += s.key[m] < v ? 1u : 0u;
       fin[j] = lo[j] + c;
     }
   }
