@@ -607,18 +607,28 @@ def main():
         step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    # Kernel and pass breakdown: the same sort again, each launch and pass
-    # bracketed by HIP events, outside the timed steps above.  The key levels
-    # the local kernels take strings through (their work units) come from one
-    # more sort with their counting variant on; durations come from the
-    # production kernels.
+    # Kernel and pass breakdown: the same sort again, outside the timed steps
+    # above.  Pass times come from sorts in which only each pass and the
+    # local phase are bracketed by HIP events (an event between two kernels
+    # holds the second back about 10 us; queued kernels otherwise follow each
+    # other with no gap), kernel times from sorts with every launch
+    # bracketed.  The key levels the local kernels take strings through
+    # (their work units) come from one more sort with their counting variant
+    # on; durations come from the production kernels.
     prof_steps = max(1, min(args.steps, 10))
+    gss.profile_reset()
+    gss.profile_enable(True, passes_only=True)
+    for _ in range(prof_steps):
+        step()
+    torch.cuda.synchronize()
+    pass_timing = {k: gss.profile_get(k) for k in PASSES}
     gss.profile_reset()
     gss.profile_enable(True)
     for _ in range(prof_steps):
         step()
     torch.cuda.synchronize()
-    timing = {k: gss.profile_get(k) for k in KERNELS + PASSES}
+    timing = {k: gss.profile_get(k) for k in KERNELS}
+    timing.update(pass_timing)
     gss.profile_reset()
     gss.profile_enable(True, levels=True)
     step()

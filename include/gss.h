@@ -352,12 +352,17 @@ int gss_ctx_get_stats(gss_ctx *ctx, gss_stats *out); /* last sort of ctx */
 /* Kernel timing: with GSS_PROFILE_EVENTS set, every kernel launch is
  * bracketed by HIP events recorded on the stream the kernel runs on; totals
  * per kernel accumulate until reset.  `name` selects the kernel ("gather",
- * "histogram", "scatter", "local", ...).  GSS_PROFILE_LEVELS runs the local
+ * "histogram", "scatter", "local", ...) or a pass ("pass0", ..., "pass9+",
+ * "local_phase").  An event between two kernels holds the second one back
+ * (about 10 us on the MI355X, while queued kernels follow each other with no
+ * gap), so pass times are read with GSS_PROFILE_PASSES alone: only each pass
+ * and the local phase are bracketed.  GSS_PROFILE_LEVELS runs the local
  * kernels' counting variant, which fills gss_stats.local_level_strings (and
  * the items of the local kernels' totals) at some cost in kernel time: time
- * with EVENTS alone, count with LEVELS.  0 turns both off. */
+ * with EVENTS or PASSES, count with LEVELS.  0 turns all off. */
 #define GSS_PROFILE_EVENTS 1
 #define GSS_PROFILE_LEVELS 2
+#define GSS_PROFILE_PASSES 4
 int gss_profile_enable(int mode);
 int gss_profile_reset(void);
 int gss_profile_get(const char *name, uint64_t *launches, double *total_ms,

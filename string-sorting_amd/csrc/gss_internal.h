@@ -89,21 +89,38 @@ struct Boundary {
 // later pass's set-up kernel takes its segment count from big_next and sets
 // big_cur, ntiles and active_cur, which the pass's other kernels (and the
 // host, when it reads them) use; the first pass's come from the host.
+struct ClassCount {
+  uint32_t segs;     // LocalSegs of the size class
+  uint32_t strings;  // their strings
+};
 struct Counters {
+  // Counters reserved together are adjacent, 8-byte aligned pairs, taken
+  // with one 64-bit atomic (atomic_add_pair): thousands of workgroups
+  // reserving list slots serialise on these few addresses.
   uint32_t big_next;     // BigSegs emitted for the next pass (initially: the first list)
-  uint32_t local[3];     // LocalSegs per size class
-  uint32_t retire;       // Retire chunks
+  uint32_t big_next_strings;  // strings of the BigSegs emitted for the next pass
   uint32_t boundary;     // Boundaries emitted so far
+  uint32_t retire;       // Retire chunks
+  ClassCount local[3];   // LocalSegs per size class and their strings
   uint32_t flat_len;     // entries of the current pass's flat count array
   uint32_t error;        // nonzero: a list overflowed or a loop bound was hit
-  uint32_t local_strings[3];  // strings handed to each local size class
   uint32_t big_cur;      // BigSegs of the current pass
   uint32_t ntiles;       // tiles of the current pass
   uint32_t bound_done;   // boundaries whose LCPs earlier passes wrote
   uint32_t active_cur;   // strings of the current pass's segments
-  uint32_t big_next_strings;  // strings of the BigSegs emitted for the next pass
 };
 static_assert(sizeof(Counters) % 8 == 0, "Counters is copied in 8-byte words");
+static_assert(offsetof(Counters, big_next) % 8 == 0 && offsetof(Counters, boundary) % 8 == 0 &&
+                  offsetof(Counters, local) % 8 == 0,
+              "pairs of counters are 8-byte aligned");
+
+// Adds a to *lo and b to the word after it (an 8-byte aligned pair) in one
+// 64-bit atomic; returns the old value of the pair (low word: *lo).  Neither
+// word may overflow.
+__device__ __forceinline__ uint64_t atomic_add_pair(uint32_t* lo, uint32_t a, uint32_t b) {
+  return atomicAdd(reinterpret_cast<unsigned long long*>(lo),
+                   ((unsigned long long)b << 32) | (unsigned long long)a);
+}
 
 // Strings active in each global pass (the statistics of a sort), written by
 // the pass's set-up kernel.

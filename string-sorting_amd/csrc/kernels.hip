@@ -618,8 +618,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(Plan p) {
 __device__ __forceinline__ void push_local(Plan& p, uint32_t begin, uint32_t size,
                                            uint32_t depth, uint32_t buf) {
   const int c = size <= kLocalClassCap[0] ? 0 : (size <= kLocalClassCap[1] ? 1 : 2);
-  const uint32_t slot = atomicAdd(&p.ctr->local[c], 1u);
-  atomicAdd(&p.ctr->local_strings[c], size);
+  const uint32_t slot = (uint32_t)atomic_add_pair(&p.ctr->local[c].segs, 1u, size);
   if (slot >= p.cap_local[c]) {
     atomicOr(&p.ctr->error, 2u);
     return;
@@ -633,8 +632,7 @@ __device__ __forceinline__ void push_child(Plan& p, BigSeg* next, uint32_t begin
     push_local(p, begin, size, depth, buf);
     return;
   }
-  const uint32_t slot = atomicAdd(&p.ctr->big_next, 1u);
-  atomicAdd(&p.ctr->big_next_strings, size);
+  const uint32_t slot = (uint32_t)atomic_add_pair(&p.ctr->big_next, 1u, size);
   if (slot >= p.cap_big) {
     atomicOr(&p.ctr->error, 4u);
     return;
@@ -657,11 +655,12 @@ __device__ __forceinline__ void push_retire(Plan& p, uint32_t begin, uint32_t si
   }
 }
 
-template <int RB>
+template <int RB, int BS>
 __device__ __forceinline__ void segscan_seg(Plan& p, const BigSeg* segs, BigSeg* next,
                                             uint32_t si) {
   constexpr uint32_t NB = 1u << RB;
-  constexpr uint32_t DPT = NB / 256;
+  constexpr uint32_t DPT = NB / BS;
+  static_assert(DPT >= 1, "at least one digit per thread");
   __shared__ uint32_t s_first;
   // The segment's count-matrix place and the alphabet are loaded with its
   // bounds, before its min/max can end the block.
@@ -763,30 +762,30 @@ __device__ __forceinline__ void segscan_seg(Plan& p, const BigSeg* segs, BigSeg*
     }
   }
 
-  if (sg_next) atomicAdd(&p.ctr->big_next_strings, sg_next);
-
-  // One reservation per list for the whole segment: same-address atomics
-  // from every bucket would serialise.
-  __shared__ uint32_t s_scan[9 * (256 / kWave + 1)];
+  // One reservation per list for the whole segment, the lists reserved in
+  // pairs (boundaries with retire chunks, next-pass segments with their
+  // strings, each class's segments with theirs): same-address atomics from
+  // every bucket, or every segment, would serialise.
+  __shared__ uint32_t s_scan[10 * (BS / kWave + 1)];
   __shared__ uint32_t s_base[6];
-  // Nine counts per thread, scanned together: boundaries, retire chunks,
-  // next-pass segments, local segments and their strings per class.
-  uint32_t sc[9] = {nb, nr, ng, nl[0], nl[1], nl[2], sl[0], sl[1], sl[2]};
-  uint32_t tot[9];
-  block_excl_scan_multi<256, 9>(sc, s_scan, tot);
+  // Ten counts per thread, scanned together: boundaries, retire chunks,
+  // next-pass segments, local segments and their strings per class, and the
+  // next-pass segments' strings.
+  uint32_t sc[10] = {nb, nr, ng, nl[0], nl[1], nl[2], sl[0], sl[1], sl[2], sg_next};
+  uint32_t tot[10];
+  block_excl_scan_multi<BS, 10>(sc, s_scan, tot);
   uint32_t ob = sc[0], orr = sc[1], og = sc[2];
   uint32_t ol[3] = {sc[3], sc[4], sc[5]};
   const uint32_t tb = tot[0], tr = tot[1], tg = tot[2], t0 = tot[3], t1 = tot[4], t2 = tot[5];
-  const uint32_t ts0 = tot[6], ts1 = tot[7], ts2 = tot[8];
+  const uint32_t ts0 = tot[6], ts1 = tot[7], ts2 = tot[8], tgs = tot[9];
   if (threadIdx.x == 0) {
     const uint32_t tl[3] = {t0, t1, t2}, ts[3] = {ts0, ts1, ts2};
-    s_base[0] = tb ? atomicAdd(&p.ctr->boundary, tb) : 0u;
-    s_base[1] = tr ? atomicAdd(&p.ctr->retire, tr) : 0u;
-    s_base[5] = tg ? atomicAdd(&p.ctr->big_next, tg) : 0u;
-    for (int c = 0; c < 3; ++c) {
-      s_base[2 + c] = tl[c] ? atomicAdd(&p.ctr->local[c], tl[c]) : 0u;
-      if (ts[c]) atomicAdd(&p.ctr->local_strings[c], ts[c]);
-    }
+    const uint64_t br = (tb || tr) ? atomic_add_pair(&p.ctr->boundary, tb, tr) : 0ull;
+    s_base[0] = (uint32_t)br;
+    s_base[1] = (uint32_t)(br >> 32);
+    s_base[5] = tg ? (uint32_t)atomic_add_pair(&p.ctr->big_next, tg, tgs) : 0u;
+    for (int c = 0; c < 3; ++c)
+      s_base[2 + c] = tl[c] ? (uint32_t)atomic_add_pair(&p.ctr->local[c].segs, tl[c], ts[c]) : 0u;
     if (s_base[0] + tb > p.cap_bounds) atomicOr(&p.ctr->error, 16u);
     if (s_base[1] + tr > p.cap_retire) atomicOr(&p.ctr->error, 8u);
     if (s_base[5] + tg > p.cap_big) atomicOr(&p.ctr->error, 4u);
@@ -828,9 +827,12 @@ __device__ __forceinline__ void segscan_seg(Plan& p, const BigSeg* segs, BigSeg*
   }
 }
 
-template <int RB>
-__global__ __launch_bounds__(256) void k_segscan(Plan p, const BigSeg* segs, BigSeg* next) {
-  if (blockIdx.x < p.ctr->big_cur) segscan_seg<RB>(p, segs, next, blockIdx.x);
+// One workgroup per segment.  A pass over one segment (the first pass of a
+// single sort) takes 1024 threads: its 2^RB buckets are then 2 per thread
+// instead of 8, and the block's latency is the pass's.
+template <int RB, int BS>
+__global__ __launch_bounds__(BS) void k_segscan(Plan p, const BigSeg* segs, BigSeg* next) {
+  if (blockIdx.x < p.ctr->big_cur) segscan_seg<RB, BS>(p, segs, next, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1111,12 +1113,19 @@ hipError_t launch_flat_scan(const Plan& p, uint64_t max_len, hipStream_t s) {
 hipError_t launch_segscan(const Plan& p, const BigSeg* segs, BigSeg* next, uint32_t grid,
                           uint32_t radix_bits, hipStream_t s) {
   if (!grid) return hipSuccess;
-  if (radix_bits == 8)
-    k_segscan<8><<<grid, 256, 0, s>>>(p, segs, next);
-  else if (radix_bits == 11)
-    k_segscan<11><<<grid, 256, 0, s>>>(p, segs, next);
-  else
+  if (radix_bits == 8) {
+    if (grid == 1)
+      k_segscan<8, 256><<<1, 256, 0, s>>>(p, segs, next);
+    else
+      k_segscan<8, 256><<<grid, 256, 0, s>>>(p, segs, next);
+  } else if (radix_bits == 11) {
+    if (grid == 1)
+      k_segscan<11, 1024><<<1, 1024, 0, s>>>(p, segs, next);
+    else
+      k_segscan<11, 256><<<grid, 256, 0, s>>>(p, segs, next);
+  } else {
     return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

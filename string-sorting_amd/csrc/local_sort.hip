@@ -486,7 +486,7 @@ __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, u
   const bool bounded = (first & 0x80000000u) != 0;
   first &= 0x7FFFFFFFu;
   const LocalSeg sg = p.local[kClass][first + blockIdx.x];
-  if (bounded && first + blockIdx.x >= min(p.ctr->local[kClass], p.cap_local[kClass])) return;
+  if (bounded && first + blockIdx.x >= min(p.ctr->local[kClass].segs, p.cap_local[kClass])) return;
   const uint32_t n = sg.size;
   const uint32_t gb = sg.begin;
   const uint32_t* __restrict__ src = p.idx[sg.buf];

@@ -69,7 +69,8 @@ struct Pending {
 
 std::mutex g_mu;  // tuning, profile totals, default contexts
 gss_tuning g_tuning = {6144u, 11u, kTile, 64u};
-std::atomic<bool> g_prof{false};    // HIP event brackets
+std::atomic<bool> g_prof{false};    // HIP event brackets around every kernel and pass
+std::atomic<bool> g_prof_pass{false};  // around passes only
 std::atomic<bool> g_levels{false};  // counting variant of the local kernels
 std::map<std::string, ProfTotal> g_prof_totals;
 thread_local std::string g_last_error;
@@ -191,9 +192,14 @@ struct ProfToken {
   uint64_t items = 0;
 };
 
-ProfToken prof_begin(Context& c, const char* name, uint64_t items, hipStream_t stream) {
+// A pass-level bracket (a global pass, the local phase) is recorded in both
+// profiling modes, a kernel's only with every kernel bracketed.
+ProfToken prof_begin(Context& c, const char* name, uint64_t items, hipStream_t stream,
+                     bool pass_level = false) {
   ProfToken t;
-  if (!g_prof.load(std::memory_order_relaxed)) return t;
+  if (!g_prof.load(std::memory_order_relaxed) &&
+      !(pass_level && g_prof_pass.load(std::memory_order_relaxed)))
+    return t;
   t.a = take_event(c);
   t.name = name;
   t.items = items;
@@ -458,8 +464,8 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
       init_retire[h0.retire++] = Retire{b, 1, b, 0, 0, 1};
     } else if (size <= T) {
       const int cls = local_class(size);
-      init[cls][h0.local[cls]++] = LocalSeg{b, size, 0, 0};
-      h0.local_strings[cls] += size;
+      init[cls][h0.local[cls].segs++] = LocalSeg{b, size, 0, 0};
+      h0.local[cls].strings += size;
     } else {
       if (nbig >= p.cap_big) return fail(GSS_E_INTERNAL, "segment table overflow");
       const uint32_t first = tiles0;
@@ -492,7 +498,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     return GSS_OK;
   };
   for (int cls = 0; cls < 3; ++cls)
-    if ((rc = add(init[cls], p.local[cls], 0, sizeof(LocalSeg) * h0.local[cls]))) return rc;
+    if ((rc = add(init[cls], p.local[cls], 0, sizeof(LocalSeg) * h0.local[cls].segs))) return rc;
   if ((rc = add(init_retire, p.retire, 0, sizeof(Retire) * h0.retire))) return rc;
   memcpy(w.h_init + w.init_ctr_off, &h0, sizeof(Counters));
   if ((rc = add(w.h_init, p.lstats, w.init_ctr_off, w.init_big_off + sizeof(BigSeg) * nbig)))
@@ -532,9 +538,10 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     }
     return GSS_OK;
   };
-  if (nbig && (start.local[0] || start.local[1] || start.local[2])) {
-    if ((rc = launch_early(start.local))) return rc;
-    for (int cls = 0; cls < 3; ++cls) early_strings += start.local_strings[cls];
+  if (nbig && (start.local[0].segs || start.local[1].segs || start.local[2].segs)) {
+    const uint32_t upto[3] = {start.local[0].segs, start.local[1].segs, start.local[2].segs};
+    if ((rc = launch_early(upto))) return rc;
+    for (int cls = 0; cls < 3; ++cls) early_strings += start.local[cls].strings;
   }
 
   // Launch bounds of the passes.  The first pass's segments and tiles are
@@ -600,11 +607,11 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   // made the sort 19% slower (one local workgroup fills a CU's LDS).
   auto maybe_launch_early = [&](uint64_t next_strings) -> int {
     uint64_t waiting = 0;
-    for (int cls = 0; cls < 3; ++cls) waiting += w.h_ctr->local_strings[cls];
+    for (int cls = 0; cls < 3; ++cls) waiting += w.h_ctr->local[cls].strings;
     waiting -= early_strings;
     if (!waiting || 2 * waiting < next_strings) return GSS_OK;
     uint32_t upto[3];
-    for (int cls = 0; cls < 3; ++cls) upto[cls] = std::min(w.h_ctr->local[cls], p.cap_local[cls]);
+    for (int cls = 0; cls < 3; ++cls) upto[cls] = std::min(w.h_ctr->local[cls].segs, p.cap_local[cls]);
     const int e = launch_early(upto);
     if (e) return e;
     early_strings += waiting;
@@ -619,7 +626,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     const uint64_t bound_segs = pass == 0 ? nbig : ub_segs;
     const BigSeg* segs = p.big[cur];
     BigSeg* next = p.big[cur ^ 1];
-    ProfToken pass_tok = prof_begin(c, kPassNames[pass < 9 ? pass : 9], 0, st);
+    ProfToken pass_tok = prof_begin(c, kPassNames[pass < 9 ? pass : 9], 0, st, true);
     if (pass > 0)  // the first pass's tables came with the start of the sort
       GSS_LAUNCH("plan", 0,
                  launch_plan(p, segs, pass, (uint32_t)bound_tiles, (uint32_t)bound_segs, st));
@@ -695,7 +702,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
           const uint64_t want = std::min<uint64_t>(
               ub, (uint64_t)launched[2] + c.hint_largest + c.hint_largest / 8 + 64);
           if (want > launched[2]) {
-            phase_tok = prof_begin(c, "local_phase", 0, st);
+            phase_tok = prof_begin(c, "local_phase", 0, st, true);
             phase_open = true;
             spec_grid = (uint32_t)(want - launched[2]);
             const size_t mark = c.pending.size();
@@ -708,7 +715,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
         if ((rc = collect())) return rc;
         if (spec_grid) {
           // It took the segments the counters show now, up to its grid.
-          const uint32_t now = std::min(w.h_ctr->local[2], p.cap_local[2]);
+          const uint32_t now = std::min(w.h_ctr->local[2].segs, p.cap_local[2]);
           const uint32_t took = std::min(now, launched[2] + spec_grid);
           largest_main += took - std::min(took, launched[2]);
           launched[2] = std::max(launched[2], took);
@@ -727,8 +734,8 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   const uint32_t nretire = std::min(w.h_ctr->retire, p.cap_retire);
   uint32_t nlocal[3];
   for (int cls = 0; cls < 3; ++cls)
-    nlocal[cls] = std::min(w.h_ctr->local[cls], p.cap_local[cls]) - launched[cls];
-  if (!phase_open) phase_tok = prof_begin(c, "local_phase", 0, st);
+    nlocal[cls] = std::min(w.h_ctr->local[cls].segs, p.cap_local[cls]) - launched[cls];
+  if (!phase_open) phase_tok = prof_begin(c, "local_phase", 0, st, true);
   if (nretire) GSS_LAUNCH("retire", nretire, launch_retire(p, std::min<uint32_t>(nretire, 8192), st));
   for (int cls = 2; cls >= 0; --cls) {
     if (!nlocal[cls]) continue;
@@ -760,8 +767,8 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     c.hint_largest = largest_main + nlocal[2];
   }
   for (int cls = 0; cls < 3; ++cls) {
-    stats.local_segments += w.h_ctr->local[cls];
-    stats.local_strings += w.h_ctr->local_strings[cls];
+    stats.local_segments += w.h_ctr->local[cls].segs;
+    stats.local_strings += w.h_ctr->local[cls].strings;
   }
   // A local launch's unit of work is one active string in one key round;
   // block 0 of the statistics holds the main stream's launches, block 1 the
@@ -996,9 +1003,10 @@ int gss_get_stats(gss_stats* out) {
 }
 
 int gss_profile_enable(int mode) {
-  if (mode & ~(GSS_PROFILE_EVENTS | GSS_PROFILE_LEVELS))
+  if (mode & ~(GSS_PROFILE_EVENTS | GSS_PROFILE_LEVELS | GSS_PROFILE_PASSES))
     return fail(GSS_E_INVALID, "unknown profile mode bits");
   g_prof.store((mode & GSS_PROFILE_EVENTS) != 0);
+  g_prof_pass.store((mode & GSS_PROFILE_PASSES) != 0);
   g_levels.store((mode & GSS_PROFILE_LEVELS) != 0);
   return GSS_OK;
 }

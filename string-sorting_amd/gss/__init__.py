@@ -582,11 +582,14 @@ def stats() -> Stats:
     return s
 
 
-def profile_enable(on: bool = True, levels: bool = False) -> None:
-    """Kernel timing (HIP event brackets) on or off; `levels` runs the
-    local kernels' counting variant, which fills Stats.local_level_strings."""
-    _check(lib().gss_profile_enable((1 if on else 0) | (2 if levels else 0)),
-           "gss_profile_enable")
+def profile_enable(on: bool = True, levels: bool = False, passes_only: bool = False) -> None:
+    """Kernel timing (HIP event brackets around every launch and pass) on or
+    off; `passes_only` brackets only the passes and the local phase (an event
+    between two kernels delays the second, so pass times are read this way);
+    `levels` runs the local kernels' counting variant, which fills
+    Stats.local_level_strings."""
+    mode = (1 if on and not passes_only else 0) | (4 if on and passes_only else 0)
+    _check(lib().gss_profile_enable(mode | (2 if levels else 0)), "gss_profile_enable")
 
 
 def profile_reset() -> None:

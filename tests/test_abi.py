@@ -111,9 +111,11 @@ def test_stats_and_profile_without_device():
     gss.profile_reset()
     assert gss.profile_get("scatter") == (0, 0.0, 0)
     lib = gss.lib()
-    assert lib.gss_profile_enable(4) == -1  # GSS_E_INVALID: unknown mode bit
+    assert lib.gss_profile_enable(8) == -1  # GSS_E_INVALID: unknown mode bit
     assert b"mode" in lib.gss_last_error()
+    assert lib.gss_profile_enable(4) == 0  # GSS_PROFILE_PASSES
     gss.profile_enable(True, levels=True)
+    gss.profile_enable(True, passes_only=True)
     gss.profile_enable(False)
 
 
