@@ -475,9 +475,10 @@ __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, u
   const uint32_t lane = lane_id();
   const uint64_t lt = lanemask_lt();
   unsigned long long round_strings = 0;
-  // 8-byte key levels this wave's strings advanced through: a round's group
-  // that stops at key e consumed e + 1 of them, a skipping group KW.  Built
-  // from ballots, so the count is wave-uniform (scalar registers).
+  // Bytes this wave's strings advanced through (eight per 8-byte key level):
+  // a round's group that stops at key e consumed e + 1 levels, a skipping
+  // group KW, a first round the bytes to the end of its key.  Built from
+  // ballots, so the count is wave-uniform (scalar registers).
   unsigned long long wave_levels = 0;
   // A launch with the top bit of `first` set was sized before the host knew
   // the class's segment count (the largest class, launched right after the
@@ -523,16 +524,18 @@ __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, u
       //    NUL.  The group's LCP is G = min l_i (the LCP of a set is the
       //    minimum over its members' LCPs with any one of them, and a NUL
       //    ends every comparison), so the group's first word that differs
-      //    or holds a NUL is e = G / 8 -- the word the later rounds' stop
-      //    rule picks.  A string keeps only l_i and its own word at l_i / 8:
-      //    its word e is that one when l_i / 8 == e, else it equals the
-      //    reference's word e.  A window with no mismatch and no NUL moves
-      //    the group PW words deeper.
+      //    or holds a NUL is at byte G.  The round sorts by the 8 bytes at G
+      //    (at most PW * 8 - 8, so that the window holds them), which all
+      //    start where the group first differs: the aligned word holding G
+      //    would leave only 8 - G % 8 varying bytes in the key.  A string
+      //    keeps only l_i and its own 8 bytes at l_i: its bytes [G, l_i) are
+      //    the reference's, the rest its own.  A window with no mismatch and
+      //    no NUL moves the group PW words deeper.
 #pragma unroll
       for (int j = 0; j < IPT; ++j)
         gv[j] = (uint32_t)j < jn && base + j * kWave < nact ? 0u : 0xFFFFu;
       wave_big = n > insertion_max(CAP);
-      // l_i and the string's word at l_i / 8 wait in s.sid and s.key (sid
+      // l_i and the string's 8 bytes at l_i wait in s.sid and s.key (sid
       // is 0 for every string of a first round and is restored below).
       // The reference window passes through LDS (registers are this class's
       // limit).
@@ -559,8 +562,10 @@ __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, u
             for (int q = PW - 1; q >= 0; --q) {  // the first stop wins
               const uint64_t r = s.refw[q];  // the same word for every lane
               if (w[q] != r || (w[q] & 0xFFu) == 0) {
-                l = 8 * q + (w[q] != r ? key_lcp(w[q], r) : key_len(w[q]));
-                o = w[q];
+                const uint32_t lb = w[q] != r ? key_lcp(w[q], r) : key_len(w[q]);  // 0..7
+                const uint64_t after = q + 1 < PW ? w[q + 1 < PW ? q + 1 : q] : 0ull;
+                l = 8 * q + lb;
+                o = lb ? (w[q] << (8 * lb)) | (after >> (64 - 8 * lb)) : w[q];
               }
             }
             s.sid[i] = (uint16_t)l;
@@ -579,7 +584,7 @@ __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, u
         if (COUNT) {
 #pragma unroll
           for (int j = 0; j < IPT; ++j)
-            if ((uint32_t)j < jn) wave_levels += PW * (uint64_t)__popcll(__ballot(gv[j] != 0xFFFFu));
+            if ((uint32_t)j < jn) wave_levels += 8 * PW * (uint64_t)__popcll(__ballot(gv[j] != 0xFFFFu));
         }
         depth += 8 * PW;
         ++rounds;
@@ -591,20 +596,29 @@ __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, u
         if (threadIdx.x == 0) atomicOr(&p.ctr->error, 32u);
         break;
       }
-      const uint32_t e = G / 8;
-      const uint64_t ref_e = s.refw[e];
+      const uint32_t gk = min(G, 8u * PW - 8);  // the key's byte in the window
+      const uint32_t qk = gk / 8, rk = gk % 8;
+      const uint64_t ref_k =
+          rk ? (s.refw[qk] << (8 * rk)) | (s.refw[qk + 1 < PW ? qk + 1 : qk] >> (64 - 8 * rk))
+             : s.refw[qk];
 #pragma unroll
       for (int j = 0; j < IPT; ++j) {
         if (gv[j] != 0xFFFFu) {
           const uint32_t i = base + j * kWave;
-          if (s.sid[i] / 8u > e) s.key[i] = ref_e;
+          const uint32_t l = s.sid[i];
+          if (l >= gk + 8) {
+            s.key[i] = ref_k;
+          } else if (l > gk) {  // l - gk reference bytes, then its own
+            const uint32_t m = l - gk;
+            s.key[i] = (ref_k & (~0ull << (64 - 8 * m))) | (s.key[i] >> (8 * m));
+          }  // l == gk: its own 8 bytes
           s.sid[i] = 0;
         }
         if (COUNT && (uint32_t)j < jn)
-          wave_levels += (e + 1) * (uint64_t)__popcll(__ballot(gv[j] != 0xFFFFu));
+          wave_levels += (gk + 8) * (uint64_t)__popcll(__ballot(gv[j] != 0xFFFFu));
       }
       if (threadIdx.x == 0) {
-        s.ss_depth[0] = depth + 8 * e;  // the chosen word is key 0 at this depth
+        s.ss_depth[0] = depth + gk;  // the key is key 0 at this depth
         s.ss_stop[0] = 1;
       }
       __syncthreads();
@@ -678,7 +692,7 @@ __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, u
       if (COUNT) {
 #pragma unroll
         for (int j = 0; j < IPT; ++j)
-          if ((uint32_t)j < jn) wave_levels += KW * (uint64_t)__popcll(__ballot(gv[j] != 0xFFFFu));
+          if ((uint32_t)j < jn) wave_levels += 8 * KW * (uint64_t)__popcll(__ballot(gv[j] != 0xFFFFu));
       }
       for (uint32_t g = threadIdx.x; g < nss; g += BLOCK) s.ss_depth[g] += 8 * KW;
       __syncthreads();
@@ -709,7 +723,7 @@ __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, u
         if (COUNT) {
 #pragma unroll
           for (int q = 1; q <= KW; ++q)
-            wave_levels += (uint64_t)__popcll(__ballot(lv >= (uint32_t)q));
+            wave_levels += 8 * (uint64_t)__popcll(__ballot(lv >= (uint32_t)q));
         }
       }
     }

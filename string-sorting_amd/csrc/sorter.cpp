@@ -785,6 +785,10 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
       }
       rounds_max = std::max<uint64_t>(rounds_max, w.h_lstats[b].v[slot][3]);
     }
+  // The local kernels count the bytes their strings advanced through: eight
+  // per 8-byte key level (a first round's key starts at any byte).
+  for (int b = 0; b < 2; ++b)
+    for (int cls = 0; cls < 3; ++cls) level_strings[b][cls] /= 8;
   uint64_t final_level_strings = 0;
   for (int cls = 0; cls < 3; ++cls) {
     stats.local_round_strings += round_strings[0][cls] + round_strings[1][cls];
