@@ -730,6 +730,15 @@ __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, u
     __syncthreads();
     }  // rounds after the first
 
+    // A segment's first round (largest class) keys every string at the
+    // group's first differing byte, so the key's top byte varies: it counts
+    // by bytes 7 and 6 at once, without the scan for varying bytes below
+    // (that scan and the other sorts remain for the cases it declines).
+    int pre_first = 2;  // 2: not tried
+    if constexpr (kClass == 2) {
+      if (rounds == 1) pre_first = local_first_count<BLOCK, CAP>(s, nact, base, jn, 0xC0u);
+    }
+    if (pre_first != 1) {
     // 4. Bytes of the chosen keys that vary inside some sub-segment.
     uint32_t vm = 0;
 #pragma unroll
@@ -811,7 +820,13 @@ __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, u
         // the two-byte passes; the LSD passes when a bucket is too large.
         int first = 0;
         if constexpr (kClass == 2) {
-          if (nss == 1 && (vm & (vm - 1u)))
+          // The two most significant varying bytes (the digit's bytes).
+          const uint32_t hb = 1u << (31 - __builtin_clz(vm));
+          const uint32_t rest = vm & ~hb;
+          const uint32_t top2 = rest ? hb | (1u << (31 - __builtin_clz(rest))) : hb;
+          if (pre_first != 2 && top2 == 0xC0u)
+            first = pre_first;  // the same digit bytes: the same answer
+          else if (nss == 1 && (vm & (vm - 1u)))
             first = local_first_count<BLOCK, CAP>(s, nact, base, jn, vm);
           if (first == 0 && nss == 1 &&
               local_two_byte_count<BLOCK, CAP>(s, nact, ipw, base, jn, vm, kCountMax))
@@ -825,6 +840,7 @@ __global__ __launch_bounds__(BLOCK, local_min_waves(CAP)) void k_local(Plan p, u
         }
       }
     }
+    }  // pre_first != 1
 
     // 6. Split into equal-key groups; retire singletons and groups whose key
     //    holds the NUL; emit the LCP of every pair split here.  Survivors are
