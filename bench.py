@@ -75,7 +75,11 @@ def measured_traffic(kernel):
     sym = SYMBOLS.get(kernel)
     if not sym or not PROFILE.exists():
         return None
-    for line in PROFILE.read_text().splitlines():
+    text = PROFILE.read_text()
+    at = text.find("## HBM traffic")
+    if at < 0:
+        return None
+    for line in text[at:].splitlines():
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
         if len(cells) == 4 and sym in cells[0]:
             try:
