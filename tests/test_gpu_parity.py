@@ -149,6 +149,36 @@ def test_deep_shared_prefix(plen, oracle_mod):
     assert_matches_oracle(ss, off, lcp, oracle_mod)
 
 
+def _first_round_set(n, plen, seed):
+    """n strings sharing a prefix of plen bytes, then 0-12 bytes over a
+    4-letter alphabet: some end at plen (their NUL is the first differing
+    byte), many are duplicates, and with n <= 6144 the whole set is one
+    segment of the largest local class at depth 0."""
+    rng = np.random.default_rng(seed)
+    pre = bytes(rng.integers(97, 123, size=plen).astype(np.uint8))
+    out = []
+    for _ in range(n):
+        k = int(rng.integers(0, 13))
+        out.append(pre + bytes(rng.integers(97, 101, size=k).astype(np.uint8)))
+    return out
+
+
+@pytest.mark.parametrize("plen,key_bytes", [(0, 64), (5, 64), (50, 64), (56, 64), (57, 64),
+                                            (60, 64), (63, 64), (64, 64), (65, 64), (121, 64),
+                                            (20, 32), (24, 32), (25, 32), (31, 32), (33, 32)])
+def test_first_round_key_position(plen, key_bytes, oracle_mod):
+    """The largest class's first round sorts by the 8 bytes at the group's
+    first differing byte G, capped where the window still holds all 8
+    (56 for a 64-byte window, 24 for 32); past the window it probes again.
+    Shared prefixes around those bounds, strings ending at G and duplicates
+    sort offset for offset as msd_CE2, LCPs included."""
+    gss.set_tuning(key_bytes=key_bytes)
+    ss = _set(_first_round_set(5000, plen, 1000 + plen), f"first{plen}_{key_bytes}")
+    log(f"first-round key position plen={plen} key_bytes={key_bytes}")
+    off, lcp = gpu_sort(ss)
+    assert_matches_oracle(ss, off, lcp, oracle_mod)
+
+
 @pytest.mark.parametrize("threshold,radix_bits,tile,key_bytes",
                          [(64, 8, 4096, 64), (64, 11, 2048, 32), (1024, 8, 2048, 64),
                           (1024, 11, 4096, 32), (6144, 8, 4096, 32), (6144, 11, 4096, 64),
