@@ -264,11 +264,17 @@ int gss_merge_lcp_device32_noalloc(const unsigned char *d_text, uint64_t text_by
  * (run_len[r]-1 entries); the three arrays are host arrays of k entries.
  * Equal strings come out in run order (run 0 first), as merge_lcp_3way's
  * ties do, so merging the stable sorts of k parts gives the stable sort of
- * their concatenation.  Adjacent runs merge pairwise in rounds (a balanced
- * tree of the 2-way merge above, all queued on the stream) through two
- * buffers of the total size (24 bytes per string) in the library's
- * per-device scratch, which grows as needed and is kept.  d_out receives the
- * merged offsets, d_out_lcp (optional) their LCP array. */
+ * their concatenation.  Three to 64 runs (under 2^31 strings in all) merge
+ * by key groups: each run's strings sharing their first 8 bytes (found from
+ * its LCP array) form a group keyed by those bytes, the groups of all runs
+ * merge stably as integers, and only the output spans where groups of
+ * different runs share a key are sorted, in place, from byte 8 (the batch
+ * sort's kernels).  Two runs, or more than 64, merge pairwise in rounds (a
+ * balanced tree of the 2-way merge above, all queued on the stream).  Both
+ * use the library's per-device scratch (about 44 and 24 bytes per string),
+ * which grows as needed and is kept; the key-group merge also sizes the
+ * default context's workspace for the total.  d_out receives the merged
+ * offsets, d_out_lcp (optional) their LCP array. */
 int gss_merge_lcp_kway_device32(const unsigned char *d_text, uint64_t text_bytes,
 				const uint32_t *const *d_runs,
 				const uint64_t *const *d_run_lcps,

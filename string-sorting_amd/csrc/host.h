@@ -18,6 +18,18 @@ int fail(int code, const std::string& msg);
 int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
                   uint32_t* out, uint64_t* lcp, uint64_t n, hipStream_t stream);
 
+// Request depth of sort_spans_device32 for a request left as it is.
+constexpr uint32_t kSpanKeep = 0xFFFFFFFFu;
+
+// Sorts, in place on the default context, the requests [req[r], req[r+1])
+// of `off` (n = req[nreq] offsets) whose strings all share their first
+// req_depth[r] bytes, starting there; requests with depth kSpanKeep keep
+// their offsets and LCPs.  LCPs are written for the pairs inside each
+// sorted request only.
+int sort_spans_device32(const unsigned char* text, uint64_t text_bytes, uint32_t* off,
+                        uint64_t* lcp, const uint64_t* req, const uint32_t* req_depth,
+                        uint32_t nreq, hipStream_t stream);
+
 // Waits for everything queued on `st` by polling it.  A blocking wait
 // sleeps once the stream has been busy for a while and wakes tens of
 // microseconds after the GPU finished, which a request's latency would pay.

@@ -418,8 +418,8 @@ struct MergeBatch {
 
 // The pair whose range [first(q), first(q) + size(q)) holds g (the first
 // pair's range starts at 0 and they follow each other).
-template <typename F>
-__device__ __forceinline__ uint32_t merge_pair_of(const MergeBatch& mb, uint64_t g, F first) {
+template <typename Batch, typename F>
+__device__ __forceinline__ uint32_t merge_pair_of(const Batch& mb, uint64_t g, F first) {
   uint32_t q = 0;
   while (q + 1 < mb.count && first(mb.p[q + 1]) <= g) ++q;
   return q;
@@ -802,6 +802,393 @@ uint32_t grid_cap(uint64_t items, uint32_t block, uint32_t cap = 65536) {
   return (uint32_t)(g > cap ? cap : g);
 }
 
+// ---------------------------------------------------------------------------
+// K-way merge by 8-byte key groups.  In a sorted run the strings that share
+// their first 8 bytes, none of them NUL, lie together, and the run's LCP
+// array marks them: a group starts where the LCP with the previous string is
+// below 8.  Only a group's first string is read, for its key (the 8 bytes
+// big-endian, zero from a NUL on).  The groups of all runs merge by key as
+// integers, stably (on equal keys the earlier run's group first); then each
+// group's strings go out as they lie in their run, with the run's LCPs
+// inside the group and the LCP of the two keys across groups.  Equal keys
+// of groups from different runs (a tie) belong to strings that share 8
+// bytes: each tie's span of the output is sorted in place from byte 8, by a
+// stable sort (equal strings keep run order), and the rest is final.  Equal
+// keys holding a NUL are equal strings, final in run order.  Strings are
+// read where the merge needs them: once per group, and again only inside a
+// tie, where 2-way rounds would compare every string log2(k) times.
+
+// A run of a k-way merge: offsets, LCP array (len - 1 entries), length,
+// and its first string's index in the concatenation of the runs.
+struct KwRun {
+  const uint32_t* off;
+  const uint64_t* lcp;
+  uint64_t len;
+  uint64_t base;
+};
+constexpr uint32_t kKwRunsMax = 64;  // the run table sits in LDS
+constexpr int kKwThreads = 256;
+constexpr int kKwItems = 16;
+constexpr uint64_t kKwBlock = (uint64_t)kKwThreads * kKwItems;  // strings per block
+
+// The run holding concatenated index g: the last r with base[r] <= g (the
+// runs are nonempty, so their bases ascend).
+__device__ __forceinline__ uint32_t kw_run_of(const uint64_t* s_base, uint32_t k, uint64_t g) {
+  uint32_t lo = 0, hi = k - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (s_base[mid] <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Group starts of the concatenated runs, block b covering strings
+// [b * kKwBlock, (b + 1) * kKwBlock).  Without EMIT: bsum[b] = the block's
+// group count.  With EMIT (bsum scanned): group h (in concatenated order)
+// gets hgid[h] = its first string's index, hkey[h] = its key; runhead[r] =
+// run r's first group; hgid[H] = n and runhead[k] = H for the H groups.
+template <bool EMIT>
+__global__ __launch_bounds__(kKwThreads) void k_kw_heads(
+    const KwRun* __restrict__ runs, uint32_t k, uint64_t n, const uint64_t* __restrict__ t64,
+    uint64_t tlen, uint64_t* __restrict__ bsum, uint32_t* __restrict__ hgid,
+    uint64_t* __restrict__ hkey, uint32_t* __restrict__ runhead) {
+  __shared__ KwRun s_runs[kKwRunsMax];
+  __shared__ uint64_t s_base[kKwRunsMax + 1];
+  __shared__ uint8_t s_flag[kKwBlock];
+  __shared__ uint64_t scratch[kKwThreads / kWave + 1];
+  for (uint32_t r = threadIdx.x; r < k; r += kKwThreads) {
+    s_runs[r] = runs[r];
+    s_base[r] = runs[r].base;
+  }
+  if (threadIdx.x == 0) s_base[k] = n;
+  __syncthreads();
+  const uint64_t b0 = (uint64_t)blockIdx.x * kKwBlock;
+  // Flags in LDS from coalesced reads of the LCP arrays.
+#pragma unroll 4
+  for (int q = 0; q < kKwItems; ++q) {
+    const uint64_t x = (uint64_t)q * kKwThreads + threadIdx.x;
+    const uint64_t g = b0 + x;
+    uint8_t f = 0;
+    if (g < n) {
+      const uint32_t r = kw_run_of(s_base, k, g);
+      const uint64_t i = g - s_base[r];
+      f = (i == 0 || s_runs[r].lcp[i - 1] < 8) ? 1 : 0;
+    }
+    s_flag[x] = f;
+  }
+  __syncthreads();
+  // Thread t owns items [t * kKwItems, (t + 1) * kKwItems) of the block.
+  const uint32_t x0 = threadIdx.x * kKwItems;
+  uint32_t bits = 0;
+#pragma unroll
+  for (int q = 0; q < kKwItems; ++q) bits |= (uint32_t)s_flag[x0 + q] << q;
+  uint64_t total;
+  const uint64_t ex = block_excl_scan64<kKwThreads>((uint64_t)__popc(bits), scratch, &total);
+  if (!EMIT) {
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+    return;
+  }
+  uint64_t h = bsum[blockIdx.x] + ex;
+  for (int q = 0; q < kKwItems; ++q) {
+    const uint64_t g = b0 + x0 + q;
+    if (g >= n) break;
+    if ((bits >> q) & 1u) {
+      const uint32_t r = kw_run_of(s_base, k, g);
+      const uint64_t i = g - s_base[r];
+      hgid[h] = (uint32_t)g;
+      hkey[h] = load_key(t64, tlen, s_runs[r].off[i]);
+      if (i == 0) runhead[r] = (uint32_t)h;
+      ++h;
+    }
+    if (g + 1 == n) {
+      hgid[h] = (uint32_t)n;
+      runhead[k] = (uint32_t)h;
+    }
+  }
+}
+
+// hval[h] = group h's first string index | its size << 32.
+__global__ __launch_bounds__(256) void k_kw_pack(const uint32_t* __restrict__ hgid, uint64_t ngroups,
+                                                uint64_t* __restrict__ hval) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < ngroups; h += stride)
+    hval[h] = (uint64_t)hgid[h] | ((uint64_t)(hgid[h + 1] - hgid[h]) << 32);
+}
+
+// The runs' group lists merged in one launch when they are few: group h
+// of run r goes to its index in r plus, in every other run, the groups with
+// a smaller key (in an earlier run, also those with an equal key: equal
+// keys keep run order).  Its value is its first string's index | its size
+// << 32.
+__global__ __launch_bounds__(256) void k_kw_rank(const uint64_t* __restrict__ hkey,
+                                                const uint32_t* __restrict__ hgid,
+                                                const uint32_t* __restrict__ runhead, uint32_t k,
+                                                uint64_t ngroups, uint64_t* __restrict__ mkey,
+                                                uint64_t* __restrict__ mval) {
+  __shared__ uint32_t s_rh[kKwRunsMax + 1];
+  for (uint32_t r = threadIdx.x; r <= k; r += 256) s_rh[r] = runhead[r];
+  __syncthreads();
+  const uint64_t h = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (h >= ngroups) return;
+  uint32_t lo = 0, hi = k - 1;  // the last run whose first group is at or before h
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (s_rh[mid] <= h)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const uint32_t r = lo;
+  const uint64_t x = hkey[h];
+  uint64_t rank = h - s_rh[r];
+  for (uint32_t q = 0; q < k; ++q) {
+    if (q == r) continue;
+    uint32_t a = s_rh[q], e = s_rh[q + 1];
+    while (a < e) {
+      const uint32_t mid = (a + e) >> 1;
+      const uint64_t y = hkey[mid];
+      if (q < r ? y <= x : y < x)
+        a = mid + 1;
+      else
+        e = mid;
+    }
+    rank += a - s_rh[q];
+  }
+  mkey[rank] = x;
+  mval[rank] = (uint64_t)hgid[h] | ((uint64_t)(hgid[h + 1] - hgid[h]) << 32);
+}
+
+// The runs' offsets side by side: cat[g] for every concatenated index g.
+__global__ __launch_bounds__(kKwThreads) void k_kw_concat(const KwRun* __restrict__ runs,
+                                                         uint32_t k, uint64_t n,
+                                                         uint32_t* __restrict__ cat) {
+  __shared__ uint64_t s_base[kKwRunsMax + 1];
+  __shared__ const uint32_t* s_off[kKwRunsMax];
+  for (uint32_t r = threadIdx.x; r < k; r += kKwThreads) {
+    s_base[r] = runs[r].base;
+    s_off[r] = runs[r].off;
+  }
+  if (threadIdx.x == 0) s_base[k] = n;
+  __syncthreads();
+  const uint64_t b0 = (uint64_t)blockIdx.x * kKwBlock;
+  for (int q = 0; q < kKwItems; ++q) {
+    const uint64_t g = b0 + (uint64_t)q * kKwThreads + threadIdx.x;
+    if (g >= n) break;
+    const uint32_t r = kw_run_of(s_base, k, g);
+    cat[g] = s_off[r][g - s_base[r]];
+  }
+}
+
+// Stable 2-way merges of (key, value) lists by key, several per launch:
+// merge-path splits every kKwmTile outputs, then one block per tile merges
+// its share through LDS.
+constexpr int kKwmThreads = 256;
+constexpr int kKwmItems = 8;
+constexpr uint64_t kKwmTile = (uint64_t)kKwmThreads * kKwmItems;
+struct KwPair {
+  const uint64_t* ak;
+  const uint64_t* av;
+  uint64_t na;
+  const uint64_t* bk;
+  const uint64_t* bv;
+  uint64_t nb;
+  uint64_t* ok;
+  uint64_t* ov;
+  uint64_t split0;  // first split entry (tiles + 1 of them)
+  uint64_t tile0;   // first tile
+};
+struct KwBatch {
+  KwPair p[kMergePairsMax];
+  uint32_t count;
+  uint64_t splits, tiles;
+};
+
+// The number of A items among the first d outputs: the smallest i in
+// [lo, hi] with a[i] > b[d - 1 - i] (equal keys: A first).
+__device__ __forceinline__ uint64_t kw_path(const uint64_t* __restrict__ a,
+                                            const uint64_t* __restrict__ b, uint64_t d,
+                                            uint64_t lo, uint64_t hi) {
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] <= b[d - 1 - mid])
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_kw_mpart(const KwBatch mb, uint64_t* __restrict__ split) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= mb.splits) return;
+  const KwPair& mp = mb.p[merge_pair_of(mb, g, [](const KwPair& x) { return x.split0; })];
+  const uint64_t p = g - mp.split0;
+  const uint64_t total = mp.na + mp.nb;
+  const uint64_t d = p * kKwmTile < total ? p * kKwmTile : total;
+  const uint64_t lo = d > mp.nb ? d - mp.nb : 0, hi = d < mp.na ? d : mp.na;
+  split[g] = kw_path(mp.ak, mp.bk, d, lo, hi);
+}
+
+__global__ __launch_bounds__(kKwmThreads) void k_kw_merge(const KwBatch mb,
+                                                         const uint64_t* __restrict__ split_all) {
+  const KwPair& mp = mb.p[merge_pair_of(mb, blockIdx.x, [](const KwPair& x) { return x.tile0; })];
+  const uint64_t tile = blockIdx.x - mp.tile0;
+  const uint64_t* __restrict__ split = split_all + mp.split0;
+  __shared__ uint64_t s_k[kKwmTile];
+  __shared__ uint64_t s_v[kKwmTile];
+  const uint64_t total = mp.na + mp.nb;
+  const uint64_t k0 = tile * kKwmTile;
+  const uint64_t k1 = k0 + kKwmTile < total ? k0 + kKwmTile : total;
+  const uint64_t i0 = split[tile], i1 = split[tile + 1];
+  const uint64_t j0 = k0 - i0;
+  const uint32_t ia = (uint32_t)(i1 - i0), m = (uint32_t)(k1 - k0), jb = m - ia;
+  for (uint32_t x = threadIdx.x; x < m; x += kKwmThreads) {
+    if (x < ia) {
+      s_k[x] = mp.ak[i0 + x];
+      s_v[x] = mp.av[i0 + x];
+    } else {
+      s_k[x] = mp.bk[j0 + (x - ia)];
+      s_v[x] = mp.bv[j0 + (x - ia)];
+    }
+  }
+  __syncthreads();
+  const uint64_t* sa = s_k;
+  const uint64_t* sb = s_k + ia;
+  const uint32_t dd = threadIdx.x * kKwmItems;
+  uint64_t ok[kKwmItems], ov[kKwmItems];
+  if (dd < m) {
+    uint32_t ai = (uint32_t)kw_path(sa, sb, dd, dd > jb ? dd - jb : 0, dd < ia ? dd : ia);
+    uint32_t bj = dd - ai;
+#pragma unroll
+    for (int q = 0; q < kKwmItems; ++q) {
+      const bool take_a = ai < ia && (bj >= jb || sa[ai] <= sb[bj]);
+      const uint32_t src = take_a ? ai : ia + bj;
+      ok[q] = s_k[src < m ? src : 0];
+      ov[q] = s_v[src < m ? src : 0];
+      ai += take_a ? 1u : 0u;
+      bj += take_a ? 0u : 1u;
+    }
+  }
+  __syncthreads();
+  if (dd < m) {
+#pragma unroll
+    for (int q = 0; q < kKwmItems; ++q) {
+      if (dd + q < m) {
+        s_k[dd + q] = ok[q];
+        s_v[dd + q] = ov[q];
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t x = threadIdx.x; x < m; x += kKwmThreads) {
+    mp.ok[k0 + x] = s_k[x];
+    mp.ov[k0 + x] = s_v[x];
+  }
+}
+
+// The size of a merged group as a scan input: the value's high word.
+struct KwSize {
+  uint64_t v;
+  __device__ __forceinline__ explicit operator uint64_t() const { return v >> 32; }
+};
+
+// first[b] = the merged group whose output range holds position 256 * b.
+__global__ __launch_bounds__(256) void k_kw_owner(const uint64_t* __restrict__ pos, uint64_t ngroups,
+                                                 uint32_t* __restrict__ first) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < ngroups; m += stride) {
+    const uint64_t a = pos[m], e = pos[m + 1];
+    for (uint64_t b = (a + 255) / 256; b * 256 < e; ++b) first[b] = (uint32_t)m;
+  }
+}
+
+// Output position p (block b covers [256 b, 256 b + 256)): its offset from
+// its group's run, and its LCP with position p + 1: the run's LCP inside
+// the group; across groups the keys' LCP when they differ, the strings'
+// length when they are equal with a NUL (equal strings), and nothing in a
+// tie (its sort writes it).
+__global__ __launch_bounds__(256) void k_kw_expand(const KwRun* __restrict__ runs, uint32_t k,
+                                                  uint64_t n, const uint64_t* __restrict__ mkey,
+                                                  const uint64_t* __restrict__ mval,
+                                                  const uint64_t* __restrict__ pos,
+                                                  uint64_t ngroups,
+                                                  const uint32_t* __restrict__ first,
+                                                  uint32_t* __restrict__ out,
+                                                  uint64_t* __restrict__ olcp) {
+  __shared__ KwRun s_runs[kKwRunsMax];
+  __shared__ uint64_t s_base[kKwRunsMax + 1];
+  for (uint32_t r = threadIdx.x; r < k; r += 256) {
+    s_runs[r] = runs[r];
+    s_base[r] = runs[r].base;
+  }
+  if (threadIdx.x == 0) s_base[k] = n;
+  __syncthreads();
+  const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  const uint64_t nblk = (n + 255) / 256;
+  uint64_t lo = first[blockIdx.x];
+  uint64_t hi = blockIdx.x + 1 < nblk ? first[blockIdx.x + 1] : ngroups - 1;
+  while (lo < hi) {  // the last group m in [lo, hi] with pos[m] <= p
+    const uint64_t mid = (lo + hi + 1) >> 1;
+    if (pos[mid] <= p)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const uint64_t m = lo;
+  const uint64_t v = mval[m];
+  const uint64_t g = (v & 0xFFFFFFFFull) + (p - pos[m]);
+  const uint32_t r = kw_run_of(s_base, k, g);
+  const uint64_t i = g - s_base[r];
+  out[p] = s_runs[r].off[i];
+  if (olcp && p + 1 < n) {
+    if (p + 1 < pos[m + 1]) {
+      olcp[p] = s_runs[r].lcp[i];
+    } else {
+      const uint64_t a = mkey[m], b = mkey[m + 1];
+      if (a != b)
+        olcp[p] = key_lcp(a, b);
+      else if ((a & 0xFFu) == 0)
+        olcp[p] = key_len(a);
+    }
+  }
+}
+
+// tflag[m] = 1 where a tie starts: group m's key holds no NUL, the next
+// group has the same key and the previous one another.
+__global__ __launch_bounds__(256) void k_kw_tieflag(const uint64_t* __restrict__ mkey,
+                                                   uint64_t ngroups, uint32_t* __restrict__ tflag) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < ngroups; m += stride) {
+    const uint64_t x = mkey[m];
+    tflag[m] = ((x & 0xFFu) != 0 && m + 1 < ngroups && mkey[m + 1] == x &&
+                (m == 0 || mkey[m - 1] != x))
+                   ? 1u
+                   : 0u;
+  }
+}
+
+// tie[tpos[m]] = the output span [start, end) of the tie starting at group
+// m, packed start | end << 32 (a tie has one group per run at most, so the
+// walk to its end is short).
+__global__ __launch_bounds__(256) void k_kw_ties(const uint64_t* __restrict__ mkey,
+                                                uint64_t ngroups, const uint32_t* __restrict__ tflag,
+                                                const uint64_t* __restrict__ tpos,
+                                                const uint64_t* __restrict__ pos,
+                                                uint64_t* __restrict__ tie) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; m < ngroups; m += stride) {
+    if (!tflag[m]) continue;
+    const uint64_t x = mkey[m];
+    uint64_t e = m + 1;
+    while (e < ngroups && mkey[e] == x) ++e;
+    tie[tpos[m]] = pos[m] | (pos[e] << 32);
+  }
+}
+
 // Per-device grow-only scratch and a pinned word for read-backs; calls that
 // use it hold its mutex for their whole duration.
 struct Scratch {
@@ -970,6 +1357,204 @@ int merge_check(const unsigned char* d_text, uint64_t text_bytes, const uint32_t
   return GSS_OK;
 }
 
+// K-way merge by key groups (see k_kw_heads): runs are the nonempty runs in
+// run order (2 <= k <= kKwRunsMax), total their strings (< 2^31).
+constexpr uint64_t kKwTieFirst = 8192;  // tie spans read back with the count
+constexpr uint64_t kKwRankMax = 1ull << 18;  // groups merged by ranks in one launch
+constexpr uint64_t kKwSortRatio = 8;  // more than n / 8 groups: sort the runs side by side
+
+int kway_by_keys(const unsigned char* d_text, uint64_t text_bytes, const std::vector<KwRun>& hr,
+                 uint64_t total, uint32_t* d_out, uint64_t* d_out_lcp, hipStream_t st) {
+  // Every step runs on one stream, the span sort included: a sort given the
+  // null stream runs on the library's own (non-blocking) stream, which would
+  // not wait for kernels queued on the null stream.  So the caller's null
+  // stream is drained first (its runs are ready) and the library's stream
+  // takes everything.
+  if (!st) {
+    PL_HIP(hipStreamSynchronize(nullptr));
+    const int lr = library_stream(&st);
+    if (lr) return lr;
+  }
+  const uint32_t k = (uint32_t)hr.size();
+  const uint64_t n = total;
+  const uint64_t nblk = (n + kKwBlock - 1) / kKwBlock;
+  const uint64_t nexp = (n + 255) / 256;
+  const uint64_t max_splits = n / kKwmTile + 2ull * k + 2;
+  Scratch* sc = nullptr;
+  int rc = scratch_for(&sc);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(sc->mu);
+  const size_t b_runs = al256(sizeof(KwRun) * k), b_bsum = al256(8 * (nblk + 2));
+  const size_t b_gid = al256(4 * (n + 1)), b_rh = al256(4ull * (k + 1));
+  const size_t b_kv = al256(8 * (n + 1)), b_split = al256(8 * max_splits);
+  const size_t b_first = al256(4 * (nexp + 1));
+  rc = scratch_reserve(*sc, b_runs + b_bsum + 256 + b_gid + b_rh + 4 * b_kv + b_split + b_kv +
+                                b_first,
+                       st);
+  if (rc) return rc;
+  char* at = sc->dev;
+  auto take = [&](size_t bytes) {
+    char* q = at;
+    at += bytes;
+    return q;
+  };
+  KwRun* d_runs = reinterpret_cast<KwRun*>(take(b_runs));
+  uint64_t* bsum = reinterpret_cast<uint64_t*>(take(b_bsum));
+  uint64_t* d_total = reinterpret_cast<uint64_t*>(take(256));
+  uint32_t* hgid = reinterpret_cast<uint32_t*>(take(b_gid));
+  uint32_t* runhead = reinterpret_cast<uint32_t*>(take(b_rh));
+  uint64_t* key[2] = {reinterpret_cast<uint64_t*>(take(b_kv)), reinterpret_cast<uint64_t*>(take(b_kv))};
+  uint64_t* val[2] = {reinterpret_cast<uint64_t*>(take(b_kv)), reinterpret_cast<uint64_t*>(take(b_kv))};
+  uint64_t* split = reinterpret_cast<uint64_t*>(take(b_split));
+  uint64_t* pos = reinterpret_cast<uint64_t*>(take(b_kv));
+  uint32_t* first = reinterpret_cast<uint32_t*>(take(b_first));
+  const uint64_t* t64 = reinterpret_cast<const uint64_t*>(d_text);
+  PL_HIP(hipMemcpyAsync(d_runs, hr.data(), sizeof(KwRun) * k, hipMemcpyHostToDevice, st));
+
+  // 1. Groups of every run: their count first.  When they are many (most
+  //    strings differ from their neighbours within 8 bytes: short or
+  //    diverse strings), each would be read and merged on its own, and a
+  //    sort of the runs side by side does less: it gives the same order
+  //    (stable: equal strings in run order).
+  k_kw_heads<false><<<(uint32_t)nblk, kKwThreads, 0, st>>>(d_runs, k, n, t64, text_bytes, bsum,
+                                                           nullptr, nullptr, nullptr);
+  k_scan64_blocks<<<1, 1024, 0, st>>>(bsum, nblk, d_total);
+  PL_HIP(hipGetLastError());
+  PL_HIP(hipMemcpyAsync(sc->host, d_total, 8, hipMemcpyDeviceToHost, st));
+  if ((rc = wait_stream(st))) return rc;
+  const uint64_t ngroups = sc->host[0];
+  if (ngroups < k || ngroups > n) return fail(GSS_E_INTERNAL, "k-way merge: bad group count");
+  if (ngroups > n / kKwSortRatio) {
+    uint32_t* cat = reinterpret_cast<uint32_t*>(key[0]);
+    k_kw_concat<<<(uint32_t)nblk, kKwThreads, 0, st>>>(d_runs, k, n, cat);
+    PL_HIP(hipGetLastError());
+    return sort_device32(d_text, text_bytes, cat, d_out, d_out_lcp, n, st);
+  }
+  k_kw_heads<true><<<(uint32_t)nblk, kKwThreads, 0, st>>>(d_runs, k, n, t64, text_bytes, bsum,
+                                                          hgid, key[0], runhead);
+  PL_HIP(hipGetLastError());
+
+  // 2. The runs' group lists merged by key, stably: by ranks in one launch
+  //    when the groups are few, else pairwise in rounds (adjacent lists, so
+  //    equal keys keep run order), each round in one pair of launches per
+  //    kMergePairsMax merges.
+  int cb = 0;
+  if (ngroups <= kKwRankMax) {
+    k_kw_rank<<<(uint32_t)((ngroups + 255) / 256), 256, 0, st>>>(key[0], hgid, runhead, k, ngroups,
+                                                                 key[1], val[1]);
+    PL_HIP(hipGetLastError());
+    cb = 1;
+  } else {
+    std::vector<uint32_t> hrh(k + 1);
+    PL_HIP(hipMemcpyAsync(hrh.data(), runhead, 4ull * (k + 1), hipMemcpyDeviceToHost, st));
+    if ((rc = wait_stream(st))) return rc;
+    if (hrh[0] != 0 || hrh[k] != ngroups) return fail(GSS_E_INTERNAL, "k-way merge: bad group lists");
+    k_kw_pack<<<grid_cap(ngroups, 256), 256, 0, st>>>(hgid, ngroups, val[0]);
+    PL_HIP(hipGetLastError());
+    struct List {
+      uint64_t at, len;
+    };
+    std::vector<List> cur;
+    for (uint32_t r = 0; r < k; ++r) cur.push_back(List{hrh[r], (uint64_t)hrh[r + 1] - hrh[r]});
+    auto launch = [&](KwBatch& mb) -> int {
+      if (!mb.count) return GSS_OK;
+      k_kw_mpart<<<(uint32_t)((mb.splits + 255) / 256), 256, 0, st>>>(mb, split);
+      k_kw_merge<<<(uint32_t)mb.tiles, kKwmThreads, 0, st>>>(mb, split);
+      PL_HIP(hipGetLastError());
+      mb = KwBatch{};
+      return GSS_OK;
+    };
+    while (cur.size() > 1) {
+      std::vector<List> next;
+      KwBatch mb{};
+      for (size_t r = 0; r < cur.size(); r += 2) {
+        const List a = cur[r];
+        const List b = r + 1 < cur.size() ? cur[r + 1] : List{a.at + a.len, 0};
+        const uint64_t len = a.len + b.len;
+        const uint64_t tiles = (len + kKwmTile - 1) / kKwmTile;
+        if (mb.count == (uint32_t)kMergePairsMax || mb.splits + tiles + 1 > max_splits)
+          if ((rc = launch(mb))) return rc;
+        KwPair& q = mb.p[mb.count++];
+        q = KwPair{key[cb] + a.at, val[cb] + a.at, a.len, key[cb] + b.at, val[cb] + b.at, b.len,
+                   key[cb ^ 1] + a.at, val[cb ^ 1] + a.at, mb.splits, mb.tiles};
+        mb.splits += tiles + 1;
+        mb.tiles += tiles;
+        next.push_back(List{a.at, len});
+      }
+      if ((rc = launch(mb))) return rc;
+      cur.swap(next);
+      cb ^= 1;
+    }
+  }
+  const uint64_t* mkey = key[cb];
+  const uint64_t* mval = val[cb];
+
+  // 3. Output positions of the merged groups, then every string's offset
+  //    and LCP.
+  const uint64_t nsb = (ngroups + kScanBlock64 - 1) / kScanBlock64;
+  const KwSize* sizes = reinterpret_cast<const KwSize*>(mval);
+  k_scan64_reduce<false, KwSize><<<(uint32_t)nsb, kScanThreads64, 0, st>>>(sizes, ngroups, bsum);
+  k_scan64_blocks<<<1, 1024, 0, st>>>(bsum, nsb, d_total);
+  k_scan64_down<false, KwSize><<<(uint32_t)nsb, kScanThreads64, 0, st>>>(sizes, ngroups, bsum, pos);
+  k_kw_owner<<<grid_cap(ngroups, 256), 256, 0, st>>>(pos, ngroups, first);
+  k_kw_expand<<<(uint32_t)nexp, 256, 0, st>>>(d_runs, k, n, mkey, mval, pos, ngroups, first, d_out,
+                                              d_out_lcp);
+  PL_HIP(hipGetLastError());
+
+  // 4. Ties: their spans, in output order, read back for the span sort (the
+  //    buffers the merge no longer needs hold the flags and spans).
+  uint32_t* tflag = hgid;
+  uint64_t* tpos = val[cb ^ 1];
+  uint64_t* tie = key[cb ^ 1];
+  k_kw_tieflag<<<grid_cap(ngroups, 256), 256, 0, st>>>(mkey, ngroups, tflag);
+  k_scan64_reduce<false, uint32_t><<<(uint32_t)nsb, kScanThreads64, 0, st>>>(tflag, ngroups, bsum);
+  k_scan64_blocks<<<1, 1024, 0, st>>>(bsum, nsb, d_total);
+  k_scan64_down<false, uint32_t><<<(uint32_t)nsb, kScanThreads64, 0, st>>>(tflag, ngroups, bsum,
+                                                                           tpos);
+  k_kw_ties<<<grid_cap(ngroups, 256), 256, 0, st>>>(mkey, ngroups, tflag, tpos, pos, tie);
+  PL_HIP(hipGetLastError());
+  const uint64_t first_n = std::min<uint64_t>(kKwTieFirst, n / 2 + 1);
+  std::vector<uint64_t> ht(first_n);
+  PL_HIP(hipMemcpyAsync(sc->host, d_total, 8, hipMemcpyDeviceToHost, st));
+  PL_HIP(hipMemcpyAsync(ht.data(), tie, 8 * first_n, hipMemcpyDeviceToHost, st));
+  if ((rc = wait_stream(st))) return rc;
+  const uint64_t nties = sc->host[0];
+  if (nties > n / 2) return fail(GSS_E_INTERNAL, "k-way merge: bad tie count");
+  if (nties > first_n) {
+    ht.resize(nties);
+    PL_HIP(hipMemcpyAsync(ht.data() + first_n, tie + first_n, 8 * (nties - first_n),
+                          hipMemcpyDeviceToHost, st));
+    if ((rc = wait_stream(st))) return rc;
+  }
+  if (nties) {
+    // 5. Every tie's span sorted in place from byte 8; the spans between
+    //    them stay as they are.
+    std::vector<uint64_t> req;
+    std::vector<uint32_t> dep;
+    uint64_t done = 0;
+    for (uint64_t t = 0; t < nties; ++t) {
+      const uint64_t s = ht[t] & 0xFFFFFFFFull, e = ht[t] >> 32;
+      if (s < done || e <= s + 1 || e > n) return fail(GSS_E_INTERNAL, "k-way merge: bad tie span");
+      if (s > done) {
+        req.push_back(done);
+        dep.push_back(kSpanKeep);
+      }
+      req.push_back(s);
+      dep.push_back(8);
+      done = e;
+    }
+    if (done < n) {
+      req.push_back(done);
+      dep.push_back(kSpanKeep);
+    }
+    req.push_back(n);
+    rc = sort_spans_device32(d_text, text_bytes, d_out, d_out_lcp, req.data(), dep.data(),
+                             (uint32_t)dep.size(), st);
+    if (rc) return rc;
+  }
+  return wait_stream(st);
+}
+
 }  // namespace gss
 
 extern "C" {
@@ -1033,6 +1618,18 @@ int gss_merge_lcp_kway_device32(const unsigned char* d_text, uint64_t text_bytes
       PL_HIP(hipMemcpyAsync(d_out_lcp, cur[0].lcp, 8 * (total - 1), hipMemcpyDeviceToDevice, st));
     PL_HIP(hipStreamSynchronize(st));
     return GSS_OK;
+  }
+  // Three or more runs: merged by key groups (one pass over the runs, and
+  // strings read once per group and inside ties, where the 2-way rounds
+  // would compare every string log2(k) times).
+  if (cur.size() >= 3 && cur.size() <= kKwRunsMax && total < (1ull << 31)) {
+    std::vector<KwRun> hr;
+    uint64_t base = 0;
+    for (const Run& r : cur) {
+      hr.push_back(KwRun{r.off, r.lcp, r.len, base});
+      base += r.len;
+    }
+    return kway_by_keys(d_text, text_bytes, hr, total, d_out, d_out_lcp, st);
   }
   // Two ping-pong buffers for the intermediate rounds (a run at position p
   // of the merged order keeps its offsets and LCPs at [p, p + len)) and the

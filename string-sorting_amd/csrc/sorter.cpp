@@ -387,10 +387,13 @@ int ensure_init_lists(Workspace& w, uint32_t nreq) {
   } while (0)
 
 // Sorts the requests [req[r], req[r+1]) of the offset array independently
-// (req == nullptr: one request of n strings).  Called with c.mu held.
+// (req == nullptr: one request of n strings).  req_depth (optional): the
+// bytes every string of request r shares, where its sort starts, or
+// kSpanKeep for a request left as it is (with out == in its offsets and
+// LCPs are not touched).  Called with c.mu held.
 int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
              uint32_t* out, uint64_t* lcp, uint64_t n, const uint64_t* req, uint32_t nreq,
-             hipStream_t stream, const gss_tuning* tuning) {
+             hipStream_t stream, const gss_tuning* tuning, const uint32_t* req_depth = nullptr) {
   gss_stats& stats = c.stats;
   memset(&stats, 0, sizeof(stats));
   stats.n = n;
@@ -459,12 +462,13 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
   Tile* tiles = reinterpret_cast<Tile*>(w.h_up + w.up_tiles_off);
   for (uint32_t r = 0; r < nreq; ++r) {
     const uint32_t b = (uint32_t)req[r], size = (uint32_t)(req[r + 1] - req[r]);
-    if (size == 0) continue;
+    const uint32_t dep = req_depth ? req_depth[r] : 0u;
+    if (size == 0 || dep == kSpanKeep) continue;
     if (size == 1) {
       init_retire[h0.retire++] = Retire{b, 1, b, 0, 0, 1};
     } else if (size <= T) {
       const int cls = local_class(size);
-      init[cls][h0.local[cls].segs++] = LocalSeg{b, size, 0, 0};
+      init[cls][h0.local[cls].segs++] = LocalSeg{b, size, dep, 0};
       h0.local[cls].strings += size;
     } else {
       if (nbig >= p.cap_big) return fail(GSS_E_INTERNAL, "segment table overflow");
@@ -475,7 +479,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
         ++tiles0;
       }
       aux0[nbig] = SegAux{0, tiles0 - first, 0, first};
-      init_big[nbig++] = BigSeg{b, size, 0, 0};
+      init_big[nbig++] = BigSeg{b, size, dep, 0};
       pass0_strings += size;
     }
   }
@@ -812,9 +816,11 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
 
 int run_locked(Context& c, const unsigned char* text, uint64_t text_bytes, const uint32_t* in,
                uint32_t* out, uint64_t* lcp, uint64_t n, const uint64_t* req, uint32_t nreq,
-               hipStream_t stream, const gss_tuning* tuning = nullptr) {
+               hipStream_t stream, const gss_tuning* tuning = nullptr,
+               const uint32_t* req_depth = nullptr) {
   std::lock_guard<std::mutex> lk(c.mu);
-  const int rc = run_sort(c, text, text_bytes, in, out, lcp, n, req, nreq, stream, tuning);
+  const int rc =
+      run_sort(c, text, text_bytes, in, out, lcp, n, req, nreq, stream, tuning, req_depth);
   if (rc) {
     // A failed sort may leave launches queued on both streams; they read the
     // caller's arrays and the workspace, so they finish before the error is
@@ -862,6 +868,17 @@ int sort_device32(const unsigned char* text, uint64_t text_bytes, const uint32_t
   if (rc) return rc;
   return run_locked(*c, text, text_bytes, in, out, lcp, n, nullptr, 0, stream,
                     use ? &tuned : nullptr);
+}
+
+int sort_spans_device32(const unsigned char* text, uint64_t text_bytes, uint32_t* off,
+                        uint64_t* lcp, const uint64_t* req, const uint32_t* req_depth,
+                        uint32_t nreq, hipStream_t stream) {
+  if (!req || !req_depth) return fail(GSS_E_INVALID, "NULL request table");
+  Context* c = nullptr;
+  const int rc = default_context(&c);
+  if (rc) return rc;
+  return run_locked(*c, text, text_bytes, off, off, lcp, req[nreq], req, nreq, stream, nullptr,
+                    req_depth);
 }
 
 int sort_device32_tuned(const unsigned char* text, uint64_t text_bytes, const uint32_t* in,

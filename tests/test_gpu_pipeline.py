@@ -276,3 +276,100 @@ def test_merge_lcp_kway_matches_oracle(sizes, oracle_mod):
         o2, l2 = gss.merge_lcp(text, runs[0], lcps[0], runs[1], lcps[1],
                                text_bytes=joined.text_bytes)
         assert np.array_equal(o2.cpu().numpy(), out.cpu().numpy())
+
+
+def _big_tie(n, seed):
+    """Strings that share their first 12 bytes, with short suffixes over a
+    three-letter alphabet and repeats: one tie of every run, far over the
+    local capacity, sorted by global passes from byte 8."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        tail = bytes(rng.integers(97, 100, size=int(rng.integers(0, 9))).astype(np.uint8))
+        out.append(b"abcdefghijkl" + tail)
+    out += out[: n // 4]
+    out += [b"abcdefg", b"abcdefgh", b"abc"] * 50  # keys with a NUL next to the tie
+    return gen.from_strings([out[i] for i in rng.permutation(len(out))], "big_tie")
+
+
+def _coded(n, ncodes, seed):
+    """n strings of an 8-letter code (ncodes of them) and two letters of
+    a-c: about n / ncodes strings per code, so a run of n / k strings has
+    some ncodes groups of 8-byte keys, and every code ties across runs."""
+    from gss.gen import PAD, StringSet
+
+    rng = np.random.default_rng(seed)
+    code = rng.integers(0, ncodes, size=n)
+    rec = np.zeros((n, 11), dtype=np.uint8)
+    for q in range(8):
+        rec[:, 7 - q] = 97 + (code // 26 ** q) % 26
+    rec[:, 8:10] = rng.integers(97, 100, size=(n, 2))
+    text = np.concatenate([rec.ravel(), np.zeros(PAD, np.uint8)])
+    return StringSet(text, (np.arange(n, dtype=np.uint64) * 11).astype(np.uint32), "coded")
+
+
+@pytest.mark.parametrize("case", ["url8", "uniform4", "zipf3", "mixed17", "prefix5", "bigtie3",
+                                  "tree3", "runs70"])
+def test_merge_kway_key_groups(case, oracle_mod):
+    """Three to 64 runs merge by 8-byte key groups: groups from the runs' LCP
+    arrays, a stable merge of their keys, and a sort from byte 8 of only the
+    spans where groups of different runs share a key.  The output equals the
+    stable sort of the runs' concatenation, LCPs included -- with ties
+    everywhere (URL prefix groups) and almost nowhere (uniform strings),
+    short words whose keys hold a NUL (Zipf), empty and uneven runs with
+    duplicates across runs, strings that are prefixes of each other, and a
+    tie far over the local capacity.  About 300,000 groups merge in 2-way
+    rounds of their keys, fewer by ranks in one launch; more than n / 8
+    groups (uniform, Zipf, mixed) take a sort of the runs side by side; 70
+    runs take the 2-way string merge rounds."""
+    import torch
+
+    cuts = None
+    if case == "url8":
+        ss, k = gen.url_like(400_000, seed=81), 8
+    elif case == "uniform4":
+        ss, k = gen.uniform_az(200_000, seed=85), 4
+    elif case == "zipf3":
+        ss, k = gen.zipf_words(300_000, seed=82), 3
+    elif case == "mixed17":
+        parts = [gen.make(c, n=80_000, seed=83 + c) for c in range(4)]
+        ss, _ = gen.concat(parts)
+        rng = np.random.default_rng(83)
+        cuts = np.sort(rng.integers(0, ss.n, size=16))
+        cuts[3] = cuts[4]  # an empty run
+        k = 17
+    elif case == "prefix5":
+        ss, k = _prefix_family(220_000, 84), 5
+    elif case == "bigtie3":
+        ss, k = _big_tie(40_000, 86), 3
+    elif case == "tree3":  # ~300,000 groups (over the one-launch limit), ~n/10
+        ss, k = _coded(3_000_000, 100_000, 88), 3
+    else:
+        parts = [gen.make(c, n=5_000, seed=87 + c) for c in range(4)]
+        ss, _ = gen.concat(parts)
+        k = 70
+    if cuts is None:
+        cuts = np.linspace(0, ss.n, k + 1).astype(np.int64)[1:-1]
+    bounds = [0] + [int(c) for c in cuts] + [ss.n]
+    text = _dev(ss.text)
+    offs = ss.offsets.copy()
+    if case in ("mixed17", "runs70"):  # the same strings again, in another run
+        offs[ss.n // 2: ss.n // 2 + ss.n // 10] = offs[: ss.n // 10]
+    runs, lcps, ins = [], [], []
+    for r in range(k):
+        part = offs[bounds[r]:bounds[r + 1]]
+        ins.append(part)
+        n = part.size
+        if n:
+            o, lc = _sorted_run(text, part, ss.text_bytes)
+            runs.append(o[:n])
+            lcps.append(lc[: max(0, n - 1)])
+        else:
+            runs.append(torch.empty(0, dtype=torch.int32, device=device()))
+            lcps.append(torch.empty(0, dtype=torch.int64, device=device()))
+    log(f"k-way merge by key groups {case}: k={k} n={ss.n}")
+    out, ol = gss.merge_lcp_kway(text, runs, lcps, text_bytes=ss.text_bytes)
+    want, want_lcp = oracle_mod.sort_offsets(ss.text, np.concatenate(ins), lcp=True)
+    got = out.cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, want)
+    assert np.array_equal(ol.cpu().numpy().view(np.uint64), want_lcp)
