@@ -847,14 +847,17 @@ __device__ __forceinline__ uint32_t kw_run_of(const uint64_t* s_base, uint32_t k
 
 // Group starts of the concatenated runs, block b covering strings
 // [b * kKwBlock, (b + 1) * kKwBlock).  Without EMIT: bsum[b] = the block's
-// group count.  With EMIT (bsum scanned): group h (in concatenated order)
-// gets hgid[h] = its first string's index, hkey[h] = its key; runhead[r] =
-// run r's first group; hgid[H] = n and runhead[k] = H for the H groups.
+// group count.  With EMIT (bsum scanned, *ngroups = H the total), unless H
+// exceeds max_groups: group h (in concatenated order) gets hgid[h] = its
+// first string's index, hkey[h] = its key; runhead[r] = run r's first
+// group; hgid[H] = n and runhead[k] = H.
 template <bool EMIT>
 __global__ __launch_bounds__(kKwThreads) void k_kw_heads(
     const KwRun* __restrict__ runs, uint32_t k, uint64_t n, const uint64_t* __restrict__ t64,
     uint64_t tlen, uint64_t* __restrict__ bsum, uint32_t* __restrict__ hgid,
-    uint64_t* __restrict__ hkey, uint32_t* __restrict__ runhead) {
+    uint64_t* __restrict__ hkey, uint32_t* __restrict__ runhead,
+    const uint64_t* __restrict__ ngroups, uint64_t max_groups) {
+  if (EMIT && *ngroups > max_groups) return;  // the merge sorts the runs instead
   __shared__ KwRun s_runs[kKwRunsMax];
   __shared__ uint64_t s_base[kKwRunsMax + 1];
   __shared__ uint8_t s_flag[kKwBlock];
@@ -1108,8 +1111,8 @@ __global__ __launch_bounds__(256) void k_kw_owner(const uint64_t* __restrict__ p
 // Output position p (block b covers [256 b, 256 b + 256)): its offset from
 // its group's run, and its LCP with position p + 1: the run's LCP inside
 // the group; across groups the keys' LCP when they differ, the strings'
-// length when they are equal with a NUL (equal strings), and nothing in a
-// tie (its sort writes it).
+// length when they are equal with a NUL (equal strings); nothing inside a
+// tie (its sort writes those).
 __global__ __launch_bounds__(256) void k_kw_expand(const KwRun* __restrict__ runs, uint32_t k,
                                                   uint64_t n, const uint64_t* __restrict__ mkey,
                                                   const uint64_t* __restrict__ mval,
@@ -1145,10 +1148,14 @@ __global__ __launch_bounds__(256) void k_kw_expand(const KwRun* __restrict__ run
   const uint64_t i = g - s_base[r];
   out[p] = s_runs[r].off[i];
   if (olcp && p + 1 < n) {
+    const uint64_t a = mkey[m];
     if (p + 1 < pos[m + 1]) {
-      olcp[p] = s_runs[r].lcp[i];
+      // Inside a group: the run's LCP, unless the group is part of a tie.
+      const bool tied = (a & 0xFFu) != 0 && ((m > 0 && mkey[m - 1] == a) ||
+                                             (m + 1 < ngroups && mkey[m + 1] == a));
+      if (!tied) olcp[p] = s_runs[r].lcp[i];
     } else {
-      const uint64_t a = mkey[m], b = mkey[m + 1];
+      const uint64_t b = mkey[m + 1];
       if (a != b)
         olcp[p] = key_lcp(a, b);
       else if ((a & 0xFFu) == 0)
@@ -1196,6 +1203,8 @@ struct Scratch {
   char* dev = nullptr;
   size_t bytes = 0;
   uint64_t* host = nullptr;
+  uint64_t* hlist = nullptr;  // pinned, grow-only: lists read back
+  size_t hlist_bytes = 0;
 };
 
 constexpr int kMaxDev = 16;
@@ -1233,6 +1242,19 @@ int scratch_reserve(Scratch& s, size_t bytes, hipStream_t st) {
 }
 
 size_t al256(size_t v) { return (v + 255) / 256 * 256; }
+
+// Ensures `bytes` of the pinned list buffer (called with the mutex held,
+// nothing in flight that writes it).
+int hlist_reserve(Scratch& s, size_t bytes) {
+  if (bytes <= s.hlist_bytes) return GSS_OK;
+  if (s.hlist) (void)hipHostFree(s.hlist);
+  s.hlist = nullptr;
+  s.hlist_bytes = 0;
+  const size_t want = std::max<size_t>(bytes, 1 << 16);
+  PL_HIP(hipHostMalloc((void**)&s.hlist, want, hipHostMallocDefault));
+  s.hlist_bytes = want;
+  return GSS_OK;
+}
 
 // Exclusive 64-bit scan of values (+1 when PLUS_ONE) into out[0..n]; writes
 // the total to *host_total after a stream sync.  bsum: scratch of
@@ -1416,23 +1438,24 @@ int kway_by_keys(const unsigned char* d_text, uint64_t text_bytes, const std::ve
   //    diverse strings), each would be read and merged on its own, and a
   //    sort of the runs side by side does less: it gives the same order
   //    (stable: equal strings in run order).
+  const uint64_t max_groups = n / kKwSortRatio;
   k_kw_heads<false><<<(uint32_t)nblk, kKwThreads, 0, st>>>(d_runs, k, n, t64, text_bytes, bsum,
-                                                           nullptr, nullptr, nullptr);
+                                                           nullptr, nullptr, nullptr, nullptr, 0);
   k_scan64_blocks<<<1, 1024, 0, st>>>(bsum, nblk, d_total);
+  k_kw_heads<true><<<(uint32_t)nblk, kKwThreads, 0, st>>>(d_runs, k, n, t64, text_bytes, bsum,
+                                                          hgid, key[0], runhead, d_total,
+                                                          max_groups);
   PL_HIP(hipGetLastError());
   PL_HIP(hipMemcpyAsync(sc->host, d_total, 8, hipMemcpyDeviceToHost, st));
   if ((rc = wait_stream(st))) return rc;
   const uint64_t ngroups = sc->host[0];
   if (ngroups < k || ngroups > n) return fail(GSS_E_INTERNAL, "k-way merge: bad group count");
-  if (ngroups > n / kKwSortRatio) {
+  if (ngroups > max_groups) {
     uint32_t* cat = reinterpret_cast<uint32_t*>(key[0]);
     k_kw_concat<<<(uint32_t)nblk, kKwThreads, 0, st>>>(d_runs, k, n, cat);
     PL_HIP(hipGetLastError());
     return sort_device32(d_text, text_bytes, cat, d_out, d_out_lcp, n, st);
   }
-  k_kw_heads<true><<<(uint32_t)nblk, kKwThreads, 0, st>>>(d_runs, k, n, t64, text_bytes, bsum,
-                                                          hgid, key[0], runhead);
-  PL_HIP(hipGetLastError());
 
   // 2. The runs' group lists merged by key, stably: by ranks in one launch
   //    when the groups are few, else pairwise in rounds (adjacent lists, so
@@ -1514,18 +1537,18 @@ int kway_by_keys(const unsigned char* d_text, uint64_t text_bytes, const std::ve
   k_kw_ties<<<grid_cap(ngroups, 256), 256, 0, st>>>(mkey, ngroups, tflag, tpos, pos, tie);
   PL_HIP(hipGetLastError());
   const uint64_t first_n = std::min<uint64_t>(kKwTieFirst, n / 2 + 1);
-  std::vector<uint64_t> ht(first_n);
+  if ((rc = hlist_reserve(*sc, 8 * first_n))) return rc;
   PL_HIP(hipMemcpyAsync(sc->host, d_total, 8, hipMemcpyDeviceToHost, st));
-  PL_HIP(hipMemcpyAsync(ht.data(), tie, 8 * first_n, hipMemcpyDeviceToHost, st));
+  PL_HIP(hipMemcpyAsync(sc->hlist, tie, 8 * first_n, hipMemcpyDeviceToHost, st));
   if ((rc = wait_stream(st))) return rc;
   const uint64_t nties = sc->host[0];
   if (nties > n / 2) return fail(GSS_E_INTERNAL, "k-way merge: bad tie count");
   if (nties > first_n) {
-    ht.resize(nties);
-    PL_HIP(hipMemcpyAsync(ht.data() + first_n, tie + first_n, 8 * (nties - first_n),
-                          hipMemcpyDeviceToHost, st));
+    if ((rc = hlist_reserve(*sc, 8 * nties))) return rc;
+    PL_HIP(hipMemcpyAsync(sc->hlist, tie, 8 * nties, hipMemcpyDeviceToHost, st));
     if ((rc = wait_stream(st))) return rc;
   }
+  const uint64_t* ht = sc->hlist;
   if (nties) {
     // 5. Every tie's span sorted in place from byte 8; the spans between
     //    them stay as they are.
