@@ -827,6 +827,12 @@ struct KwRun {
   uint64_t base;
 };
 constexpr uint32_t kKwRunsMax = 64;  // the run table sits in LDS
+// The run table, passed by value (kernel arguments) and staged in LDS.
+struct KwRuns {
+  KwRun r[kKwRunsMax];
+};
+// More than n / kKwSortRatio groups: the merge sorts the runs side by side.
+constexpr uint64_t kKwSortRatio = 8;
 constexpr int kKwThreads = 256;
 constexpr int kKwItems = 16;
 constexpr uint64_t kKwBlock = (uint64_t)kKwThreads * kKwItems;  // strings per block
@@ -846,25 +852,25 @@ __device__ __forceinline__ uint32_t kw_run_of(const uint64_t* s_base, uint32_t k
 }
 
 // Group starts of the concatenated runs, block b covering strings
-// [b * kKwBlock, (b + 1) * kKwBlock).  Without EMIT: bsum[b] = the block's
-// group count.  With EMIT (bsum scanned, *ngroups = H the total), unless H
-// exceeds max_groups: group h (in concatenated order) gets hgid[h] = its
-// first string's index, hkey[h] = its key; runhead[r] = run r's first
-// group; hgid[H] = n and runhead[k] = H.
+// [b * kKwBlock, (b + 1) * kKwBlock), unless the sample (*sampled of nsamp
+// strings start groups) shows many groups.  Without EMIT: bsum[b] = the
+// block's group count.  With EMIT (bsum scanned): group h (in concatenated order)
+// gets hgid[h] = its first string's index, hkey[h] = its key; runhead[r] =
+// run r's first group; hgid[H] = n and runhead[k] = H for the H groups.
 template <bool EMIT>
 __global__ __launch_bounds__(kKwThreads) void k_kw_heads(
-    const KwRun* __restrict__ runs, uint32_t k, uint64_t n, const uint64_t* __restrict__ t64,
+    const KwRuns runs, uint32_t k, uint64_t n, const uint64_t* __restrict__ t64,
     uint64_t tlen, uint64_t* __restrict__ bsum, uint32_t* __restrict__ hgid,
     uint64_t* __restrict__ hkey, uint32_t* __restrict__ runhead,
-    const uint64_t* __restrict__ ngroups, uint64_t max_groups) {
-  if (EMIT && *ngroups > max_groups) return;  // the merge sorts the runs instead
+    const uint64_t* __restrict__ sampled, uint64_t nsamp) {
+  if (*sampled * kKwSortRatio > nsamp) return;  // many groups: the runs are sorted instead
   __shared__ KwRun s_runs[kKwRunsMax];
   __shared__ uint64_t s_base[kKwRunsMax + 1];
   __shared__ uint8_t s_flag[kKwBlock];
   __shared__ uint64_t scratch[kKwThreads / kWave + 1];
   for (uint32_t r = threadIdx.x; r < k; r += kKwThreads) {
-    s_runs[r] = runs[r];
-    s_base[r] = runs[r].base;
+    s_runs[r] = runs.r[r];
+    s_base[r] = runs.r[r].base;
   }
   if (threadIdx.x == 0) s_base[k] = n;
   __syncthreads();
@@ -913,6 +919,35 @@ __global__ __launch_bounds__(kKwThreads) void k_kw_heads(
   }
 }
 
+// Of kKwSamples evenly spaced strings of the concatenated runs (all of
+// them when n is smaller), those that start a group: bsum[b] for block b's
+// 1024 samples (the estimate of the group count times kKwSamples / n).
+constexpr uint32_t kKwSampleLog = 16;
+constexpr uint64_t kKwSamples = 1ull << kKwSampleLog;
+__global__ __launch_bounds__(1024) void k_kw_sample(const KwRuns runs, uint32_t k,
+                                                   uint64_t n, uint64_t* __restrict__ bsum) {
+  __shared__ KwRun s_runs[kKwRunsMax];
+  __shared__ uint64_t s_base[kKwRunsMax + 1];
+  __shared__ uint64_t scratch[1024 / kWave + 1];
+  for (uint32_t r = threadIdx.x; r < k; r += 1024) {
+    s_runs[r] = runs.r[r];
+    s_base[r] = runs.r[r].base;
+  }
+  if (threadIdx.x == 0) s_base[k] = n;
+  __syncthreads();
+  const uint64_t j = (uint64_t)blockIdx.x * 1024 + threadIdx.x;
+  uint32_t c = 0;
+  if (n <= kKwSamples ? j < n : j < kKwSamples) {
+    const uint64_t g = n <= kKwSamples ? j : (j * n) >> kKwSampleLog;
+    const uint32_t r = kw_run_of(s_base, k, g);
+    const uint64_t i = g - s_base[r];
+    c = (i == 0 || s_runs[r].lcp[i - 1] < 8) ? 1u : 0u;
+  }
+  uint64_t total;
+  block_excl_scan64<1024>(c, scratch, &total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
 // hval[h] = group h's first string index | its size << 32.
 __global__ __launch_bounds__(256) void k_kw_pack(const uint32_t* __restrict__ hgid, uint64_t ngroups,
                                                 uint64_t* __restrict__ hval) {
@@ -925,12 +960,15 @@ __global__ __launch_bounds__(256) void k_kw_pack(const uint32_t* __restrict__ hg
 // of run r goes to its index in r plus, in every other run, the groups with
 // a smaller key (in an earlier run, also those with an equal key: equal
 // keys keep run order).  Its value is its first string's index | its size
-// << 32.
+// << 32, and its output position the strings of those groups and of the
+// groups before it in r (hgid of a run's bound group less hgid of its first
+// group: hgid[H] = n ends the last run).
 __global__ __launch_bounds__(256) void k_kw_rank(const uint64_t* __restrict__ hkey,
                                                 const uint32_t* __restrict__ hgid,
                                                 const uint32_t* __restrict__ runhead, uint32_t k,
                                                 uint64_t ngroups, uint64_t* __restrict__ mkey,
-                                                uint64_t* __restrict__ mval) {
+                                                uint64_t* __restrict__ mval,
+                                                uint64_t* __restrict__ mpos) {
   __shared__ uint32_t s_rh[kKwRunsMax + 1];
   for (uint32_t r = threadIdx.x; r <= k; r += 256) s_rh[r] = runhead[r];
   __syncthreads();
@@ -946,7 +984,9 @@ __global__ __launch_bounds__(256) void k_kw_rank(const uint64_t* __restrict__ hk
   }
   const uint32_t r = lo;
   const uint64_t x = hkey[h];
+  const uint32_t gh = hgid[h];
   uint64_t rank = h - s_rh[r];
+  uint64_t at = gh - hgid[s_rh[r]];
   for (uint32_t q = 0; q < k; ++q) {
     if (q == r) continue;
     uint32_t a = s_rh[q], e = s_rh[q + 1];
@@ -959,20 +999,23 @@ __global__ __launch_bounds__(256) void k_kw_rank(const uint64_t* __restrict__ hk
         e = mid;
     }
     rank += a - s_rh[q];
+    at += hgid[a] - hgid[s_rh[q]];
   }
   mkey[rank] = x;
-  mval[rank] = (uint64_t)hgid[h] | ((uint64_t)(hgid[h + 1] - hgid[h]) << 32);
+  mval[rank] = (uint64_t)gh | ((uint64_t)(hgid[h + 1] - gh) << 32);
+  mpos[rank] = at;
+  if (h == 0) mpos[ngroups] = hgid[ngroups];
 }
 
 // The runs' offsets side by side: cat[g] for every concatenated index g.
-__global__ __launch_bounds__(kKwThreads) void k_kw_concat(const KwRun* __restrict__ runs,
+__global__ __launch_bounds__(kKwThreads) void k_kw_concat(const KwRuns runs,
                                                          uint32_t k, uint64_t n,
                                                          uint32_t* __restrict__ cat) {
   __shared__ uint64_t s_base[kKwRunsMax + 1];
   __shared__ const uint32_t* s_off[kKwRunsMax];
   for (uint32_t r = threadIdx.x; r < k; r += kKwThreads) {
-    s_base[r] = runs[r].base;
-    s_off[r] = runs[r].off;
+    s_base[r] = runs.r[r].base;
+    s_off[r] = runs.r[r].off;
   }
   if (threadIdx.x == 0) s_base[k] = n;
   __syncthreads();
@@ -1113,7 +1156,7 @@ __global__ __launch_bounds__(256) void k_kw_owner(const uint64_t* __restrict__ p
 // the group; across groups the keys' LCP when they differ, the strings'
 // length when they are equal with a NUL (equal strings); nothing inside a
 // tie (its sort writes those).
-__global__ __launch_bounds__(256) void k_kw_expand(const KwRun* __restrict__ runs, uint32_t k,
+__global__ __launch_bounds__(256) void k_kw_expand(const KwRuns runs, uint32_t k,
                                                   uint64_t n, const uint64_t* __restrict__ mkey,
                                                   const uint64_t* __restrict__ mval,
                                                   const uint64_t* __restrict__ pos,
@@ -1124,8 +1167,8 @@ __global__ __launch_bounds__(256) void k_kw_expand(const KwRun* __restrict__ run
   __shared__ KwRun s_runs[kKwRunsMax];
   __shared__ uint64_t s_base[kKwRunsMax + 1];
   for (uint32_t r = threadIdx.x; r < k; r += 256) {
-    s_runs[r] = runs[r];
-    s_base[r] = runs[r].base;
+    s_runs[r] = runs.r[r];
+    s_base[r] = runs.r[r].base;
   }
   if (threadIdx.x == 0) s_base[k] = n;
   __syncthreads();
@@ -1383,7 +1426,6 @@ int merge_check(const unsigned char* d_text, uint64_t text_bytes, const uint32_t
 // run order (2 <= k <= kKwRunsMax), total their strings (< 2^31).
 constexpr uint64_t kKwTieFirst = 8192;  // tie spans read back with the count
 constexpr uint64_t kKwRankMax = 1ull << 18;  // groups merged by ranks in one launch
-constexpr uint64_t kKwSortRatio = 8;  // more than n / 8 groups: sort the runs side by side
 
 int kway_by_keys(const unsigned char* d_text, uint64_t text_bytes, const std::vector<KwRun>& hr,
                  uint64_t total, uint32_t* d_out, uint64_t* d_out_lcp, hipStream_t st) {
@@ -1406,11 +1448,11 @@ int kway_by_keys(const unsigned char* d_text, uint64_t text_bytes, const std::ve
   int rc = scratch_for(&sc);
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(sc->mu);
-  const size_t b_runs = al256(sizeof(KwRun) * k), b_bsum = al256(8 * (nblk + 2));
+  const size_t b_bsum = al256(8 * (nblk + 2)), b_ssum = al256(8 * (kKwSamples / 1024 + 2));
   const size_t b_gid = al256(4 * (n + 1)), b_rh = al256(4ull * (k + 1));
   const size_t b_kv = al256(8 * (n + 1)), b_split = al256(8 * max_splits);
   const size_t b_first = al256(4 * (nexp + 1));
-  rc = scratch_reserve(*sc, b_runs + b_bsum + 256 + b_gid + b_rh + 4 * b_kv + b_split + b_kv +
+  rc = scratch_reserve(*sc, b_bsum + b_ssum + 256 + b_gid + b_rh + 4 * b_kv + b_split + b_kv +
                                 b_first,
                        st);
   if (rc) return rc;
@@ -1420,9 +1462,9 @@ int kway_by_keys(const unsigned char* d_text, uint64_t text_bytes, const std::ve
     at += bytes;
     return q;
   };
-  KwRun* d_runs = reinterpret_cast<KwRun*>(take(b_runs));
   uint64_t* bsum = reinterpret_cast<uint64_t*>(take(b_bsum));
-  uint64_t* d_total = reinterpret_cast<uint64_t*>(take(256));
+  uint64_t* ssum = reinterpret_cast<uint64_t*>(take(b_ssum));
+  uint64_t* d_total = reinterpret_cast<uint64_t*>(take(256));  // [0] groups, [1] sampled starts
   uint32_t* hgid = reinterpret_cast<uint32_t*>(take(b_gid));
   uint32_t* runhead = reinterpret_cast<uint32_t*>(take(b_rh));
   uint64_t* key[2] = {reinterpret_cast<uint64_t*>(take(b_kv)), reinterpret_cast<uint64_t*>(take(b_kv))};
@@ -1431,31 +1473,37 @@ int kway_by_keys(const unsigned char* d_text, uint64_t text_bytes, const std::ve
   uint64_t* pos = reinterpret_cast<uint64_t*>(take(b_kv));
   uint32_t* first = reinterpret_cast<uint32_t*>(take(b_first));
   const uint64_t* t64 = reinterpret_cast<const uint64_t*>(d_text);
-  PL_HIP(hipMemcpyAsync(d_runs, hr.data(), sizeof(KwRun) * k, hipMemcpyHostToDevice, st));
+  KwRuns runs{};
+  for (uint32_t r = 0; r < k; ++r) runs.r[r] = hr[r];
 
-  // 1. Groups of every run: their count first.  When they are many (most
-  //    strings differ from their neighbours within 8 bytes: short or
-  //    diverse strings), each would be read and merged on its own, and a
-  //    sort of the runs side by side does less: it gives the same order
-  //    (stable: equal strings in run order).
-  const uint64_t max_groups = n / kKwSortRatio;
-  k_kw_heads<false><<<(uint32_t)nblk, kKwThreads, 0, st>>>(d_runs, k, n, t64, text_bytes, bsum,
-                                                           nullptr, nullptr, nullptr, nullptr, 0);
+  // 1. Whether the groups are few, from a sample of the LCP arrays, and
+  //    (when they are) every group: its first string and key; one read-back
+  //    for both.  When they are many (most strings differ from their
+  //    neighbours within 8 bytes: short or diverse strings), each would be
+  //    read and merged on its own, and a sort of the runs side by side does
+  //    less: it gives the same order (stable: equal strings in run order).
+  //    The group kernels then exit at once.
+  const uint64_t nsamp = std::min<uint64_t>(n, kKwSamples);
+  const uint32_t sblk = (uint32_t)((nsamp + 1023) / 1024);
+  k_kw_sample<<<sblk, 1024, 0, st>>>(runs, k, n, ssum);
+  k_scan64_blocks<<<1, 1024, 0, st>>>(ssum, sblk, d_total + 1);
+  k_kw_heads<false><<<(uint32_t)nblk, kKwThreads, 0, st>>>(runs, k, n, t64, text_bytes, bsum,
+                                                           nullptr, nullptr, nullptr,
+                                                           d_total + 1, nsamp);
   k_scan64_blocks<<<1, 1024, 0, st>>>(bsum, nblk, d_total);
-  k_kw_heads<true><<<(uint32_t)nblk, kKwThreads, 0, st>>>(d_runs, k, n, t64, text_bytes, bsum,
-                                                          hgid, key[0], runhead, d_total,
-                                                          max_groups);
+  k_kw_heads<true><<<(uint32_t)nblk, kKwThreads, 0, st>>>(runs, k, n, t64, text_bytes, bsum, hgid,
+                                                          key[0], runhead, d_total + 1, nsamp);
   PL_HIP(hipGetLastError());
-  PL_HIP(hipMemcpyAsync(sc->host, d_total, 8, hipMemcpyDeviceToHost, st));
+  PL_HIP(hipMemcpyAsync(sc->host, d_total, 16, hipMemcpyDeviceToHost, st));
   if ((rc = wait_stream(st))) return rc;
-  const uint64_t ngroups = sc->host[0];
-  if (ngroups < k || ngroups > n) return fail(GSS_E_INTERNAL, "k-way merge: bad group count");
-  if (ngroups > max_groups) {
+  if (sc->host[1] * kKwSortRatio > nsamp) {
     uint32_t* cat = reinterpret_cast<uint32_t*>(key[0]);
-    k_kw_concat<<<(uint32_t)nblk, kKwThreads, 0, st>>>(d_runs, k, n, cat);
+    k_kw_concat<<<(uint32_t)nblk, kKwThreads, 0, st>>>(runs, k, n, cat);
     PL_HIP(hipGetLastError());
     return sort_device32(d_text, text_bytes, cat, d_out, d_out_lcp, n, st);
   }
+  const uint64_t ngroups = sc->host[0];
+  if (ngroups < k || ngroups > n) return fail(GSS_E_INTERNAL, "k-way merge: bad group count");
 
   // 2. The runs' group lists merged by key, stably: by ranks in one launch
   //    when the groups are few, else pairwise in rounds (adjacent lists, so
@@ -1464,7 +1512,7 @@ int kway_by_keys(const unsigned char* d_text, uint64_t text_bytes, const std::ve
   int cb = 0;
   if (ngroups <= kKwRankMax) {
     k_kw_rank<<<(uint32_t)((ngroups + 255) / 256), 256, 0, st>>>(key[0], hgid, runhead, k, ngroups,
-                                                                 key[1], val[1]);
+                                                                 key[1], val[1], pos);
     PL_HIP(hipGetLastError());
     cb = 1;
   } else {
@@ -1512,15 +1560,18 @@ int kway_by_keys(const unsigned char* d_text, uint64_t text_bytes, const std::ve
   const uint64_t* mkey = key[cb];
   const uint64_t* mval = val[cb];
 
-  // 3. Output positions of the merged groups, then every string's offset
-  //    and LCP.
+  // 3. Output positions of the merged groups (the rank merge wrote them),
+  //    then every string's offset and LCP.
   const uint64_t nsb = (ngroups + kScanBlock64 - 1) / kScanBlock64;
-  const KwSize* sizes = reinterpret_cast<const KwSize*>(mval);
-  k_scan64_reduce<false, KwSize><<<(uint32_t)nsb, kScanThreads64, 0, st>>>(sizes, ngroups, bsum);
-  k_scan64_blocks<<<1, 1024, 0, st>>>(bsum, nsb, d_total);
-  k_scan64_down<false, KwSize><<<(uint32_t)nsb, kScanThreads64, 0, st>>>(sizes, ngroups, bsum, pos);
+  if (ngroups > kKwRankMax) {
+    const KwSize* sizes = reinterpret_cast<const KwSize*>(mval);
+    k_scan64_reduce<false, KwSize><<<(uint32_t)nsb, kScanThreads64, 0, st>>>(sizes, ngroups, bsum);
+    k_scan64_blocks<<<1, 1024, 0, st>>>(bsum, nsb, d_total);
+    k_scan64_down<false, KwSize><<<(uint32_t)nsb, kScanThreads64, 0, st>>>(sizes, ngroups, bsum,
+                                                                           pos);
+  }
   k_kw_owner<<<grid_cap(ngroups, 256), 256, 0, st>>>(pos, ngroups, first);
-  k_kw_expand<<<(uint32_t)nexp, 256, 0, st>>>(d_runs, k, n, mkey, mval, pos, ngroups, first, d_out,
+  k_kw_expand<<<(uint32_t)nexp, 256, 0, st>>>(runs, k, n, mkey, mval, pos, ngroups, first, d_out,
                                               d_out_lcp);
   PL_HIP(hipGetLastError());
 
