@@ -264,83 +264,68 @@ __global__ __launch_bounds__(256) void k_strlen(const uint64_t* __restrict__ t64
   }
 }
 
-// first[b] = the string whose output range holds byte 16*b.
-__global__ __launch_bounds__(256) void k_block_owner(const uint64_t* __restrict__ pos, uint64_t n,
-                                                    uint32_t* __restrict__ first) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
-    const uint64_t a = pos[s], e = pos[s + 1];
-    for (uint64_t b = (a + 15) / 16; b * 16 < e; ++b) first[b] = (uint32_t)s;
-  }
-}
-
-// 128-bit helpers on (lo, hi) little-endian byte pairs.
-struct U128 {
-  uint64_t lo, hi;
-};
-
-// Shifted towards higher byte positions by k bytes (0 <= k < 16).
-__device__ __forceinline__ U128 shl_bytes(U128 v, uint32_t k) {
-  if (k == 0) return v;
-  if (k >= 8) return U128{0ull, v.lo << (8 * (k - 8))};
-  return U128{v.lo << (8 * k), (v.hi << (8 * k)) | (v.lo >> (64 - 8 * k))};
-}
-
-// Bytes [0, k) set (0 <= k <= 16).
-__device__ __forceinline__ U128 low_bytes(uint32_t k) {
-  if (k >= 16) return U128{~0ull, ~0ull};
-  if (k >= 8) return U128{~0ull, k == 8 ? 0ull : (~0ull >> (64 - 8 * (k - 8)))};
-  return U128{k == 0 ? 0ull : (~0ull >> (64 - 8 * k)), 0ull};
-}
-
-// One thread per 16 output bytes.  Each string that reaches into the block
-// contributes one 16-byte text window, shifted to its place and masked to
-// its bytes; the newline after it is ORed in (the text byte there is the
-// NUL, or lies past the text, and is masked out).
-__global__ __launch_bounds__(256) void k_write(const uint64_t* __restrict__ t64, uint64_t len,
-                                              const uint32_t* __restrict__ off,
-                                              const uint64_t* __restrict__ pos, uint64_t n,
-                                              const uint32_t* __restrict__ first,
-                                              uint64_t total, unsigned char* __restrict__ out) {
-  const uint64_t nblk = (total + 15) / 16;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nblk; b += stride) {
-    const uint64_t p0 = b * 16, p1 = p0 + 16 < total ? p0 + 16 : total;
-    U128 acc{0ull, 0ull};  // little-endian output bytes
-    uint64_t s = first[b];
-    uint64_t p = p0;
-    while (p < p1 && s < n) {
-      const uint64_t ps = pos[s], pe = pos[s + 1];  // string s: [ps, pe-1), '\n' at pe-1
-      const uint64_t hi = pe < p1 ? pe : p1;
-      const uint64_t tend = pe - 1 < hi ? pe - 1 : hi;  // end of its text bytes here
-      // Up to 16 string bytes starting at byte (p - ps) of string s.
-      const uint64_t src = (uint64_t)off[s] + (p - ps);
-      uint64_t raw[3];
-      text_words<3>(t64, len, src >> 3, raw);
-      const uint32_t sh = (uint32_t)(src & 7) * 8;
-      const U128 v{funnel(raw[0], raw[1], sh), funnel(raw[1], raw[2], sh)};
-      const uint32_t a = (uint32_t)(p - p0);  // first output byte of this string here
-      const U128 w = shl_bytes(v, a);
-      const U128 m = low_bytes((uint32_t)(tend - p0)), m0 = low_bytes(a);
-      acc.lo |= w.lo & m.lo & ~m0.lo;
-      acc.hi |= w.hi & m.hi & ~m0.hi;
-      if (pe - 1 < p1) {  // the newline falls in this block
-        const uint32_t d = (uint32_t)(pe - 1 - p0);
-        if (d < 8)
-          acc.lo |= (uint64_t)'\n' << (8 * d);
-        else
-          acc.hi |= (uint64_t)'\n' << (8 * (d - 8));
+// --write, string-parallel: block b takes sorted strings [256 b, 256 b +
+// 256), whose output is one contiguous range [pos[s0], pos[s_end]).  Each
+// thread reads its string (72 bytes in flight per load round) and writes
+// it, and its newline, into LDS laid out on the output's 16-byte grid; the
+// block then stores the range with aligned 16-byte stores (byte stores only
+// for the two chunks it shares with its neighbours, whose bytes differ).
+// A range over the stage's capacity (long strings) is written straight from
+// the threads.  Nothing is written when the output (pos[n]) exceeds cap.
+constexpr int kWrThreads = 256;
+constexpr uint32_t kWrStage = 32768;
+__global__ __launch_bounds__(kWrThreads) void k_write_staged(const uint64_t* __restrict__ t64,
+                                                           uint64_t len,
+                                                           const uint32_t* __restrict__ off,
+                                                           const uint64_t* __restrict__ pos,
+                                                           uint64_t n, uint64_t cap,
+                                                           unsigned char* __restrict__ out) {
+  __shared__ __align__(16) unsigned char s_buf[kWrStage];
+  if (pos[n] > cap) return;
+  const uint64_t s0 = (uint64_t)blockIdx.x * kWrThreads;
+  const uint64_t s_end = s0 + kWrThreads < n ? s0 + kWrThreads : n;
+  const uint64_t p0 = pos[s0], p1 = pos[s_end];
+  const uint64_t base = p0 & ~15ull;  // LDS byte 0 is output byte `base`
+  const bool staged = p1 - base <= kWrStage;
+  const uint64_t s = s0 + threadIdx.x;
+  if (s < s_end) {
+    const uint64_t ps = pos[s], pe = pos[s + 1];  // bytes [ps, pe - 1), '\n' at pe - 1
+    const uint64_t slen = pe - 1 - ps;
+    const uint64_t o = off[s];
+    for (uint64_t k = 0; k < slen; k += 64) {
+      uint64_t raw[9];
+      text_words<9>(t64, len, (o + k) >> 3, raw);
+      const uint32_t sh = (uint32_t)((o + k) & 7) * 8;
+      const uint64_t m = slen - k < 64 ? slen - k : 64;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if ((uint64_t)(8 * q) >= m) break;
+        const uint64_t w = funnel(raw[q], raw[q + 1], sh);
+        const uint64_t dst = ps + k + 8 * q;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          if ((uint64_t)(8 * q + b) >= m) break;
+          const unsigned char c = (unsigned char)(w >> (8 * b));
+          if (staged)
+            s_buf[dst + b - base] = c;
+          else
+            out[dst + b] = c;
+        }
       }
-      p = hi;
-      ++s;
     }
-    if (p1 - p0 == 16) {
-      *reinterpret_cast<TextPair*>(out + p0) = TextPair{acc.lo, acc.hi};
+    if (staged)
+      s_buf[pe - 1 - base] = '\n';
+    else
+      out[pe - 1] = '\n';
+  }
+  if (!staged) return;  // the same for the whole block
+  __syncthreads();
+  for (uint64_t c = base + (uint64_t)threadIdx.x * 16; c < p1; c += (uint64_t)kWrThreads * 16) {
+    if (c >= p0 && c + 16 <= p1) {
+      *reinterpret_cast<TextPair*>(out + c) = *reinterpret_cast<const TextPair*>(s_buf + (c - base));
     } else {
-      for (uint64_t q = p0; q < p1; ++q) {
-        const uint32_t d = (uint32_t)(q - p0);
-        out[q] = (unsigned char)((d < 8 ? acc.lo : acc.hi) >> (8 * (d & 7)));
-      }
+      const uint64_t a = c > p0 ? c : p0, e = c + 16 < p1 ? c + 16 : p1;
+      for (uint64_t q = a; q < e; ++q) out[q] = s_buf[q - base];
     }
   }
 }
@@ -1798,35 +1783,28 @@ int gss_write_device32(const unsigned char* d_text, uint64_t text_bytes, const u
   const uint64_t* t64 = reinterpret_cast<const uint64_t*>(d_text);
   k_strlen<<<grid_cap(n, 256), 256, 0, st>>>(t64, text_bytes, d_off, n, lens);
   PL_HIP(hipGetLastError());
-  rc = scan64<true, uint32_t>(lens, n, bsum, pos, d_total, sc->host, st);
-  if (rc) return rc;
-  const uint64_t total = *sc->host;
-  *out_bytes = total;
-  if (!d_out) return GSS_OK;  // size query
-  if (total > out_cap)
-    return fail(GSS_E_TOO_LARGE, "output of " + std::to_string(total) + " bytes exceeds out_cap");
-  const uint64_t nblk = (total + 15) / 16;
-  // The block-owner table follows the scan arrays (the scratch may grow: the
-  // lengths and positions are rebuilt after a regrow).
-  const size_t need = head + al256(4 * nblk);
-  if (need > sc->bytes) {
-    rc = scratch_reserve(*sc, need, st);
-    if (rc) return rc;
-    lens = reinterpret_cast<uint32_t*>(sc->dev);
-    pos = reinterpret_cast<uint64_t*>(sc->dev + al256(4 * n));
-    bsum = reinterpret_cast<uint64_t*>(sc->dev + al256(4 * n) + al256(8 * (n + 1)));
-    d_total = reinterpret_cast<uint64_t*>(sc->dev + head - 256);
-    k_strlen<<<grid_cap(n, 256), 256, 0, st>>>(t64, text_bytes, d_off, n, lens);
-    PL_HIP(hipGetLastError());
+  if (!d_out) {  // size query
     rc = scan64<true, uint32_t>(lens, n, bsum, pos, d_total, sc->host, st);
     if (rc) return rc;
+    *out_bytes = *sc->host;
+    return GSS_OK;
   }
-  uint32_t* first = reinterpret_cast<uint32_t*>(sc->dev + head);
-  k_block_owner<<<grid_cap(n, 256), 256, 0, st>>>(pos, n, first);
-  k_write<<<grid_cap(nblk, 256), 256, 0, st>>>(t64, text_bytes, d_off, pos, n, first, total,
-                                                d_out);
+  // The positions, then the strings, with no read-back in between: the
+  // write kernel finds the total (pos[n]) itself and writes nothing when it
+  // exceeds out_cap.
+  const uint64_t nsb2 = (n + kScanBlock64 - 1) / kScanBlock64;
+  k_scan64_reduce<true, uint32_t><<<(uint32_t)nsb2, kScanThreads64, 0, st>>>(lens, n, bsum);
+  k_scan64_blocks<<<1, 1024, 0, st>>>(bsum, nsb2, d_total);
+  k_scan64_down<true, uint32_t><<<(uint32_t)nsb2, kScanThreads64, 0, st>>>(lens, n, bsum, pos);
+  k_write_staged<<<(uint32_t)((n + kWrThreads - 1) / kWrThreads), kWrThreads, 0, st>>>(
+      t64, text_bytes, d_off, pos, n, out_cap, d_out);
   PL_HIP(hipGetLastError());
+  PL_HIP(hipMemcpyAsync(sc->host, d_total, 8, hipMemcpyDeviceToHost, st));
   PL_HIP(hipStreamSynchronize(st));
+  const uint64_t total = *sc->host;
+  *out_bytes = total;
+  if (total > out_cap)
+    return fail(GSS_E_TOO_LARGE, "output of " + std::to_string(total) + " bytes exceeds out_cap");
   return GSS_OK;
 }
 

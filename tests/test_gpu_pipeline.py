@@ -95,6 +95,40 @@ def test_write_edge_cases(oracle_mod):
     assert got.cpu().numpy().tobytes() == b"hij\ncdefghij\n"
 
 
+def test_write_capacity_long_strings(oracle_mod):
+    """gss_write_device32 through the C ABI: the size query (no output), an
+    output one byte too small (GSS_E_TOO_LARGE with the exact size, nothing
+    written), and the exact size; strings of up to 3,000 bytes, so that some
+    blocks of 256 strings overflow the write kernel's 32 KiB stage and take
+    its direct path, among short ones that do not."""
+    import ctypes as C
+    import torch
+
+    rng = np.random.default_rng(5)
+    lens = np.where(rng.random(1500) < 0.3, rng.integers(500, 3000, 1500), rng.integers(0, 40, 1500))
+    strs = [bytes(rng.integers(97, 123, int(k)).astype(np.uint8)) for k in lens]
+    ss = gen.from_strings(strs)
+    text = _dev(ss.text)
+    order = ss.offsets[rng.permutation(ss.n)]
+    off = _dev(order.view(np.int32))
+    want = oracle_mod.write(ss.text, order, length=ss.text_bytes)
+    lib = gss.lib()
+    size = C.c_uint64()
+    assert lib.gss_write_device32(text.data_ptr(), ss.text_bytes, off.data_ptr(), ss.n, None, 0,
+                                  C.byref(size), None) == 0
+    assert size.value == len(want)
+    out = torch.full((len(want) + 32,), 0xAB, dtype=torch.uint8, device=device())
+    rc = lib.gss_write_device32(text.data_ptr(), ss.text_bytes, off.data_ptr(), ss.n,
+                                out.data_ptr(), len(want) - 1, C.byref(size), None)
+    assert rc == -4 and size.value == len(want)  # GSS_E_TOO_LARGE
+    assert bool((out == 0xAB).all())
+    rc = lib.gss_write_device32(text.data_ptr(), ss.text_bytes, off.data_ptr(), ss.n,
+                                out.data_ptr(), len(want), C.byref(size), None)
+    assert rc == 0 and size.value == len(want)
+    got = out.cpu().numpy().tobytes()
+    assert got[: len(want)] == want and got[len(want):] == b"\xab" * 32
+
+
 @pytest.mark.parametrize("kind", ["plain", "raw"])
 def test_file_pipeline_on_device(kind, oracle_mod):
     """sortstring's path on the device: the file moves once, strings are cut
