@@ -119,11 +119,16 @@ __global__ __launch_bounds__(kIngThreads) void k_ingest_count(const uint64_t* __
 
 // Every delimiter j at byte i writes off[j+1] = i+1 (j+1 < n) and, unless
 // the delimiter is NUL, becomes a NUL.  Block bases come from the scanned
-// counts, ranks inside the block from one scan of per-thread counts.
+// counts, ranks inside the block from one scan of per-thread counts.  n is
+// the scanned total (*total): when it is 0 or exceeds cap nothing is
+// written, the text included.
 __global__ __launch_bounds__(kIngThreads) void k_ingest_emit(uint64_t* __restrict__ t64,
                                                             uint64_t len, uint64_t pat,
                                                             bool clear, const uint64_t* __restrict__ base,
-                                                            uint64_t n, uint32_t* __restrict__ off) {
+                                                            const uint64_t* __restrict__ total,
+                                                            uint64_t cap, uint32_t* __restrict__ off) {
+  const uint64_t n = *total;
+  if (n == 0 || n > cap) return;
   __shared__ uint64_t scratch[kIngThreads / kWave + 1];
   const uint64_t w0 = (uint64_t)blockIdx.x * (kIngChunk / 8) + (uint64_t)threadIdx.x * kIngWords;
   uint64_t f[kIngWords];
@@ -1337,20 +1342,34 @@ int gss_ingest_device32(unsigned char* d_text, uint64_t text_bytes, int delim, u
     k_ingest_count<<<(uint32_t)nblk, kIngThreads, 0, st>>>(t64, text_bytes, pat, counts);
     PL_HIP(hipGetLastError());
   }
-  rc = scan64<false, uint64_t>(counts, nblk, bsum, bases, d_total, sc->host, st);
-  if (rc) return rc;
-  const uint64_t n = *sc->host;
-  if (n == 0) return fail(GSS_E_INVALID, "unable to read any lines from the input");
-  if (n > max_strings || !d_off) {
-    *n_strings = n;
-    return d_off ? fail(GSS_E_TOO_LARGE, std::to_string(n) + " strings exceed max_strings")
-                 : GSS_OK;  // size query
+  if (!d_off) {  // size query
+    rc = scan64<false, uint64_t>(counts, nblk, bsum, bases, d_total, sc->host, st);
+    if (rc) return rc;
+    *n_strings = *sc->host;
+    if (!*n_strings) return fail(GSS_E_INVALID, "unable to read any lines from the input");
+    return GSS_OK;
   }
-  k_ingest_emit<<<(uint32_t)nblk, kIngThreads, 0, st>>>(t64, text_bytes, pat, delim != 0, bases,
-                                                        n, d_off);
+  // The offsets with no read-back in between: the emit kernel takes the
+  // count from the scan and writes nothing (text included) when it is 0 or
+  // over max_strings.
+  const uint64_t nsb2 = (nblk + kScanBlock64 - 1) / kScanBlock64;
+  if (nblk)
+    k_scan64_reduce<false, uint64_t><<<(uint32_t)nsb2, kScanThreads64, 0, st>>>(counts, nblk, bsum);
+  k_scan64_blocks<<<1, 1024, 0, st>>>(bsum, nsb2, d_total);
+  if (nblk) {
+    k_scan64_down<false, uint64_t><<<(uint32_t)nsb2, kScanThreads64, 0, st>>>(counts, nblk, bsum,
+                                                                              bases);
+    k_ingest_emit<<<(uint32_t)nblk, kIngThreads, 0, st>>>(t64, text_bytes, pat, delim != 0, bases,
+                                                          d_total, max_strings, d_off);
+  }
   PL_HIP(hipGetLastError());
+  PL_HIP(hipMemcpyAsync(sc->host, d_total, 8, hipMemcpyDeviceToHost, st));
   PL_HIP(hipStreamSynchronize(st));
+  const uint64_t n = *sc->host;
   *n_strings = n;
+  if (n == 0) return fail(GSS_E_INVALID, "unable to read any lines from the input");
+  if (n > max_strings)
+    return fail(GSS_E_TOO_LARGE, std::to_string(n) + " strings exceed max_strings");
   return GSS_OK;
 }
 

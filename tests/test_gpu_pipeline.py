@@ -60,6 +60,34 @@ def test_ingest_matches_oracle(delim, kind, oracle_mod):
     assert not text.cpu().numpy()[len(data):].any(), "bytes past the text were written"
 
 
+def test_ingest_capacity(oracle_mod):
+    """gss_ingest_device32 through the C ABI: max_strings one short gives
+    GSS_E_TOO_LARGE with the count, and leaves the text and the offsets as
+    they were; the exact capacity splits the text as the oracle does."""
+    import ctypes as C
+    import torch
+
+    rng = np.random.default_rng(9)
+    lines = [bytes(rng.integers(97, 123, int(k)).astype(np.uint8)) for k in rng.integers(0, 90, 3000)]
+    data = b"\n".join(lines) + b"\n"
+    want_off, want_text = oracle_mod.ingest(np.frombuffer(data, np.uint8), ord("\n"))
+    n = len(want_off)
+    lib = gss.lib()
+    cnt = C.c_uint64()
+    text = _text_tensor(data)
+    before = text.clone()
+    off = torch.full((n,), -1, dtype=torch.int32, device=device())
+    rc = lib.gss_ingest_device32(text.data_ptr(), len(data), ord("\n"), off.data_ptr(), n - 1,
+                                 C.byref(cnt), None)
+    assert rc == -4 and cnt.value == n  # GSS_E_TOO_LARGE
+    assert torch.equal(text, before) and bool((off == -1).all())
+    rc = lib.gss_ingest_device32(text.data_ptr(), len(data), ord("\n"), off.data_ptr(), n,
+                                 C.byref(cnt), None)
+    assert rc == 0 and cnt.value == n
+    assert np.array_equal(off.cpu().numpy().view(np.uint32), want_off)
+    assert text.cpu().numpy().tobytes()[: len(data)] == bytes(want_text[: len(data)])
+
+
 def test_ingest_without_delimiter_is_an_error():
     text = _text_tensor(b"no newline here")
     with pytest.raises(gss.GssError, match="unable to read any lines"):
