@@ -371,7 +371,7 @@ def _coded(n, ncodes, seed):
 
 
 @pytest.mark.parametrize("case", ["url8", "uniform4", "zipf3", "mixed17", "prefix5", "bigtie3",
-                                  "tree3", "runs70"])
+                                  "tree3", "runs70", "tiny64"])
 def test_merge_kway_key_groups(case, oracle_mod):
     """Three to 64 runs merge by 8-byte key groups: groups from the runs' LCP
     arrays, a stable merge of their keys, and a sort from byte 8 of only the
@@ -406,6 +406,12 @@ def test_merge_kway_key_groups(case, oracle_mod):
         ss, k = _big_tie(40_000, 86), 3
     elif case == "tree3":  # ~300,000 groups (over the one-launch limit), ~n/10
         ss, k = _coded(3_000_000, 100_000, 88), 3
+    elif case == "tiny64":  # 64 runs of 1-4 strings (runs of one have no LCPs), ties and NULs
+        words = [b"abcdefghij", b"abcdefghik", b"abcdefgh", b"abc", b"abcdefghij", b""]
+        rng = np.random.default_rng(89)
+        ss = gen.from_strings([words[int(i)] for i in rng.integers(0, len(words), 160)])
+        k = 64
+        cuts = np.sort(rng.choice(np.arange(1, ss.n), size=63, replace=False))
     else:
         parts = [gen.make(c, n=5_000, seed=87 + c) for c in range(4)]
         ss, _ = gen.concat(parts)
