@@ -269,7 +269,10 @@ int gss_merge_lcp_device32_noalloc(const unsigned char *d_text, uint64_t text_by
  * its LCP array) form a group keyed by those bytes, the groups of all runs
  * merge stably as integers, and only the output spans where groups of
  * different runs share a key are sorted, in place, from byte 8 (the batch
- * sort's kernels).  Two runs, or more than 64, merge pairwise in rounds (a
+ * sort's kernels).  When a sample of the LCP arrays shows more than one
+ * group per 8 strings (short or diverse strings), the runs are sorted side
+ * by side instead (the same order: the sort is stable).  Two runs, or more
+ * than 64, merge pairwise in rounds (a
  * balanced tree of the 2-way merge above, all queued on the stream).  Both
  * use the library's per-device scratch (about 44 and 24 bytes per string),
  * which grows as needed and is kept; the key-group merge also sizes the
