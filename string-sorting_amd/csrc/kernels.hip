@@ -559,6 +559,10 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(Plan p) {
   if (threadIdx.x == 0) p.flat[len] = carry;
 }
 
+// SELF_PREFIX: flat_blk holds the blocks' raw sums (no k_scan_blocks ran):
+// each block adds up those before it itself (at most kScanSelfMax of them,
+// a few reads per thread) and the block holding the end writes the total.
+template <bool SELF_PREFIX>
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(Plan p) {
   static_assert(kScanItems % 4 == 0, "whole 16-byte groups per thread");
   constexpr int Q = kScanItems / 4;  // 16-byte groups per thread
@@ -567,7 +571,15 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(Plan p) {
   const uint32_t len = p.ctr->flat_len;
   const uint32_t blk = blockIdx.x;
   const uint64_t b0 = (uint64_t)blk * kScanBlock;
+  if (SELF_PREFIX && len == 0 && blk == 0 && threadIdx.x == 0) p.flat[0] = 0;
   if (b0 >= len) return;
+  uint32_t before = 0;
+  if (SELF_PREFIX) {
+    uint32_t part = 0;
+    for (uint32_t b = threadIdx.x; b < blk; b += kScanThreads) part += p.flat_blk[b];
+    (void)block_excl_scan<kScanThreads>(part, s_scan, &before);
+    __syncthreads();  // s_scan is reused below
+  }
   // The block's entries go through LDS so that global loads and stores are
   // 16-byte and coalesced while each thread scans consecutive entries.
   const bool whole = b0 + kScanBlock <= len;
@@ -590,7 +602,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(Plan p) {
     sum += q[r].x + q[r].y + q[r].z + q[r].w;
   }
   uint32_t total;
-  uint32_t run = block_excl_scan<kScanThreads>(sum, s_scan, &total) + p.flat_blk[blk];
+  uint32_t run = block_excl_scan<kScanThreads>(sum, s_scan, &total) +
+                 (SELF_PREFIX ? before : p.flat_blk[blk]);
+  if (SELF_PREFIX && b0 + kScanBlock >= len && threadIdx.x == 0) p.flat[len] = before + total;
 #pragma unroll
   for (int r = 0; r < Q; ++r) {
     uint4 o;
@@ -1104,9 +1118,17 @@ hipError_t launch_histogram(const Plan& p, uint32_t grid, uint32_t radix_bits, h
 
 hipError_t launch_flat_scan(const Plan& p, uint64_t max_len, hipStream_t s) {
   const uint32_t blocks = (uint32_t)((max_len + kScanBlock - 1) / kScanBlock);
-  if (blocks) k_scan_reduce<<<blocks, kScanThreads, 0, s>>>(p);
+  // Up to kScanSelfMax blocks, k_scan_down adds up the block sums before
+  // each block itself, one launch fewer than with k_scan_blocks.
+  constexpr uint32_t kScanSelfMax = 2048;
+  if (blocks <= kScanSelfMax) {
+    if (blocks) k_scan_reduce<<<blocks, kScanThreads, 0, s>>>(p);
+    k_scan_down<true><<<blocks ? blocks : 1, kScanThreads, 0, s>>>(p);
+    return hipGetLastError();
+  }
+  k_scan_reduce<<<blocks, kScanThreads, 0, s>>>(p);
   k_scan_blocks<<<1, 1024, 0, s>>>(p);
-  if (blocks) k_scan_down<<<blocks, kScanThreads, 0, s>>>(p);
+  k_scan_down<false><<<blocks, kScanThreads, 0, s>>>(p);
   return hipGetLastError();
 }
 
