@@ -1229,6 +1229,29 @@ __global__ __launch_bounds__(256) void k_kw_ties(const uint64_t* __restrict__ mk
   }
 }
 
+// The span sort's request table from the T = *ntie tie spans: for tie t,
+// request 2t keeps [end of tie t - 1, its start) (possibly empty) and
+// request 2t + 1 sorts the tie from byte 8; request 2T keeps the rest.
+// req: 2T + 2 bounds, dep: 2T + 1 depths.
+__global__ __launch_bounds__(256) void k_kw_reqs(const uint64_t* __restrict__ tie,
+                                                const uint64_t* __restrict__ ntie, uint64_t n,
+                                                uint64_t* __restrict__ req,
+                                                uint32_t* __restrict__ dep) {
+  const uint64_t nt = *ntie;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += stride) {
+    req[2 * t] = t ? tie[t - 1] >> 32 : 0;
+    dep[2 * t] = kSpanKeep;
+    req[2 * t + 1] = tie[t] & 0xFFFFFFFFull;
+    dep[2 * t + 1] = 8;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    req[2 * nt] = nt ? tie[nt - 1] >> 32 : 0;
+    dep[2 * nt] = kSpanKeep;
+    req[2 * nt + 1] = n;
+  }
+}
+
 // Per-device grow-only scratch and a pinned word for read-backs; calls that
 // use it hold its mutex for their whole duration.
 struct Scratch {
@@ -1454,7 +1477,7 @@ int kway_by_keys(const unsigned char* d_text, uint64_t text_bytes, const std::ve
   std::lock_guard<std::mutex> lk(sc->mu);
   const size_t b_bsum = al256(8 * (nblk + 2)), b_ssum = al256(8 * (kKwSamples / 1024 + 2));
   const size_t b_gid = al256(4 * (n + 1)), b_rh = al256(4ull * (k + 1));
-  const size_t b_kv = al256(8 * (n + 1)), b_split = al256(8 * max_splits);
+  const size_t b_kv = al256(8 * (n + 2)), b_split = al256(8 * max_splits);
   const size_t b_first = al256(4 * (nexp + 1));
   rc = scratch_reserve(*sc, b_bsum + b_ssum + 256 + b_gid + b_rh + 4 * b_kv + b_split + b_kv +
                                 b_first,
@@ -1591,45 +1614,43 @@ int kway_by_keys(const unsigned char* d_text, uint64_t text_bytes, const std::ve
                                                                            tpos);
   k_kw_ties<<<grid_cap(ngroups, 256), 256, 0, st>>>(mkey, ngroups, tflag, tpos, pos, tie);
   PL_HIP(hipGetLastError());
-  const uint64_t first_n = std::min<uint64_t>(kKwTieFirst, n / 2 + 1);
-  if ((rc = hlist_reserve(*sc, 8 * first_n))) return rc;
+  // 5. The span sort's request table, built on the device (the spans
+  //    between ties kept as they are, every tie sorted in place from byte
+  //    8) and read back with the tie count (a second copy when there are
+  //    more than kKwTieFirst ties).
+  uint64_t* dreq = val[cb ^ 1];
+  uint32_t* ddep = hgid;
+  k_kw_reqs<<<grid_cap(ngroups / 2 + 1, 256), 256, 0, st>>>(tie, d_total, n, dreq, ddep);
+  PL_HIP(hipGetLastError());
+  auto table_bytes = [](uint64_t t) { return al256(8 * (2 * t + 2)) + 4 * (2 * t + 1); };
+  const uint64_t first_n = std::min<uint64_t>(kKwTieFirst, ngroups / 2);
+  if ((rc = hlist_reserve(*sc, table_bytes(first_n)))) return rc;
   PL_HIP(hipMemcpyAsync(sc->host, d_total, 8, hipMemcpyDeviceToHost, st));
-  PL_HIP(hipMemcpyAsync(sc->hlist, tie, 8 * first_n, hipMemcpyDeviceToHost, st));
+  PL_HIP(hipMemcpyAsync(sc->hlist, dreq, 8 * (2 * first_n + 2), hipMemcpyDeviceToHost, st));
+  PL_HIP(hipMemcpyAsync(reinterpret_cast<char*>(sc->hlist) + al256(8 * (2 * first_n + 2)), ddep,
+                        4 * (2 * first_n + 1), hipMemcpyDeviceToHost, st));
   if ((rc = wait_stream(st))) return rc;
   const uint64_t nties = sc->host[0];
-  if (nties > n / 2) return fail(GSS_E_INTERNAL, "k-way merge: bad tie count");
+  if (nties > ngroups / 2) return fail(GSS_E_INTERNAL, "k-way merge: bad tie count");
+  if (nties == 0) return wait_stream(st);
+  uint64_t m = first_n;  // ties whose requests the pinned table holds
   if (nties > first_n) {
-    if ((rc = hlist_reserve(*sc, 8 * nties))) return rc;
-    PL_HIP(hipMemcpyAsync(sc->hlist, tie, 8 * nties, hipMemcpyDeviceToHost, st));
+    m = nties;
+    if ((rc = hlist_reserve(*sc, table_bytes(m)))) return rc;
+    PL_HIP(hipMemcpyAsync(sc->hlist, dreq, 8 * (2 * m + 2), hipMemcpyDeviceToHost, st));
+    PL_HIP(hipMemcpyAsync(reinterpret_cast<char*>(sc->hlist) + al256(8 * (2 * m + 2)), ddep,
+                          4 * (2 * m + 1), hipMemcpyDeviceToHost, st));
     if ((rc = wait_stream(st))) return rc;
   }
-  const uint64_t* ht = sc->hlist;
-  if (nties) {
-    // 5. Every tie's span sorted in place from byte 8; the spans between
-    //    them stay as they are.
-    std::vector<uint64_t> req;
-    std::vector<uint32_t> dep;
-    uint64_t done = 0;
-    for (uint64_t t = 0; t < nties; ++t) {
-      const uint64_t s = ht[t] & 0xFFFFFFFFull, e = ht[t] >> 32;
-      if (s < done || e <= s + 1 || e > n) return fail(GSS_E_INTERNAL, "k-way merge: bad tie span");
-      if (s > done) {
-        req.push_back(done);
-        dep.push_back(kSpanKeep);
-      }
-      req.push_back(s);
-      dep.push_back(8);
-      done = e;
-    }
-    if (done < n) {
-      req.push_back(done);
-      dep.push_back(kSpanKeep);
-    }
-    req.push_back(n);
-    rc = sort_spans_device32(d_text, text_bytes, d_out, d_out_lcp, req.data(), dep.data(),
-                             (uint32_t)dep.size(), st);
-    if (rc) return rc;
-  }
+  const uint64_t* hreq = sc->hlist;
+  const uint32_t* hdep =
+      reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(sc->hlist) + al256(8 * (2 * m + 2)));
+  // The table covers [0, n) with 2 * nties + 1 requests (the span sort
+  // checks that the bounds ascend from 0).
+  if (hreq[2 * nties + 1] != n) return fail(GSS_E_INTERNAL, "k-way merge: bad tie spans");
+  rc = sort_spans_device32(d_text, text_bytes, d_out, d_out_lcp, hreq, hdep,
+                           (uint32_t)(2 * nties + 1), st);
+  if (rc) return rc;
   return wait_stream(st);
 }
 
