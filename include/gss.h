@@ -353,6 +353,10 @@ typedef struct gss_stats {
 					 round reads 32 bytes: up to 4 levels);
 					 counted only under GSS_PROFILE_LEVELS,
 					 else 0 */
+	uint32_t first_pass_fused; /* 1: the first global pass's digit counts
+				      came from its key gather (its digit was
+				      formed as in the last sort on the same
+				      context), 0: a separate count */
 } gss_stats;
 
 int gss_get_stats(gss_stats *out);            /* last sort on this thread */

@@ -108,6 +108,8 @@ struct Counters {
   uint32_t ntiles;       // tiles of the current pass
   uint32_t bound_done;   // boundaries whose LCPs earlier passes wrote
   uint32_t active_cur;   // strings of the current pass's segments
+  uint32_t hist_done;    // the gather counted the pass's digits (set by k_layout)
+  uint32_t hist_fused;   // the first pass's digits were counted by its gather
 };
 static_assert(sizeof(Counters) % 8 == 0, "Counters is copied in 8-byte words");
 static_assert(offsetof(Counters, big_next) % 8 == 0 && offsetof(Counters, boundary) % 8 == 0 &&
@@ -143,6 +145,9 @@ constexpr uint32_t kLocalClassCap[3] = {128, 1024, 6144};
 constexpr uint32_t kLocalClassThreads[3] = {64, 256, 1024};
 
 // Pointers into the workspace for one sort.
+// Words of Plan::guess: the alphabet (8), the first differing key byte, valid.
+constexpr int kGuessWords = 10;
+
 struct Plan {
   const uint64_t* text64;
   uint64_t text_bytes;   // text length: bytes at or beyond it read as 0 (>= 1)
@@ -180,6 +185,13 @@ struct Plan {
   uint32_t key_bytes;    // first-round window of the largest local class (32 or 64)
   uint32_t cap_big, cap_retire, cap_bounds, cap_tiles;
   uint32_t cap_local[3];
+  // A sort whose first pass has one segment: the gather also counts its
+  // tiles' digits as the last such sort's digit would be formed (`guess`:
+  // that pass's alphabet, 8 words, its first differing key byte and a valid
+  // flag; k_layout checks the guess against the pass's own alphabet and
+  // min/max, sets hist_done on a match and stores the new guess).
+  uint32_t* guess;
+  uint32_t fuse_hist;
 };
 
 // ---------------------------------------------------------------------------
@@ -458,7 +470,7 @@ __device__ __forceinline__ uint32_t block_max(uint32_t v, uint32_t* scratch) {
 hipError_t launch_plan(const Plan& p, const BigSeg* segs, uint32_t pass, uint32_t grid_tiles,
                        uint32_t grid_segs, hipStream_t s);
 hipError_t launch_gather_minmax(const Plan& p, const BigSeg* segs, uint32_t grid,
-                                hipStream_t s);
+                                uint32_t radix_bits, hipStream_t s);
 hipError_t launch_segreduce(const Plan& p, uint32_t grid, hipStream_t s);
 hipError_t launch_layout(const Plan& p, uint32_t radix_bits, hipStream_t s);
 hipError_t launch_histogram(const Plan& p, uint32_t grid, uint32_t radix_bits, hipStream_t s);

@@ -243,10 +243,11 @@ __device__ __forceinline__ uint32_t build_rank(const uint32_t* __restrict__ alph
   return total;
 }
 
+// The digit of a segment whose keys first differ at byte fb0.
 template <int RB>
-__device__ __forceinline__ DigitSpec digit_spec(uint64_t mn, uint64_t mx, uint32_t A) {
+__device__ __forceinline__ DigitSpec digit_spec_at(uint32_t fb0, uint32_t A) {
   DigitSpec ds;
-  ds.fb0 = (uint32_t)__builtin_clzll(mn ^ mx) >> 3;
+  ds.fb0 = fb0;
   ds.A = A < 2 ? 2 : A;
   uint32_t k = 1, v = ds.A;
   while (k < 8 && (uint64_t)v * ds.A <= (1ull << RB)) {
@@ -258,6 +259,11 @@ __device__ __forceinline__ DigitSpec digit_spec(uint64_t mn, uint64_t mx, uint32
   for (uint32_t c = 0; c < ds.k; ++c) nbu *= ds.A;
   ds.nbu = nbu;
   return ds;
+}
+
+template <int RB>
+__device__ __forceinline__ DigitSpec digit_spec(uint64_t mn, uint64_t mx, uint32_t A) {
+  return digit_spec_at<RB>((uint32_t)__builtin_clzll(mn ^ mx) >> 3, A);
 }
 
 __device__ __forceinline__ uint32_t digit_of(uint64_t key, const DigitSpec& ds,
@@ -287,13 +293,19 @@ __device__ __forceinline__ uint64_t match_digit_n(uint32_t d, uint64_t live, uin
 // ---------------------------------------------------------------------------
 // Global pass, step 1: gather keys; per tile the min/max key and the set of
 // byte values seen (no atomics: thousands of tiles would serialise on one
-// segment's words).
+// segment's words).  A first pass over one segment (fuse_hist) also counts
+// the tile's digits as the guess forms them, into the segment's count
+// column (its counts start the flat array); k_layout tells k_histogram
+// whether they stand.
 
-template <int TI>
+template <int RB, int TI>
 __device__ __forceinline__ void gather_tile(Plan& p, const BigSeg* segs, uint32_t b,
                                             uint32_t nt) {
+  constexpr uint32_t NB = 1u << RB;
   __shared__ uint64_t s_mn[kTileThreads / kWave], s_mx[kTileThreads / kWave];
   __shared__ uint8_t s_present[256];
+  __shared__ uint32_t s_h[NB];
+  __shared__ uint8_t s_rank[256];
   if (threadIdx.x < 256) s_present[threadIdx.x] = 0;
   const uint32_t ti = xcd_tile(b, nt);
   const Tile t = p.tiles[ti];
@@ -335,7 +347,20 @@ __device__ __forceinline__ void gather_tile(Plan& p, const BigSeg* segs, uint32_
       }
     }
   }
-  __syncthreads();  // s_present cleared
+  // The guessed digit (fuse_hist is a kernel argument: uniform).
+  bool fuse = false;
+  DigitSpec ds{};
+  if (p.fuse_hist && !p.keys_carried) {
+    uint32_t g[kGuessWords];
+#pragma unroll
+    for (int w = 0; w < kGuessWords; ++w) g[w] = p.guess[w];
+    fuse = g[9] != 0;
+    if (fuse) {
+      ds = digit_spec_at<RB>(g[8], build_rank(g, s_rank));
+      for (uint32_t d = threadIdx.x; d < ds.nbu; d += kTileThreads) s_h[d] = 0;
+    }
+  }
+  __syncthreads();  // s_present, s_h cleared; s_rank ready
 #pragma unroll
   for (int j = 0; j < TI; ++j) {
     const uint32_t i = j * kTileThreads + threadIdx.x;
@@ -344,6 +369,12 @@ __device__ __forceinline__ void gather_tile(Plan& p, const BigSeg* segs, uint32_
       mx = k[j] > mx ? k[j] : mx;
 #pragma unroll
       for (int b = 0; b < 8; ++b) s_present[(uint32_t)(k[j] >> (8 * b)) & 0xFFu] = 1;
+      if (fuse) {
+        // A byte outside the guessed alphabet ranks A and can form a digit
+        // >= nbu (the guess then fails its check): not counted.
+        const uint32_t d = digit_of(k[j], ds, s_rank);
+        if (d < ds.nbu) atomicAdd(&s_h[d], 1u);
+      }
     }
   }
   mn = wave_min64(mn);
@@ -371,16 +402,21 @@ __device__ __forceinline__ void gather_tile(Plan& p, const BigSeg* segs, uint32_
     p.tile_mm[2ull * ti] = mn;
     p.tile_mm[2ull * ti + 1] = mx;
   }
+  if (fuse) {  // the only segment's counts start at flat entry 0
+    const uint32_t ntiles = p.aux[t.seg].ntiles;
+    uint32_t* col = p.flat + t.tidx;
+    for (uint32_t d = threadIdx.x; d < ds.nbu; d += kTileThreads) col[(uint64_t)d * ntiles] = s_h[d];
+  }
 }
 
 // One tile per block; a launch may hold more blocks than the pass has tiles
 // (an upper bound), the rest exit at once.  (A loop over tiles would let one
 // launch take any count, but the compiler then keeps the per-thread indices
 // of every item live across it: far more registers, lower occupancy.)
-template <int TI>
+template <int RB, int TI>
 __global__ __launch_bounds__(kTileThreads) void k_gather_minmax(Plan p, const BigSeg* segs) {
   const uint32_t nt = p.ctr->ntiles;
-  if (blockIdx.x < nt) gather_tile<TI>(p, segs, blockIdx.x, nt);
+  if (blockIdx.x < nt) gather_tile<RB, TI>(p, segs, blockIdx.x, nt);
 }
 
 // One workgroup per segment: min/max key and byte set over its tiles.  A
@@ -468,6 +504,24 @@ __global__ __launch_bounds__(1024) void k_layout(Plan p) {
     carry += total;
   }
   if (threadIdx.x == 0) {
+    // The gather's counts stand when this pass's one segment forms its
+    // digit as guessed: the same alphabet and first differing byte.  The
+    // pass's own values become the next guess.
+    uint32_t hit = 0;
+    if (p.fuse_hist && !p.keys_carried && nsegs == 1) {
+      const uint64_t mn = p.segmm[0], mx = p.segmm[1];
+      const uint32_t fb0 = mn == mx ? 0u : (uint32_t)__builtin_clzll(mn ^ mx) >> 3;
+      hit = p.guess[9] != 0 && mn != mx && p.guess[8] == fb0;
+#pragma unroll
+      for (int w = 0; w < 8; ++w) {
+        hit &= p.guess[w] == s_al[w];
+        p.guess[w] = s_al[w];
+      }
+      p.guess[8] = fb0;
+      p.guess[9] = mn != mx;
+      p.ctr->hist_fused = hit;
+    }
+    p.ctr->hist_done = hit;
     p.ctr->flat_len = carry;
     p.ctr->big_next = 0;  // the coming segscan counts this pass's big children
     p.ctr->big_next_strings = 0;
@@ -514,6 +568,7 @@ __device__ __forceinline__ void histogram_tile(Plan& p, uint32_t b, uint32_t nt)
 
 template <int RB, int TI>
 __global__ __launch_bounds__(kTileThreads) void k_histogram(Plan p) {
+  if (p.fuse_hist && p.ctr->hist_done) return;  // the gather counted them
   const uint32_t nt = p.ctr->ntiles;
   if (blockIdx.x < nt) histogram_tile<RB, TI>(p, blockIdx.x, nt);
 }
@@ -1082,11 +1137,15 @@ hipError_t launch_segreduce(const Plan& p, uint32_t grid, hipStream_t s) {
 }
 
 hipError_t launch_gather_minmax(const Plan& p, const BigSeg* segs, uint32_t grid,
-                                hipStream_t s) {
+                                uint32_t radix_bits, hipStream_t s) {
   if (!grid) return hipSuccess;
-  switch (p.tile) {
-    case 2048: k_gather_minmax<4><<<grid, kTileThreads, 0, s>>>(p, segs); break;
-    case 4096: k_gather_minmax<8><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+  if (p.tile != 2048 && p.tile != 4096) return hipErrorInvalidValue;
+  const int v = (radix_bits == 11 ? 2 : (radix_bits == 8 ? 0 : 4)) + (p.tile == 4096 ? 1 : 0);
+  switch (v) {
+    case 0: k_gather_minmax<8, 4><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+    case 1: k_gather_minmax<8, 8><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+    case 2: k_gather_minmax<11, 4><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+    case 3: k_gather_minmax<11, 8><<<grid, kTileThreads, 0, s>>>(p, segs); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -303,6 +303,7 @@ std::vector<Item> workspace_items(Plan& p, uint32_t** idx_copy, const Caps& k, u
       {(void**)&p.tile_mm, 16 * k.tiles},
       {(void**)&p.tile_alpha, 32 * k.tiles},
       {(void**)&p.seg_alpha, 32 * k.big},
+      {(void**)&p.guess, 4 * kGuessWords},
   };
 }
 
@@ -340,6 +341,11 @@ int ensure_workspace(Context& c, uint64_t n, uint32_t rb, uint32_t t, uint32_t t
     off += align_up(it.bytes, 256);
   }
   p.pass_active = reinterpret_cast<unsigned long long*>(p.lstats + 2);
+  if (hipMemset(p.guess, 0, 4 * kGuessWords) != hipSuccess) {  // no guess yet
+    (void)hipGetLastError();
+    (void)hipFree(base);
+    return fail(GSS_E_INTERNAL, "device workspace initialisation failed");
+  }
   w.base = base;
   w.bytes = total;
   w.n_cap = ncap;
@@ -653,7 +659,10 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     // they lie.
     p.keys_carried = pass % 2;
     if (p.keys_carried) std::swap(p.keys, p.keys_next);
-    GSS_LAUNCH("gather", 0, launch_gather_minmax(p, segs, grid_tiles, st));
+    // A first pass over one segment counts its digits in the gather when
+    // they are formed as in the last such sort (checked on the device).
+    p.fuse_hist = pass == 0 && nbig == 1;
+    GSS_LAUNCH("gather", 0, launch_gather_minmax(p, segs, grid_tiles, rb, st));
     GSS_LAUNCH("segreduce", 0, launch_segreduce(p, grid_segs, st));
     GSS_LAUNCH("layout", 0, launch_layout(p, rb, st));
     GSS_LAUNCH("histogram", 0, launch_histogram(p, grid_tiles, rb, st));
@@ -661,6 +670,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     GSS_LAUNCH("segscan", 0, launch_segscan(p, segs, next, grid_segs, rb, st));
     GSS_LAUNCH("scatter", 0, launch_scatter(p, segs, grid_tiles, rb, st));
     if (lcp) GSS_LAUNCH("boundary", 0, launch_boundary_lcp(p, st));
+    p.fuse_hist = 0;
     prof_end(c, pass_tok, st);
     cur ^= 1;
     ++pass;
@@ -765,6 +775,7 @@ int run_sort(Context& c, const unsigned char* text, uint64_t text_bytes, const u
     if (act) passes = i + 1;
   }
   stats.global_passes = passes;
+  stats.first_pass_fused = w.h_ctr->hist_fused;
   if (nbig) {
     c.pass_hint = std::max<uint32_t>(1, passes);
     c.hint_strings = pass0_strings;

@@ -45,6 +45,7 @@ class Stats(C.Structure):
         ("local_rounds_max", C.c_uint32),
         ("local_round_strings", C.c_uint64),
         ("local_level_strings", C.c_uint64),
+        ("first_pass_fused", C.c_uint32),
     ]
 
 
