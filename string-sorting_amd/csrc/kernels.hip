@@ -1048,19 +1048,44 @@ __device__ __forceinline__ void scatter_tile(Plan& p, const BigSeg* segs, uint32
     run += tot[q];
   }
   __syncthreads();
+  // The next pass's keys follow the offsets through LDS, in the space of the
+  // wave counters once every item has its place (it holds a tile's keys at
+  // 11-bit digits), and leave in bucket order: runs of consecutive
+  // positions instead of one scattered 8-byte store per string.
+  constexpr bool kStageKeys = sizeof(s_whist) >= sizeof(uint64_t) * kTileThreads * TI;
+  uint64_t* const s_key = reinterpret_cast<uint64_t*>(&s_whist[0][0]);
+  if (kStageKeys && carry) {
+    uint32_t lpos[TI];
 #pragma unroll
-  for (int j = 0; j < TI; ++j) {
-    const uint32_t i = wid * IPW + j * kWave + lane;
-    if (i < t.len) {
+    for (int j = 0; j < TI; ++j) {
       const uint32_t d = dig[j];
-      const uint32_t within = s_whist[d][wid] + rnk[j];  // place among the tile's digit d
-      const uint32_t lp = s_tstart[d] + within;
-      s_off[lp] = off[j];
-      s_dig[lp] = (uint16_t)d;
-      // Keys for the next pass go straight to the strings' new positions
-      // (only for buckets that take one).
-      const uint32_t pos = s_gbase[d] + within;
-      if (carry && s_kind[d] == kBucketNext && pos - sg.begin < sg.size) p.keys_next[pos] = nk[j];
+      lpos[j] = s_tstart[d] + s_whist[d][wid] + rnk[j];  // the item's place in the tile
+    }
+    __syncthreads();  // the wave counters are read: their space takes the keys
+#pragma unroll
+    for (int j = 0; j < TI; ++j) {
+      const uint32_t i = wid * IPW + j * kWave + lane;
+      if (i < t.len) {
+        s_off[lpos[j]] = off[j];
+        s_dig[lpos[j]] = (uint16_t)dig[j];
+        s_key[lpos[j]] = nk[j];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < TI; ++j) {
+      const uint32_t i = wid * IPW + j * kWave + lane;
+      if (i < t.len) {
+        const uint32_t d = dig[j];
+        const uint32_t within = s_whist[d][wid] + rnk[j];  // place among the tile's digit d
+        const uint32_t lp = s_tstart[d] + within;
+        s_off[lp] = off[j];
+        s_dig[lp] = (uint16_t)d;
+        // Keys for the next pass go straight to the strings' new positions
+        // (only for buckets that take one).
+        const uint32_t pos = s_gbase[d] + within;
+        if (carry && s_kind[d] == kBucketNext && pos - sg.begin < sg.size) p.keys_next[pos] = nk[j];
+      }
     }
   }
   __syncthreads();
@@ -1073,7 +1098,9 @@ __device__ __forceinline__ void scatter_tile(Plan& p, const BigSeg* segs, uint32
       continue;
     }
     dst[pos] = o;
-    if (s_kind[d] == kBucketFinal) p.out[pos] = o;
+    const uint32_t kind = s_kind[d];
+    if (kind == kBucketFinal) p.out[pos] = o;
+    if (kStageKeys && carry && kind == kBucketNext) p.keys_next[pos] = s_key[i];
   }
 }
 
