@@ -909,7 +909,7 @@ __global__ __launch_bounds__(BS) void k_segscan(Plan p, const BigSeg* segs, BigS
 // among the tile's strings of its digit comes from wave match-ranking; the
 // scanned flat array gives where the tile's strings of each digit start.
 
-template <int RB, int TI>
+template <int RB, int TI, bool CARRY>
 __device__ __forceinline__ void scatter_tile(Plan& p, const BigSeg* segs, uint32_t b,
                                              uint32_t nt) {
   constexpr uint32_t NB = 1u << RB;
@@ -920,13 +920,24 @@ __device__ __forceinline__ void scatter_tile(Plan& p, const BigSeg* segs, uint32
   __shared__ uint32_t s_tstart[NB];
   __shared__ uint32_t s_gbase[NB];
   __shared__ uint8_t s_kind[NB];  // kBucket* kind of each digit's bucket
-  __shared__ uint32_t s_off[kTileThreads * TI];
-  __shared__ uint16_t s_dig[kTileThreads * TI];
+  // Once every item has its place, the wave counters' space is free: a pass
+  // that hands on keys stages them there (a tile's keys fill it at 11-bit
+  // digits); a carried pass stages its offsets and digits there instead,
+  // which leaves the block about 52 KB of LDS and three blocks per CU.
+  constexpr uint32_t kItems = kTileThreads * TI;
+  constexpr bool kStageKeys = CARRY && NB * W * 2 >= 8 * kItems;
+  constexpr bool kAliasOff = !CARRY && NB * W * 2 >= 6 * kItems;
+  __shared__ uint32_t s_off_own[kAliasOff ? 1 : kItems];
+  __shared__ uint16_t s_dig_own[kAliasOff ? 1 : kItems];
   __shared__ uint32_t s_scan[W + 1];
   __shared__ uint8_t rank[256];
 
   const Tile t = p.tiles[xcd_tile(b, nt)];
-  const bool carry = !p.keys_carried;  // hand the next pass its keys
+  constexpr bool carry = CARRY;  // hand the next pass its keys (a text-reading pass)
+  char* const whist_bytes = reinterpret_cast<char*>(&s_whist[0][0]);
+  uint32_t* const s_off = kAliasOff ? reinterpret_cast<uint32_t*>(whist_bytes) : s_off_own;
+  uint16_t* const s_dig =
+      kAliasOff ? reinterpret_cast<uint16_t*>(whist_bytes + 4 * kItems) : s_dig_own;
   const int wid = threadIdx.x / kWave;
   const int lane = lane_id();
   // Everything that needs only the tile is loaded before the segment's
@@ -1049,26 +1060,25 @@ __device__ __forceinline__ void scatter_tile(Plan& p, const BigSeg* segs, uint32
   }
   __syncthreads();
   // The next pass's keys follow the offsets through LDS, in the space of the
-  // wave counters once every item has its place (it holds a tile's keys at
-  // 11-bit digits), and leave in bucket order: runs of consecutive
-  // positions instead of one scattered 8-byte store per string.
-  constexpr bool kStageKeys = sizeof(s_whist) >= sizeof(uint64_t) * kTileThreads * TI;
-  uint64_t* const s_key = reinterpret_cast<uint64_t*>(&s_whist[0][0]);
-  if (kStageKeys && carry) {
+  // wave counters once every item has its place, and leave in bucket order:
+  // runs of consecutive positions instead of one scattered 8-byte store per
+  // string.
+  uint64_t* const s_key = reinterpret_cast<uint64_t*>(whist_bytes);
+  if (kStageKeys || kAliasOff) {
     uint32_t lpos[TI];
 #pragma unroll
     for (int j = 0; j < TI; ++j) {
       const uint32_t d = dig[j];
       lpos[j] = s_tstart[d] + s_whist[d][wid] + rnk[j];  // the item's place in the tile
     }
-    __syncthreads();  // the wave counters are read: their space takes the keys
+    __syncthreads();  // the wave counters are read: their space is reused
 #pragma unroll
     for (int j = 0; j < TI; ++j) {
       const uint32_t i = wid * IPW + j * kWave + lane;
       if (i < t.len) {
         s_off[lpos[j]] = off[j];
         s_dig[lpos[j]] = (uint16_t)dig[j];
-        s_key[lpos[j]] = nk[j];
+        if (kStageKeys) s_key[lpos[j]] = nk[j];
       }
     }
   } else {
@@ -1100,14 +1110,14 @@ __device__ __forceinline__ void scatter_tile(Plan& p, const BigSeg* segs, uint32
     dst[pos] = o;
     const uint32_t kind = s_kind[d];
     if (kind == kBucketFinal) p.out[pos] = o;
-    if (kStageKeys && carry && kind == kBucketNext) p.keys_next[pos] = s_key[i];
+    if (kStageKeys && kind == kBucketNext) p.keys_next[pos] = s_key[i];
   }
 }
 
-template <int RB, int TI>
+template <int RB, int TI, bool CARRY>
 __global__ __launch_bounds__(kTileThreads) void k_scatter(Plan p, const BigSeg* segs) {
   const uint32_t nt = p.ctr->ntiles;
-  if (blockIdx.x < nt) scatter_tile<RB, TI>(p, segs, blockIdx.x, nt);
+  if (blockIdx.x < nt) scatter_tile<RB, TI, CARRY>(p, segs, blockIdx.x, nt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1242,11 +1252,16 @@ hipError_t launch_scatter(const Plan& p, const BigSeg* segs, uint32_t grid,
   if (!grid) return hipSuccess;
   const int v = (radix_bits == 11 ? 2 : (radix_bits == 8 ? 0 : 4)) + (p.tile == 4096 ? 1 : 0);
   if (p.tile != 2048 && p.tile != 4096) return hipErrorInvalidValue;
-  switch (v) {
-    case 0: k_scatter<8, 4><<<grid, kTileThreads, 0, s>>>(p, segs); break;
-    case 1: k_scatter<8, 8><<<grid, kTileThreads, 0, s>>>(p, segs); break;
-    case 2: k_scatter<11, 4><<<grid, kTileThreads, 0, s>>>(p, segs); break;
-    case 3: k_scatter<11, 8><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+  const bool carry = !p.keys_carried;  // a text-reading pass hands on keys
+  switch (v + (carry ? 8 : 0)) {
+    case 0: k_scatter<8, 4, false><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+    case 1: k_scatter<8, 8, false><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+    case 2: k_scatter<11, 4, false><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+    case 3: k_scatter<11, 8, false><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+    case 8: k_scatter<8, 4, true><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+    case 9: k_scatter<8, 8, true><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+    case 10: k_scatter<11, 4, true><<<grid, kTileThreads, 0, s>>>(p, segs); break;
+    case 11: k_scatter<11, 8, true><<<grid, kTileThreads, 0, s>>>(p, segs); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
