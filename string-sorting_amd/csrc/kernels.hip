@@ -847,19 +847,22 @@ __device__ __forceinline__ void segscan_seg(Plan& p, const BigSeg* segs, BigSeg*
   uint32_t ol[3] = {sc[3], sc[4], sc[5]};
   const uint32_t tb = tot[0], tr = tot[1], tg = tot[2], t0 = tot[3], t1 = tot[4], t2 = tot[5];
   const uint32_t ts0 = tot[6], ts1 = tot[7], ts2 = tot[8], tgs = tot[9];
-  if (threadIdx.x == 0) {
-    const uint32_t tl[3] = {t0, t1, t2}, ts[3] = {ts0, ts1, ts2};
-    const uint64_t br = (tb || tr) ? atomic_add_pair(&p.ctr->boundary, tb, tr) : 0ull;
-    s_base[0] = (uint32_t)br;
-    s_base[1] = (uint32_t)(br >> 32);
-    s_base[5] = tg ? (uint32_t)atomic_add_pair(&p.ctr->big_next, tg, tgs) : 0u;
-    for (int c = 0; c < 3; ++c)
-      s_base[2 + c] = tl[c] ? (uint32_t)atomic_add_pair(&p.ctr->local[c].segs, tl[c], ts[c]) : 0u;
-    if (s_base[0] + tb > p.cap_bounds) atomicOr(&p.ctr->error, 16u);
-    if (s_base[1] + tr > p.cap_retire) atomicOr(&p.ctr->error, 8u);
-    if (s_base[5] + tg > p.cap_big) atomicOr(&p.ctr->error, 4u);
-    for (int c = 0; c < 3; ++c)
-      if (s_base[2 + c] + tl[c] > p.cap_local[c]) atomicOr(&p.ctr->error, 2u);
+  // The five reservations are independent: lanes 0-4 of the first wave
+  // make them with one atomic instruction (one trip to L2, not five).
+  if (threadIdx.x < 5) {
+    const uint32_t r = threadIdx.x;  // 0 boundaries + retire chunks, 1 next pass, 2-4 local class
+    uint32_t* const at = r == 0 ? &p.ctr->boundary
+                                : (r == 1 ? &p.ctr->big_next : &p.ctr->local[r - 2].segs);
+    const uint32_t a = r == 0 ? tb : (r == 1 ? tg : (r == 2 ? t0 : (r == 3 ? t1 : t2)));
+    const uint32_t b = r == 0 ? tr : (r == 1 ? tgs : (r == 2 ? ts0 : (r == 3 ? ts1 : ts2)));
+    const uint64_t old = (a || b) ? atomic_add_pair(at, a, b) : 0ull;
+    const uint32_t lo = (uint32_t)old, hi = (uint32_t)(old >> 32);
+    const uint32_t cap = r == 0 ? p.cap_bounds : (r == 1 ? p.cap_big : p.cap_local[r - 2]);
+    uint32_t err = lo + a > cap ? (r == 0 ? 16u : (r == 1 ? 4u : 2u)) : 0u;
+    if (r == 0 && hi + b > p.cap_retire) err |= 8u;
+    s_base[r == 0 ? 0 : (r == 1 ? 5 : r)] = lo;
+    if (r == 0) s_base[1] = hi;
+    if (err) atomicOr(&p.ctr->error, err);
   }
   __syncthreads();
   ob += s_base[0];
