@@ -169,7 +169,8 @@ struct Plan {
   uint32_t* alpha;       // 256-bit set of byte values seen in the pass's keys
   Tile* tiles;
   SegAux* aux;           // per BigSeg: layout of its counts in `flat`
-  uint32_t* flat;        // digit-major per-tile counts, scanned in place (+1 total)
+  uint16_t* cnt;         // digit-major per-tile counts (a tile's count fits 16 bits)
+  uint32_t* flat;        // their exclusive scan, in the same places (+1 total)
   uint32_t* flat_blk;    // block sums of the flat scan
   LocalSeg* local[3];
   Retire* retire;

@@ -404,8 +404,9 @@ __device__ __forceinline__ void gather_tile(Plan& p, const BigSeg* segs, uint32_
   }
   if (fuse) {  // the only segment's counts start at flat entry 0
     const uint32_t ntiles = p.aux[t.seg].ntiles;
-    uint32_t* col = p.flat + t.tidx;
-    for (uint32_t d = threadIdx.x; d < ds.nbu; d += kTileThreads) col[(uint64_t)d * ntiles] = s_h[d];
+    uint16_t* col = p.cnt + t.tidx;
+    for (uint32_t d = threadIdx.x; d < ds.nbu; d += kTileThreads)
+      col[(uint64_t)d * ntiles] = (uint16_t)s_h[d];
   }
 }
 
@@ -562,8 +563,9 @@ __device__ __forceinline__ void histogram_tile(Plan& p, uint32_t b, uint32_t nt)
     if (i < t.len) atomicAdd(&h[digit_of(key[j], ds, rank)], 1u);
   }
   __syncthreads();
-  uint32_t* col = p.flat + ax.flat_base + t.tidx;
-  for (uint32_t d = threadIdx.x; d < ds.nbu; d += kTileThreads) col[(uint64_t)d * ax.ntiles] = h[d];
+  uint16_t* col = p.cnt + ax.flat_base + t.tidx;
+  for (uint32_t d = threadIdx.x; d < ds.nbu; d += kTileThreads)
+    col[(uint64_t)d * ax.ntiles] = (uint16_t)h[d];
 }
 
 template <int RB, int TI>
@@ -574,12 +576,15 @@ __global__ __launch_bounds__(kTileThreads) void k_histogram(Plan p) {
 }
 
 // ---------------------------------------------------------------------------
-// Global pass, step 4: exclusive scan of the flat count array in place
-// (block sums, scan of the block sums, block-local scans plus offsets).
-// flat[len] receives the total.
+// Global pass, step 4: exclusive scan of the 16-bit count array into the
+// 32-bit flat array (block sums, their scan, block-local scans plus
+// offsets).  flat[len] receives the total.
 
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = kScanBlock / kScanThreads;
+
+// The two 16-bit counts of a 32-bit word, added.
+__device__ __forceinline__ uint32_t count_pair_sum(uint32_t w) { return (w & 0xFFFFu) + (w >> 16); }
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(Plan p) {
   __shared__ uint32_t s_scan[kScanThreads / kWave + 1];
@@ -587,11 +592,21 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(Plan p) {
   const uint32_t blk = blockIdx.x;
   const uint64_t b0 = (uint64_t)blk * kScanBlock;
   if (b0 >= len) return;
+  static_assert(kScanItems % 8 == 0, "whole 16-byte groups of counts per thread");
   uint32_t sum = 0;
+  if (b0 + kScanBlock <= len) {  // 16-byte loads of eight counts
+    const uint4* __restrict__ g = reinterpret_cast<const uint4*>(p.cnt + b0);
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
-    const uint64_t i = b0 + j * kScanThreads + threadIdx.x;
-    if (i < len) sum += p.flat[i];
+    for (int r = 0; r < kScanItems / 8; ++r) {
+      const uint4 v = g[r * kScanThreads + threadIdx.x];
+      sum += count_pair_sum(v.x) + count_pair_sum(v.y) + count_pair_sum(v.z) +
+             count_pair_sum(v.w);
+    }
+  } else {
+    for (int j = 0; j < kScanItems; ++j) {
+      const uint64_t i = b0 + j * kScanThreads + threadIdx.x;
+      if (i < len) sum += p.cnt[i];
+    }
   }
   uint32_t total;
   (void)block_excl_scan<kScanThreads>(sum, s_scan, &total);
@@ -623,6 +638,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(Plan p) {
   constexpr int Q = kScanItems / 4;  // 16-byte groups per thread
   __shared__ uint32_t s_scan[kScanThreads / kWave + 1];
   __shared__ __attribute__((aligned(16))) uint32_t s_v[kScanBlock];
+  __shared__ __attribute__((aligned(16))) uint16_t s_c[kScanBlock];
   const uint32_t len = p.ctr->flat_len;
   const uint32_t blk = blockIdx.x;
   const uint64_t b0 = (uint64_t)blk * kScanBlock;
@@ -635,39 +651,47 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(Plan p) {
     (void)block_excl_scan<kScanThreads>(part, s_scan, &before);
     __syncthreads();  // s_scan is reused below
   }
-  // The block's entries go through LDS so that global loads and stores are
-  // 16-byte and coalesced while each thread scans consecutive entries.
+  // The block's counts come in and its scan goes out through LDS so that
+  // global loads and stores are 16-byte and coalesced while each thread
+  // scans consecutive entries.
   const bool whole = b0 + kScanBlock <= len;
   uint4* const v4 = reinterpret_cast<uint4*>(s_v);
+  uint4* const c4 = reinterpret_cast<uint4*>(s_c);
+  constexpr int QC = kScanItems / 8;  // 16-byte groups of counts per thread
   if (whole) {
-    const uint4* __restrict__ g4 = reinterpret_cast<const uint4*>(p.flat + b0);
+    const uint4* __restrict__ g = reinterpret_cast<const uint4*>(p.cnt + b0);
 #pragma unroll
-    for (int r = 0; r < Q; ++r) v4[r * kScanThreads + threadIdx.x] = g4[r * kScanThreads + threadIdx.x];
+    for (int r = 0; r < QC; ++r) c4[r * kScanThreads + threadIdx.x] = g[r * kScanThreads + threadIdx.x];
   } else {
     for (uint32_t i = threadIdx.x; i < (uint32_t)kScanBlock; i += kScanThreads)
-      s_v[i] = b0 + i < len ? p.flat[b0 + i] : 0u;
+      s_c[i] = b0 + i < len ? p.cnt[b0 + i] : (uint16_t)0;
   }
   __syncthreads();
   // Thread t owns entries [t * kScanItems, (t + 1) * kScanItems).
-  uint4 q[Q];
+  uint32_t w[4 * QC];  // two counts each
   uint32_t sum = 0;
 #pragma unroll
-  for (int r = 0; r < Q; ++r) {
-    q[r] = v4[threadIdx.x * Q + r];
-    sum += q[r].x + q[r].y + q[r].z + q[r].w;
+  for (int r = 0; r < QC; ++r) {
+    const uint4 c = c4[threadIdx.x * QC + r];
+    w[4 * r] = c.x;
+    w[4 * r + 1] = c.y;
+    w[4 * r + 2] = c.z;
+    w[4 * r + 3] = c.w;
+    sum += count_pair_sum(c.x) + count_pair_sum(c.y) + count_pair_sum(c.z) + count_pair_sum(c.w);
   }
   uint32_t total;
   uint32_t run = block_excl_scan<kScanThreads>(sum, s_scan, &total) +
                  (SELF_PREFIX ? before : p.flat_blk[blk]);
   if (SELF_PREFIX && b0 + kScanBlock >= len && threadIdx.x == 0) p.flat[len] = before + total;
+  static_assert(Q == 2 * QC, "four scanned entries per two count words");
 #pragma unroll
   for (int r = 0; r < Q; ++r) {
     uint4 o;
     o.x = run;
-    o.y = o.x + q[r].x;
-    o.z = o.y + q[r].y;
-    o.w = o.z + q[r].z;
-    run = o.w + q[r].w;
+    o.y = o.x + (w[2 * r] & 0xFFFFu);
+    o.z = o.y + (w[2 * r] >> 16);
+    o.w = o.z + (w[2 * r + 1] & 0xFFFFu);
+    run = o.w + (w[2 * r + 1] >> 16);
     v4[threadIdx.x * Q + r] = o;
   }
   __syncthreads();

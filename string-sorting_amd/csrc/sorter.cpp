@@ -293,6 +293,7 @@ std::vector<Item> workspace_items(Plan& p, uint32_t** idx_copy, const Caps& k, u
       {(void**)&p.alpha, 32},
       {(void**)&p.aux, sizeof(SegAux) * k.big},
       {(void**)&p.tiles, sizeof(Tile) * k.tiles},
+      {(void**)&p.cnt, 2 * (k.tiles * nb + 8)},
       {(void**)&p.flat, 4 * (k.tiles * nb + 1)},
       {(void**)&p.flat_blk, 4 * ((k.tiles * nb) / kScanBlock + 2)},
       {(void**)&p.local[0], sizeof(LocalSeg) * k.local[0]},
